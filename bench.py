@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the fd23 staggered-grid loop on one MI355X.
+
+A step is one time step of the loop (stress pass with fused C-PML, source
+injection, velocity pass with fused C-PML, receiver gather) over every shot
+of the workload, replayed from HIP graphs.  Prints ONE JSON line.
+
+Default workload: plain P-SV FP32, 2000x1000, dx=5 m, dt=0.3 ms, npml 32,
+seeded 12-layer + GRF model, explosive Ricker 15 Hz at (1000,10), 1000
+receiver pairs at z=5 (vx, vz).  Inputs are resident in HBM before timing.
+
+roofline: algorithmic HBM bytes of the dominant pass (fields read once,
+updated arrays read and written once, material once, psi read+written in
+the C-PML strips) divided by that pass's average duration from HIP events
+on the context stream.  cpu_baseline: the oracle's C restatement of the
+same passes on this host's cores over a bounded number of steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def pass_bytes(wl, ctx_bytes_per_step=None):
+    """Algorithmic bytes of the stress and velocity passes of one step."""
+    from fd23_amd import _lib
+
+    S, N = wl.nshots, float(wl.nx) * wl.nz
+    wf = {_lib.STORE_F32: 4, _lib.STORE_F16: 2, _lib.STORE_BF16: 2, _lib.STORE_F64: 8}[wl.storage]
+    wm = 8 if wl.storage == _lib.STORE_F64 else 4
+    nxp = 2.0 * wl.npml * wl.nz
+    nzp = (2.0 if wl.topabsorb else 1.0) * wl.npml * wl.nx
+    if wl.wave == _lib.WAVE_PSV:
+        stress = S * (8 * wf * N + 4 * wf * (nxp + nzp)) + 3 * wm * N
+        vel = S * (7 * wf * N + 4 * wf * (nxp + nzp)) + 2 * wm * N
+    else:
+        stress = S * (5 * wf * N + 2 * wf * (nxp + nzp)) + 2 * wm * N
+        vel = S * (4 * wf * N + 2 * wf * (nxp + nzp)) + 1 * wm * N
+    return stress, vel
+
+
+def sync():
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+            return
+    except Exception:
+        pass
+
+
+def run_gpu(wl, steps, warmup, autotune=True):
+    ctx = wl.context(max_frames=warmup + steps)
+    tile, tune_ms = (ctx.tile, None)
+    if autotune:
+        tile, tune_ms = ctx.autotune(4)
+    # warm-up: also builds the HIP graphs
+    ctx.run(0, warmup, 1)
+    ctx.sync()
+    sync()
+    t0 = time.perf_counter()
+    ctx.run(warmup, steps, 1)
+    ctx.sync()
+    sync()
+    el = time.perf_counter() - t0
+    s_ms, v_ms = ctx.time_passes(20)
+    seis = ctx.seismograms(0, 0, warmup + steps)
+    finite = bool(np.isfinite(seis).all())
+    ctx.close()
+    return dict(elapsed=el, stress_ms=s_ms, vel_ms=v_ms, tile=tile, tune_ms=tune_ms, finite=finite)
+
+
+def cpu_baseline(wl, budget_s=12.0):
+    """Oracle passes (C, OpenMP) on this host over a bounded sample."""
+    import oracle as O
+    from fd23_amd import _lib
+
+    lib = O.lib()
+    lib.or_num_threads.restype = __import__("ctypes").c_int
+    cores = int(lib.or_num_threads())
+    names = ("BU", "BW", "MU", "MUA", "LAM")
+    mat = dict(zip(names, wl.mat))
+    cp, off = {}, 0
+    for name, n in zip(O.PSV_CPML_X + O.PSV_CPML_Z, [wl.nx] * 4 + [wl.nz] * 4):
+        cp[name] = tuple(wl.packed[off + j * n: off + (j + 1) * n].astype(np.float32) for j in range(3))
+        off += 3 * n
+    assert wl.wave == _lib.WAVE_PSV
+    sim = O.PSV(wl.nx, wl.nz, wl.npml, mat, cp, wl.topabsorb)
+    sim.stress()
+    sim.velocity()
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        sim.stress()
+        sim.inject(0, k, wl.stf.astype(np.float32), wl.M.astype(np.float32), *wl.sources[0])
+        sim.velocity()
+        k += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or k >= 2000:
+            break
+    return dict(value=wl.nx * wl.nz * k / el / 1e9, unit="Gcell/s", cores=cores, kind="port",
+                sample="%d steps of the %dx%d P-SV FP32 loop (oracle C restatement, OpenMP)" % (k, wl.nx, wl.nz))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--workload", default="psv2000", choices=["psv2000", "batched", "f16", "bf16", "sh400"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (shots and requests shard "
+              "across GPUs with no communication); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from fd23_amd import _lib, workloads as W
+
+    if args.workload == "psv2000":
+        wl = W.config_psv_2000x1000()
+    elif args.workload == "batched":
+        wl = W.config_psv_batched(64)
+    elif args.workload == "f16":
+        wl = W.config_psv_precision(_lib.STORE_F16)
+    elif args.workload == "bf16":
+        wl = W.config_psv_precision(_lib.STORE_BF16)
+    else:
+        wl = W.config_sh_400x200()
+    wl.build()
+    r = run_gpu(wl, args.steps, args.warmup, autotune=not args.no_autotune)
+    cells = float(wl.nshots) * wl.nx * wl.nz * args.steps
+    gcell = cells / r["elapsed"] / 1e9
+    ms = r["elapsed"] / args.steps * 1e3
+    sb, vb = pass_bytes(wl)
+    if r["stress_ms"] >= r["vel_ms"]:
+        name, nbytes, kms = "stress", sb, r["stress_ms"]
+    else:
+        name, nbytes, kms = "velocity", vb, r["vel_ms"]
+    achieved = nbytes / (kms * 1e-3) / 1e9
+    step_bytes = sb + vb
+    out = {
+        "metric": "Gcell-updates/s",
+        "value": round(gcell, 3),
+        "unit": "Gcell/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {_lib.STORE_F32: "f32", _lib.STORE_F16: "f32 (f16 storage)",
+                  _lib.STORE_BF16: "f32 (bf16 storage)", _lib.STORE_F64: "f64"}[wl.storage],
+        "data": "synthetic (seeded layered + GRF model, Ricker source)",
+        "config": {"workload": wl.name, "nx": wl.nx, "nz": wl.nz, "nshots": wl.nshots, "npml": wl.npml,
+                   "dx": wl.dx, "dt": wl.dt, "nt": wl.nt, "tile": r["tile"]},
+        "shots_per_s": round(wl.nshots / (ms * 1e-3 * wl.nt), 4),
+        "hbm_fraction_step": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel_ms": round(kms, 5), "algorithmic_bytes": nbytes},
+        "finite": r["finite"],
+    }
+    if not args.no_cpu and wl.wave == _lib.WAVE_PSV and wl.nshots == 1:
+        out["cpu_baseline"] = cpu_baseline(wl)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,967 @@
+// fd23 context: device buffers, launchers, graph-captured loop, and the
+// C ABI declared in include/fd23.h.
+#include "fd23.h"
+#include "fd_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+using namespace fd23;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+static int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+#define HIPCHK(x)                                                                  \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess)                                                      \
+            return fail(FD23_EHIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_),    \
+                        __FILE__, __LINE__);                                       \
+    } while (0)
+
+extern "C" const char *fd23_last_error(void) { return g_err.c_str(); }
+extern "C" void fd23_set_error_(const char *msg) { g_err = msg; }
+
+// ---------------------------------------------------------------------------
+// tile shapes: rows per thread (tile height 4*RZ) x shots per workgroup
+// ---------------------------------------------------------------------------
+struct TileCfg {
+    int rz, spb;
+};
+static const TileCfg kTiles[] = {{4, 1}, {2, 1}, {8, 1}, {4, 2}, {4, 4}, {4, 8}, {8, 2}, {8, 4}, {2, 4}, {2, 8}};
+static const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
+
+extern "C" int fd23_num_tiles(void) { return kNumTiles; }
+
+constexpr int kMaxTZ = 32;     // largest tile height (BY * max RZ)
+constexpr int kGuard = 256;    // zero elements before shot 0 / after the last
+
+static int round_up(int a, int b) { return (a + b - 1) / b * b; }
+
+struct HostSrc {
+    int kind = -1, xs = 0, zs = 0;
+    std::vector<double> stf;
+    double M[3] = {0, 0, 0};
+};
+
+struct fd23_ctx {
+    fd23_desc d{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int esize = 4;     // bytes per stored field/psi element
+    int csize = 4;     // bytes per compute/material element
+    int px = 0, rows = 0;
+    long sstride = 0;  // elements between shots
+    long xpsi = 0, zpsi = 0;
+    int nfield = 0, npsi = 0, nmat = 0, ncomp = 0, nscale = 0;
+
+    void *field_alloc[5] = {};
+    void *fields[5] = {};   // shot 0, row 0, col 0
+    void *psi[8] = {};
+    void *mat[5] = {};
+    void *prof = nullptr;   // [npsi][3][n] compute type
+    long prof_off[8] = {};  // element offset of profile p (b, then c, then k)
+    int prof_n[8] = {};
+    void **d_fieldtab = nullptr;
+    void **d_comptab = nullptr;
+    int *d_compfield = nullptr;
+    float *scl = nullptr;
+    unsigned *amax = nullptr;
+    int *clock = nullptr;
+
+    // sources
+    std::vector<HostSrc> src;
+    Tap *d_taps = nullptr;
+    SrcInfo *d_src = nullptr;
+    double *d_stf = nullptr;
+    bool src_dirty = true;
+    size_t taps_cap = 0, stf_cap = 0;
+
+    // receivers
+    long *d_roff = nullptr;
+    int *d_nrec = nullptr;
+    std::vector<long> h_roff;
+    std::vector<int> h_nrec;
+    void *seis = nullptr;
+
+    int tile = 0;
+    bool have_mat = false, have_cpml = false;
+
+    // graphs keyed by (itrecord, steps)
+    std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+
+    // box coupling state (see fd23_box.hip)
+    void *box = nullptr;
+};
+
+// implemented in fd23_box.hip
+int fd23_box_launch_pre_stress(fd23_ctx *c);
+int fd23_box_launch_post_stress(fd23_ctx *c);
+int fd23_box_launch_post_velocity(fd23_ctx *c);
+void fd23_box_free(fd23_ctx *c);
+
+// accessors used by the box module
+hipStream_t fd23_ctx_stream(fd23_ctx *c) { return c->stream; }
+void *fd23_ctx_field(fd23_ctx *c, int internal) { return c->fields[internal]; }
+void *fd23_ctx_material(fd23_ctx *c, int i) { return c->mat[i]; }
+int fd23_ctx_px(fd23_ctx *c) { return c->px; }
+long fd23_ctx_sstride(fd23_ctx *c) { return c->sstride; }
+void *&fd23_ctx_box(fd23_ctx *c) { return c->box; }
+const fd23_desc &fd23_ctx_desc(fd23_ctx *c) { return c->d; }
+int *fd23_ctx_clock(fd23_ctx *c) { return c->clock; }
+const float *fd23_ctx_scl(fd23_ctx *c) { return c->scl; }
+
+static bool is_psv(const fd23_ctx *c) { return c->d.wave == FD23_WAVE_PSV; }
+
+static size_t layout(const fd23_desc *d, int *px, int *rows, long *sstride, long *xpsi, long *zpsi, int *esize,
+                     int *csize) {
+    *px = round_up(d->nx + HALO, 64);
+    *rows = round_up(d->nz, kMaxTZ) + 2 * HALO + 1;
+    *sstride = (long)(*rows) * (*px);
+    *xpsi = 2L * d->npml * d->nz;
+    *zpsi = 2L * d->npml * d->nx;
+    switch (d->storage) {
+    case FD23_STORE_F16:
+    case FD23_STORE_BF16: *esize = 2; *csize = 4; break;
+    case FD23_STORE_F64: *esize = 8; *csize = 8; break;
+    default: *esize = 4; *csize = 4; break;
+    }
+    const bool psv = d->wave == FD23_WAVE_PSV;
+    const int nf = psv ? 5 : 3, np = psv ? 4 : 2, nm = psv ? 5 : 3, nc = psv ? 2 : 1;
+    size_t S = (size_t)d->nshots;
+    size_t bytes = 0;
+    bytes += (size_t)nf * ((size_t)S * (*sstride) + 2 * kGuard) * (*esize);
+    bytes += (size_t)np * S * ((*xpsi) + (*zpsi)) * (*esize);
+    bytes += (size_t)nm * (size_t)d->nz * (*px) * (*csize);
+    bytes += (size_t)2 * np * 3 * (d->nx + d->nz) * (*csize);
+    bytes += (size_t)d->max_frames * S * nc * d->max_rec * (*csize);
+    bytes += (size_t)S * nc * d->max_rec * sizeof(long);
+    return bytes;
+}
+
+extern "C" size_t fd23_desc_device_bytes(const fd23_desc *d) {
+    int px, rows, es, cs;
+    long ss, xp, zp;
+    if (!d) return 0;
+    return layout(d, &px, &rows, &ss, &xp, &zp, &es, &cs);
+}
+
+extern "C" size_t fd23_ctx_device_bytes(const fd23_ctx *c) { return c ? fd23_desc_device_bytes(&c->d) : 0; }
+
+static void free_ctx(fd23_ctx *c) {
+    if (!c) return;
+    for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+    fd23_box_free(c);
+    for (auto p : c->field_alloc) if (p) hipFree(p);
+    for (auto p : c->psi) if (p) hipFree(p);
+    for (auto p : c->mat) if (p) hipFree(p);
+    void *others[] = {c->prof, c->d_fieldtab, c->d_comptab, c->d_compfield, c->scl, c->amax, c->clock,
+                      c->d_taps, c->d_src, c->d_stf, c->d_roff, c->d_nrec, c->seis};
+    for (auto p : others) if (p) hipFree(p);
+    if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" int fd23_ctx_create(const fd23_desc *desc, fd23_ctx **out) {
+    if (!desc || !out) return fail(FD23_EINVAL, "null argument");
+    *out = nullptr;
+    const fd23_desc &d = *desc;
+    if (d.wave != FD23_WAVE_PSV && d.wave != FD23_WAVE_SH) return fail(FD23_EINVAL, "bad wave type %d", d.wave);
+    if (d.storage < 0 || d.storage > 3) return fail(FD23_EINVAL, "bad storage type %d", d.storage);
+    if (d.nx < 16 || d.nz < 16) return fail(FD23_EINVAL, "grid %dx%d too small (min 16)", d.nx, d.nz);
+    if (d.npml < 0 || 2 * d.npml > d.nx || 2 * d.npml > d.nz)
+        return fail(FD23_EINVAL, "npml %d does not fit grid %dx%d", d.npml, d.nx, d.nz);
+    if (d.nshots < 1) return fail(FD23_EINVAL, "nshots must be >= 1");
+    if (d.max_frames < 0 || d.max_rec < 0) return fail(FD23_EINVAL, "negative record capacity");
+    if ((long)d.nx * d.nz > (1L << 30)) return fail(FD23_EINVAL, "grid too large");
+
+    fd23_ctx *c = new fd23_ctx();
+    c->d = d;
+    size_t bytes = layout(&d, &c->px, &c->rows, &c->sstride, &c->xpsi, &c->zpsi, &c->esize, &c->csize);
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) == hipSuccess && bytes + (64u << 20) > freeb) {
+        delete c;
+        return fail(FD23_ENOMEM, "context needs %zu bytes, %zu free", bytes, freeb);
+    }
+    const bool psv = is_psv(c);
+    c->nfield = psv ? 5 : 3;
+    c->npsi = psv ? 8 : 4;
+    c->nmat = psv ? 5 : 3;
+    c->ncomp = psv ? 2 : 1;
+    c->nscale = psv ? SC_PSV_N : SC_SH_N;
+    c->src.resize(d.nshots);
+    c->h_nrec.assign((size_t)d.nshots * c->ncomp, 0);
+    c->h_roff.assign((size_t)d.nshots * c->ncomp * std::max(d.max_rec, 1), -1);
+
+#define ALLOC(ptr, n)                                                           \
+    do {                                                                        \
+        hipError_t e_ = hipMalloc((void **)&(ptr), (n));                         \
+        if (e_ != hipSuccess) {                                                 \
+            free_ctx(c);                                                        \
+            return fail(FD23_ENOMEM, "hipMalloc(%zu): %s", (size_t)(n), hipGetErrorString(e_)); \
+        }                                                                       \
+        hipMemset((ptr), 0, (n));                                               \
+    } while (0)
+
+    if (d.stream) {
+        c->stream = (hipStream_t)d.stream;
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            delete c;
+            return fail(FD23_EHIP, "stream creation failed");
+        }
+        c->own_stream = true;
+    }
+    const size_t S = d.nshots;
+    for (int f = 0; f < c->nfield; f++) {
+        size_t n = ((size_t)S * c->sstride + 2 * kGuard) * c->esize;
+        ALLOC(c->field_alloc[f], n);
+        c->fields[f] = (char *)c->field_alloc[f] + (size_t)(kGuard + (long)HALO * c->px) * c->esize;
+    }
+    for (int p = 0; p < c->npsi; p++) {
+        const bool xdir = psv ? p < 4 : p < 2;
+        size_t n = std::max<size_t>(1, (size_t)S * (xdir ? c->xpsi : c->zpsi)) * c->esize;
+        ALLOC(c->psi[p], n);
+    }
+    for (int m = 0; m < c->nmat; m++) ALLOC(c->mat[m], (size_t)d.nz * c->px * c->csize);
+    {
+        long off = 0;
+        for (int p = 0; p < c->npsi; p++) {
+            const bool xdir = psv ? p < 4 : p < 2;
+            c->prof_n[p] = xdir ? d.nx : d.nz;
+            c->prof_off[p] = off;
+            off += 3L * c->prof_n[p];
+        }
+        ALLOC(c->prof, (size_t)off * c->csize);
+    }
+    ALLOC(c->d_fieldtab, sizeof(void *) * 5);
+    ALLOC(c->d_comptab, sizeof(void *) * 2);
+    ALLOC(c->d_compfield, sizeof(int) * 2);
+    ALLOC(c->scl, sizeof(float) * 16);
+    ALLOC(c->amax, sizeof(unsigned) * 16);
+    ALLOC(c->clock, sizeof(int) * 2);
+    ALLOC(c->d_src, sizeof(SrcInfo) * S);
+    ALLOC(c->d_nrec, sizeof(int) * S * c->ncomp);
+    ALLOC(c->d_roff, sizeof(long) * S * c->ncomp * std::max(d.max_rec, 1));
+    if (d.max_frames > 0 && d.max_rec > 0)
+        ALLOC(c->seis, (size_t)d.max_frames * S * c->ncomp * d.max_rec * c->csize);
+#undef ALLOC
+    // pointer tables
+    hipMemcpy(c->d_fieldtab, c->fields, sizeof(void *) * 5, hipMemcpyHostToDevice);
+    void *comps[2] = {c->fields[0], psv ? c->fields[1] : nullptr};
+    hipMemcpy(c->d_comptab, comps, sizeof(void *) * 2, hipMemcpyHostToDevice);
+    int cf[2] = {0, 1};
+    hipMemcpy(c->d_compfield, cf, sizeof cf, hipMemcpyHostToDevice);
+    std::vector<float> ones(16, 1.0f);
+    hipMemcpy(c->scl, ones.data(), sizeof(float) * 16, hipMemcpyHostToDevice);
+    hipMemcpy(c->d_roff, c->h_roff.data(), sizeof(long) * c->h_roff.size(), hipMemcpyHostToDevice);
+    c->tile = (d.nshots >= 4) ? 4 : 0;
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        free_ctx(c);
+        return fail(FD23_EHIP, "device init: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return FD23_OK;
+}
+
+extern "C" int fd23_ctx_destroy(fd23_ctx *c) {
+    if (!c) return FD23_OK;
+    hipStreamSynchronize(c->stream);
+    free_ctx(c);
+    return FD23_OK;
+}
+
+// ---------------------------------------------------------------------------
+// material, C-PML, scales
+// ---------------------------------------------------------------------------
+extern "C" int fd23_set_material(fd23_ctx *c, const void *const *arrays) {
+    if (!c || !arrays) return fail(FD23_EINVAL, "null argument");
+    for (int m = 0; m < c->nmat; m++) {
+        if (!arrays[m]) return fail(FD23_EINVAL, "material array %d is null", m);
+        HIPCHK(hipMemcpy2DAsync(c->mat[m], (size_t)c->px * c->csize, arrays[m], (size_t)c->d.nx * c->csize,
+                                (size_t)c->d.nx * c->csize, c->d.nz, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_mat = true;
+    return FD23_OK;
+}
+
+extern "C" int fd23_set_cpml(fd23_ctx *c, const double *packed) {
+    if (!c || !packed) return fail(FD23_EINVAL, "null argument");
+    long total = c->prof_off[c->npsi - 1] + 3L * c->prof_n[c->npsi - 1];
+    std::vector<char> buf((size_t)total * c->csize);
+    for (long i = 0; i < total; i++) {
+        if (c->csize == 8)
+            ((double *)buf.data())[i] = packed[i];
+        else
+            ((float *)buf.data())[i] = (float)packed[i];
+    }
+    HIPCHK(hipMemcpyAsync(c->prof, buf.data(), buf.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_cpml = true;
+    return FD23_OK;
+}
+
+// per-field power-of-two scales for FP16/BF16 storage
+extern "C" int fd23_set_scales(fd23_ctx *c, const float *scales, int n) {
+    if (!c || !scales || n < 1 || n > 16) return fail(FD23_EINVAL, "bad scales");
+    HIPCHK(hipMemcpyAsync(c->scl, scales, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FD23_OK;
+}
+extern "C" int fd23_get_scales(fd23_ctx *c, float *scales, unsigned *amax_bits, int n) {
+    if (!c || n < 1 || n > 16) return fail(FD23_EINVAL, "bad scales");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (scales) HIPCHK(hipMemcpy(scales, c->scl, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (amax_bits) HIPCHK(hipMemcpy(amax_bits, c->amax, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
+    return FD23_OK;
+}
+
+// ---------------------------------------------------------------------------
+// sources
+// ---------------------------------------------------------------------------
+static long cell_off(const fd23_ctx *c, int x, int z) { return (long)z * c->px + x; }
+
+extern "C" int fd23_set_source(fd23_ctx *c, int shot, int kind, int xs, int zs, const double *stf, int lsrc,
+                               const double *M) {
+    if (!c || shot < 0 || shot >= c->d.nshots) return fail(FD23_EINVAL, "bad shot %d", shot);
+    if (kind < FD23_SRC_LINE || kind > FD23_SRC_FORCE || (kind == FD23_SRC_FORCE && is_psv(c)))
+        return fail(FD23_EINVAL, "bad source kind %d", kind);
+    if (lsrc < 0 || (lsrc > 0 && (!stf || !M))) return fail(FD23_EINVAL, "bad source samples");
+    HostSrc &h = c->src[shot];
+    h.kind = kind;
+    h.xs = xs;
+    h.zs = zs;
+    h.stf.assign(stf, stf + lsrc);
+    for (int i = 0; i < (is_psv(c) ? 3 : 2); i++) h.M[i] = M ? M[i] : 0.0;
+    c->src_dirty = true;
+    return FD23_OK;
+}
+
+// Build the tap list of one shot in the reference's order.  Taps outside
+// the domain are skipped, as the reference's nz1/nz2 test would (its x
+// range is never checked; such sources are outside its valid inputs).
+static void build_taps(const fd23_ctx *c, const HostSrc &h, std::vector<Tap> &taps) {
+    const int nx = c->d.nx, nz = c->d.nz;
+    auto add = [&](int field, int x, int z, double m, double q, int mode) {
+        if (x < 0 || x >= nx || z < 0 || z >= nz) return;
+        taps.push_back(Tap{field, cell_off(c, x, z), m, q, mode});
+    };
+    const int xs = h.xs, zs = h.zs;
+    if (is_psv(c)) {
+        // internal field order U W Txx Tzz Txz
+        const double Mxx = h.M[0], Mxz = h.M[1], Mzz = h.M[2];
+        if (h.kind == FD23_SRC_LINE) {
+            add(2, xs, zs, Mxx, -1.0, 0);
+            add(3, xs, zs, Mzz, -1.0, 0);
+            for (int x = xs - 1; x <= xs; x++)
+                for (int z = zs; z <= zs + 1; z++) add(4, x, z, Mxz, -0.25, 0);
+        } else {
+            add(2, xs, zs, Mxx, -1.0, 0);
+            add(2, xs - 1, zs, Mxx, 1.0, 0);
+            add(3, xs, zs, Mzz, -1.0, 0);
+            add(3, xs - 1, zs, Mzz, 1.0, 0);
+            for (int x = xs - 1; x <= xs; x++)
+                for (int z = zs; z <= zs + 1; z++) {
+                    add(4, x, z, Mxz, -0.25, 0);
+                    add(4, x - 1, z, Mxz, 0.25, 0);
+                }
+        }
+    } else {
+        // internal field order V Tx Tz
+        const double Mxy = h.M[0], Mzy = h.M[1];
+        if (h.kind == FD23_SRC_FORCE) {
+            add(0, xs, zs, Mxy, -0.5, 1);
+            return;
+        }
+        add(1, xs, zs, Mxy, -0.5, 1);
+        add(1, xs - 1, zs, Mxy, -0.5, 1);
+        add(2, xs, zs, Mzy, -0.5, 1);
+        add(2, xs, zs + 1, Mzy, -0.5, 1);
+        if (h.kind == FD23_SRC_P) {
+            add(1, xs - 1, zs, Mxy, 0.5, 1);
+            add(1, xs - 2, zs, Mxy, 0.5, 1);
+            add(2, xs - 1, zs, Mzy, 0.5, 1);
+            add(2, xs - 1, zs + 1, Mzy, 0.5, 1);
+        }
+    }
+}
+
+static int upload_sources(fd23_ctx *c) {
+    if (!c->src_dirty) return FD23_OK;
+    std::vector<Tap> taps;
+    std::vector<SrcInfo> info(c->d.nshots);
+    std::vector<double> stf;
+    for (int s = 0; s < c->d.nshots; s++) {
+        const HostSrc &h = c->src[s];
+        SrcInfo si{};
+        si.tap0 = (int)taps.size();
+        si.stf0 = (long)stf.size();
+        if (h.kind >= 0) {
+            build_taps(c, h, taps);
+            si.lsrc = (int)h.stf.size();
+            si.clamp = h.kind == FD23_SRC_P ? 1 : 0;
+            stf.insert(stf.end(), h.stf.begin(), h.stf.end());
+        }
+        si.ntap = (int)taps.size() - si.tap0;
+        info[s] = si;
+    }
+    if (taps.size() > c->taps_cap) {
+        if (c->d_taps) hipFree(c->d_taps);
+        c->taps_cap = std::max<size_t>(taps.size(), 16);
+        HIPCHK(hipMalloc((void **)&c->d_taps, sizeof(Tap) * c->taps_cap));
+    }
+    if (stf.size() > c->stf_cap || !c->d_stf) {
+        if (c->d_stf) hipFree(c->d_stf);
+        c->stf_cap = std::max<size_t>(stf.size(), 16);
+        HIPCHK(hipMalloc((void **)&c->d_stf, sizeof(double) * c->stf_cap));
+    }
+    if (!c->d_taps) {
+        HIPCHK(hipMalloc((void **)&c->d_taps, sizeof(Tap) * 16));
+        c->taps_cap = 16;
+    }
+    if (!taps.empty())
+        HIPCHK(hipMemcpyAsync(c->d_taps, taps.data(), sizeof(Tap) * taps.size(), hipMemcpyHostToDevice, c->stream));
+    if (!stf.empty())
+        HIPCHK(hipMemcpyAsync(c->d_stf, stf.data(), sizeof(double) * stf.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_src, info.data(), sizeof(SrcInfo) * info.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->src_dirty = false;
+    // a graph holds the old pointers if the buffers moved
+    for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+    return FD23_OK;
+}
+
+// ---------------------------------------------------------------------------
+// receivers
+// ---------------------------------------------------------------------------
+extern "C" int fd23_set_receivers(fd23_ctx *c, int shot, int comp, const int *rx, const int *rz, int n) {
+    if (!c || shot < 0 || shot >= c->d.nshots) return fail(FD23_EINVAL, "bad shot %d", shot);
+    if (comp < 0 || comp >= c->ncomp) return fail(FD23_EINVAL, "bad component %d", comp);
+    if (n < 0 || n > c->d.max_rec) return fail(FD23_EINVAL, "%d receivers exceed max_rec %d", n, c->d.max_rec);
+    if (n > 0 && (!rx || !rz)) return fail(FD23_EINVAL, "null receiver arrays");
+    const long nx = c->d.nx, N = nx * (long)c->d.nz;
+    const long base = ((long)shot * c->ncomp + comp) * c->d.max_rec;
+    for (int i = 0; i < c->d.max_rec; i++) {
+        long off = -1;
+        if (i < n) {
+            // flat index rx + rz*nx (cu_record.cu:144): a receiver at x=-1
+            // lands on the previous row's last cell
+            long q = (long)rx[i] + (long)rz[i] * nx;
+            if (q >= 0 && q < N) off = (q / nx) * c->px + (q % nx);
+        }
+        c->h_roff[base + i] = off;
+    }
+    c->h_nrec[(size_t)shot * c->ncomp + comp] = n;
+    HIPCHK(hipMemcpyAsync(c->d_roff + base, c->h_roff.data() + base, sizeof(long) * c->d.max_rec,
+                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_nrec, c->h_nrec.data(), sizeof(int) * c->h_nrec.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FD23_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static Grid make_grid(const fd23_ctx *c) {
+    Grid g;
+    g.nx = c->d.nx;
+    g.nz = c->d.nz;
+    g.px = c->px;
+    g.sstride = c->sstride;
+    g.npml = c->d.npml;
+    g.top = c->d.topabsorb ? 1 : 0;
+    g.nshots = c->d.nshots;
+    g.xpsi_stride = c->xpsi;
+    g.zpsi_stride = c->zpsi;
+    return g;
+}
+
+template <class C> static Prof<C> prof_of(const fd23_ctx *c, int p) {
+    const C *base = (const C *)c->prof + c->prof_off[p];
+    const int n = c->prof_n[p];
+    return Prof<C>{base, base + n, base + 2 * n};
+}
+
+template <class S, int RZ, int SPB> static void launch_stress_t(fd23_ctx *c) {
+    using C = typename Store<S>::C;
+    const Grid g = make_grid(c);
+    dim3 block(TX, BY);
+    dim3 grid((g.nx + TX - 1) / TX, (g.nz + BY * RZ - 1) / (BY * RZ), (g.nshots + SPB - 1) / SPB);
+    if (is_psv(c)) {
+        PsvFields<S> f{(S *)c->fields[0], (S *)c->fields[1], (S *)c->fields[2], (S *)c->fields[3],
+                       (S *)c->fields[4]};
+        PsvStressMat<C> m{(const C *)c->mat[4], (const C *)c->mat[2], (const C *)c->mat[3]};
+        PsvStressPsi<S> ps{(S *)c->psi[2], (S *)c->psi[3], (S *)c->psi[6], (S *)c->psi[7]};
+        PsvStressProf<C> pr{prof_of<C>(c, 2), prof_of<C>(c, 3), prof_of<C>(c, 6), prof_of<C>(c, 7)};
+        hipLaunchKernelGGL((k_psv_stress<S, RZ, SPB>), grid, block, 0, c->stream, f, m, ps, pr, g, c->scl,
+                           c->amax);
+    } else {
+        ShFields<S> f{(S *)c->fields[0], (S *)c->fields[1], (S *)c->fields[2]};
+        ShStressMat<C> m{(const C *)c->mat[1], (const C *)c->mat[2]};
+        ShPsi2<S> ps{(S *)c->psi[1], (S *)c->psi[3]};
+        ShProf2<C> pr{prof_of<C>(c, 1), prof_of<C>(c, 3)};
+        hipLaunchKernelGGL((k_sh_stress<S, RZ, SPB>), grid, block, 0, c->stream, f, m, ps, pr, g, c->scl,
+                           c->amax);
+    }
+}
+
+template <class S, int RZ, int SPB> static void launch_velocity_t(fd23_ctx *c) {
+    using C = typename Store<S>::C;
+    const Grid g = make_grid(c);
+    dim3 block(TX, BY);
+    dim3 grid((g.nx + TX - 1) / TX, (g.nz + BY * RZ - 1) / (BY * RZ), (g.nshots + SPB - 1) / SPB);
+    if (is_psv(c)) {
+        PsvFields<S> f{(S *)c->fields[0], (S *)c->fields[1], (S *)c->fields[2], (S *)c->fields[3],
+                       (S *)c->fields[4]};
+        PsvVelMat<C> m{(const C *)c->mat[0], (const C *)c->mat[1]};
+        PsvVelPsi<S> ps{(S *)c->psi[0], (S *)c->psi[1], (S *)c->psi[5], (S *)c->psi[4]};
+        PsvVelProf<C> pr{prof_of<C>(c, 0), prof_of<C>(c, 1), prof_of<C>(c, 5), prof_of<C>(c, 4)};
+        hipLaunchKernelGGL((k_psv_velocity<S, RZ, SPB>), grid, block, 0, c->stream, f, m, ps, pr, g, c->scl,
+                           c->amax);
+    } else {
+        ShFields<S> f{(S *)c->fields[0], (S *)c->fields[1], (S *)c->fields[2]};
+        ShPsi2<S> ps{(S *)c->psi[0], (S *)c->psi[2]};
+        ShProf2<C> pr{prof_of<C>(c, 0), prof_of<C>(c, 2)};
+        hipLaunchKernelGGL((k_sh_velocity<S, RZ, SPB>), grid, block, 0, c->stream, f, (const C *)c->mat[0], ps,
+                           pr, g, c->scl, c->amax);
+    }
+}
+
+template <class S> static void launch_pass(fd23_ctx *c, bool stress) {
+    const TileCfg t = kTiles[c->tile];
+#define TILE_CASE(R, P)                                                        \
+    if (t.rz == R && t.spb == P) {                                             \
+        if (stress) launch_stress_t<S, R, P>(c); else launch_velocity_t<S, R, P>(c); \
+        return;                                                                \
+    }
+    TILE_CASE(4, 1) TILE_CASE(2, 1) TILE_CASE(8, 1) TILE_CASE(4, 2) TILE_CASE(4, 4) TILE_CASE(4, 8)
+    TILE_CASE(8, 2) TILE_CASE(8, 4) TILE_CASE(2, 4) TILE_CASE(2, 8)
+#undef TILE_CASE
+}
+
+static void dispatch_pass(fd23_ctx *c, bool stress) {
+    switch (c->d.storage) {
+    case FD23_STORE_F16: launch_pass<__half>(c, stress); break;
+    case FD23_STORE_BF16: launch_pass<__hip_bfloat16>(c, stress); break;
+    case FD23_STORE_F64: launch_pass<double>(c, stress); break;
+    default: launch_pass<float>(c, stress); break;
+    }
+}
+
+template <class S> static void launch_inject_t(fd23_ctx *c, int it, bool use_clock) {
+    // one workgroup: its thread 0 advances the step clock after every
+    // shot's taps are done
+    const int nthr = std::min(256, round_up(c->d.nshots, 64));
+    hipLaunchKernelGGL((k_inject<S>), dim3(1), dim3(nthr), 0, c->stream, (S *const *)c->d_fieldtab,
+                       c->nfield, c->sstride, c->d_taps, c->d_src, c->d_stf, c->d.nshots, it,
+                       use_clock ? c->clock : nullptr, c->scl);
+}
+
+static void dispatch_inject(fd23_ctx *c, int it, bool use_clock) {
+    switch (c->d.storage) {
+    case FD23_STORE_F16: launch_inject_t<__half>(c, it, use_clock); break;
+    case FD23_STORE_BF16: launch_inject_t<__hip_bfloat16>(c, it, use_clock); break;
+    case FD23_STORE_F64: launch_inject_t<double>(c, it, use_clock); break;
+    default: launch_inject_t<float>(c, it, use_clock); break;
+    }
+}
+
+template <class S> static void launch_record_t(fd23_ctx *c, int it, bool use_clock, int itrecord) {
+    using C = typename Store<S>::C;
+    const long total = (long)c->d.nshots * c->ncomp * c->d.max_rec;
+    if (total == 0 || !c->seis) return;
+    const int nblk = (int)std::min<long>((total + 255) / 256, 1024);
+    hipLaunchKernelGGL((k_record<S>), dim3(nblk), dim3(256), 0, c->stream, (S *const *)c->d_comptab, c->ncomp,
+                       c->sstride, c->d_roff, c->d_nrec, c->d.max_rec, c->d.nshots, (C *)c->seis,
+                       c->d.max_frames, it, use_clock ? c->clock : nullptr, itrecord, c->scl, c->d_compfield);
+}
+
+static void dispatch_record(fd23_ctx *c, int it, bool use_clock, int itrecord) {
+    switch (c->d.storage) {
+    case FD23_STORE_F16: launch_record_t<__half>(c, it, use_clock, itrecord); break;
+    case FD23_STORE_BF16: launch_record_t<__hip_bfloat16>(c, it, use_clock, itrecord); break;
+    case FD23_STORE_F64: launch_record_t<double>(c, it, use_clock, itrecord); break;
+    default: launch_record_t<float>(c, it, use_clock, itrecord); break;
+    }
+}
+
+static int ready(fd23_ctx *c) {
+    if (!c) return fail(FD23_EINVAL, "null context");
+    if (!c->have_mat) return fail(FD23_EINVAL, "material not set");
+    if (!c->have_cpml) return fail(FD23_EINVAL, "C-PML profiles not set");
+    return upload_sources(c);
+}
+
+static int check_launch() {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(FD23_EHIP, "kernel launch: %s", hipGetErrorString(e));
+    return FD23_OK;
+}
+
+extern "C" int fd23_step_stress(fd23_ctx *c) {
+    int r = ready(c);
+    if (r) return r;
+    if (c->box && (r = fd23_box_launch_pre_stress(c))) return r;
+    dispatch_pass(c, true);
+    if (c->box && (r = fd23_box_launch_post_stress(c))) return r;
+    return check_launch();
+}
+extern "C" int fd23_inject(fd23_ctx *c, int it) {
+    int r = ready(c);
+    if (r) return r;
+    dispatch_inject(c, it, false);
+    return check_launch();
+}
+extern "C" int fd23_step_velocity(fd23_ctx *c) {
+    int r = ready(c);
+    if (r) return r;
+    dispatch_pass(c, false);
+    if (c->box && (r = fd23_box_launch_post_velocity(c))) return r;
+    return check_launch();
+}
+extern "C" int fd23_record(fd23_ctx *c, int it, int itrecord) {
+    if (!c) return fail(FD23_EINVAL, "null context");
+    if (itrecord < 1) return fail(FD23_EINVAL, "itrecord must be >= 1");
+    dispatch_record(c, it, false, itrecord);
+    return check_launch();
+}
+
+// one step with the device clock (graph body)
+static int enqueue_step(fd23_ctx *c, int itrecord) {
+    int r;
+    if (c->box && (r = fd23_box_launch_pre_stress(c))) return r;
+    dispatch_pass(c, true);
+    if (c->box && (r = fd23_box_launch_post_stress(c))) return r;
+    dispatch_inject(c, 0, true);
+    dispatch_pass(c, false);
+    if (c->box && (r = fd23_box_launch_post_velocity(c))) return r;
+    dispatch_record(c, 0, true, itrecord);
+    return FD23_OK;
+}
+
+static const int kGraphSteps = 32;
+
+static int get_graph(fd23_ctx *c, int itrecord, int steps, hipGraphExec_t *out) {
+    auto key = std::make_pair(itrecord, steps);
+    auto it = c->graphs.find(key);
+    if (it != c->graphs.end()) {
+        *out = it->second;
+        return FD23_OK;
+    }
+    hipGraph_t graph;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int r = FD23_OK;
+    for (int s = 0; s < steps && r == FD23_OK; s++) r = enqueue_step(c, itrecord);
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (r) return r;
+    if (e != hipSuccess) return fail(FD23_EHIP, "graph capture: %s", hipGetErrorString(e));
+    hipGraphExec_t exec;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(FD23_EHIP, "graph instantiate: %s", hipGetErrorString(e));
+    c->graphs[key] = exec;
+    *out = exec;
+    return FD23_OK;
+}
+
+extern "C" int fd23_run(fd23_ctx *c, int it0, int nt, int itrecord) {
+    int r = ready(c);
+    if (r) return r;
+    if (nt < 0 || it0 < 0 || itrecord < 1) return fail(FD23_EINVAL, "bad run range");
+    int clk[2] = {it0, it0};
+    HIPCHK(hipMemcpyAsync(c->clock, clk, sizeof clk, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream)); // clk is a stack buffer
+    const int full = nt / kGraphSteps, rest = nt % kGraphSteps;
+    if (full > 0) {
+        hipGraphExec_t g;
+        if ((r = get_graph(c, itrecord, kGraphSteps, &g))) return r;
+        for (int i = 0; i < full; i++) HIPCHK(hipGraphLaunch(g, c->stream));
+    }
+    for (int i = 0; i < rest; i++)
+        if ((r = enqueue_step(c, itrecord))) return r;
+    return check_launch();
+}
+
+extern "C" int fd23_sync(fd23_ctx *c) {
+    if (!c) return fail(FD23_EINVAL, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FD23_OK;
+}
+
+// ---------------------------------------------------------------------------
+// outputs and state
+// ---------------------------------------------------------------------------
+extern "C" int fd23_get_seismograms(fd23_ctx *c, int shot, int comp, float *out, int nframes) {
+    if (!c || !out || shot < 0 || shot >= c->d.nshots || comp < 0 || comp >= c->ncomp)
+        return fail(FD23_EINVAL, "bad arguments");
+    if (nframes < 0 || nframes > c->d.max_frames) return fail(FD23_EINVAL, "nframes %d > capacity", nframes);
+    const int n = c->h_nrec[(size_t)shot * c->ncomp + comp];
+    if (nframes == 0 || n == 0) return FD23_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const long frame_stride = (long)c->d.nshots * c->ncomp * c->d.max_rec;
+    const long off = ((long)shot * c->ncomp + comp) * c->d.max_rec;
+    // copies rows of n values out of the frame-major buffer
+    HIPCHK(hipMemcpy2D(out, (size_t)n * c->csize, (char *)c->seis + (size_t)off * c->csize,
+                       (size_t)frame_stride * c->csize, (size_t)n * c->csize, nframes, hipMemcpyDeviceToHost));
+    return FD23_OK;
+}
+
+static int internal_field(const fd23_ctx *c, int field) {
+    if (is_psv(c)) {
+        static const int map[5] = {0, 1, 2, 4, 3}; // U W Txx Txz Tzz -> U W Txx Tzz Txz
+        return field >= 0 && field < 5 ? map[field] : -1;
+    }
+    return field >= 0 && field < 3 ? field : -1;
+}
+
+static float host_half_to_float(uint16_t h, int storage) {
+    if (storage == FD23_STORE_BF16) {
+        uint32_t u = (uint32_t)h << 16;
+        float f;
+        memcpy(&f, &u, 4);
+        return f;
+    }
+    __half hv;
+    memcpy(&hv, &h, 2);
+    return __half2float(hv);
+}
+static uint16_t host_float_to_half(float v, int storage) {
+    if (storage == FD23_STORE_BF16) {
+        __hip_bfloat16 b = __float2bfloat16(v);
+        uint16_t u;
+        memcpy(&u, &b, 2);
+        return u;
+    }
+    __half hv = __float2half(v);
+    uint16_t u;
+    memcpy(&u, &hv, 2);
+    return u;
+}
+
+// copy one pitched array (n rows of w elements) between device and host,
+// converting scaled storage to the compute type
+static int xfer(fd23_ctx *c, void *dev, long pitch, int w, int h, void *host, float scale, bool to_host) {
+    const int es = c->esize, cs = c->csize;
+    std::vector<char> tmp((size_t)w * h * es);
+    if (to_host) {
+        HIPCHK(hipMemcpy2DAsync(tmp.data(), (size_t)w * es, dev, (size_t)pitch * es, (size_t)w * es, h,
+                                hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (es == cs) {
+            memcpy(host, tmp.data(), tmp.size());
+        } else {
+            const uint16_t *src = (const uint16_t *)tmp.data();
+            float *dst = (float *)host;
+            for (size_t i = 0; i < (size_t)w * h; i++) dst[i] = host_half_to_float(src[i], c->d.storage) * scale;
+        }
+    } else {
+        if (es == cs) {
+            memcpy(tmp.data(), host, tmp.size());
+        } else {
+            const float *src = (const float *)host;
+            uint16_t *dst = (uint16_t *)tmp.data();
+            for (size_t i = 0; i < (size_t)w * h; i++) dst[i] = host_float_to_half(src[i] / scale, c->d.storage);
+        }
+        HIPCHK(hipMemcpy2DAsync(dev, (size_t)pitch * es, tmp.data(), (size_t)w * es, (size_t)w * es, h,
+                                hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return FD23_OK;
+}
+
+static int field_xfer(fd23_ctx *c, int shot, int field, void *host, bool to_host) {
+    const int fi = internal_field(c, field);
+    if (fi < 0 || shot < 0 || shot >= c->d.nshots || !host) return fail(FD23_EINVAL, "bad field/shot");
+    float sc[16];
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(sc, c->scl, sizeof sc, hipMemcpyDeviceToHost));
+    void *dev = (char *)c->fields[fi] + (size_t)shot * c->sstride * c->esize;
+    return xfer(c, dev, c->px, c->d.nx, c->d.nz, host, sc[fi], to_host);
+}
+
+extern "C" int fd23_get_field(fd23_ctx *c, int shot, int field, void *out) {
+    if (!c) return fail(FD23_EINVAL, "null context");
+    return field_xfer(c, shot, field, out, true);
+}
+extern "C" int fd23_set_field(fd23_ctx *c, int shot, int field, const void *in) {
+    if (!c) return fail(FD23_EINVAL, "null context");
+    return field_xfer(c, shot, field, (void *)in, false);
+}
+
+extern "C" size_t fd23_state_bytes(const fd23_ctx *c) {
+    if (!c) return 0;
+    const size_t N = (size_t)c->d.nx * c->d.nz;
+    const int half = c->npsi / 2;
+    return ((size_t)c->nfield * N + (size_t)half * (c->xpsi + c->zpsi)) * c->csize;
+}
+
+static int state_xfer(fd23_ctx *c, int shot, void *buf, size_t bytes, bool to_host) {
+    if (!c || !buf || shot < 0 || shot >= c->d.nshots) return fail(FD23_EINVAL, "bad arguments");
+    if (bytes < fd23_state_bytes(c)) return fail(FD23_EINVAL, "state buffer too small");
+    float sc[16];
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(sc, c->scl, sizeof sc, hipMemcpyDeviceToHost));
+    char *p = (char *)buf;
+    const size_t N = (size_t)c->d.nx * c->d.nz;
+    for (int f = 0; f < c->nfield; f++) {
+        const int fi = internal_field(c, f);
+        void *dev = (char *)c->fields[fi] + (size_t)shot * c->sstride * c->esize;
+        int r = xfer(c, dev, c->px, c->d.nx, c->d.nz, p, sc[fi], to_host);
+        if (r) return r;
+        p += N * c->csize;
+    }
+    const int half = c->npsi / 2;
+    for (int q = 0; q < c->npsi; q++) {
+        const long n = q < half ? c->xpsi : c->zpsi;
+        if (n == 0) continue;
+        void *dev = (char *)c->psi[q] + (size_t)shot * n * c->esize;
+        int r = xfer(c, dev, n, (int)n, 1, p, sc[c->nfield + q], to_host);
+        if (r) return r;
+        p += (size_t)n * c->csize;
+    }
+    return FD23_OK;
+}
+
+extern "C" int fd23_get_state(fd23_ctx *c, int shot, void *buf, size_t bytes) {
+    return state_xfer(c, shot, buf, bytes, true);
+}
+extern "C" int fd23_set_state(fd23_ctx *c, int shot, const void *buf, size_t bytes) {
+    return state_xfer(c, shot, (void *)buf, bytes, false);
+}
+
+// ---------------------------------------------------------------------------
+// tuning and measurement
+// ---------------------------------------------------------------------------
+extern "C" int fd23_set_tile(fd23_ctx *c, int tile) {
+    if (!c || tile < 0 || tile >= kNumTiles) return fail(FD23_EINVAL, "bad tile %d", tile);
+    if (c->tile != tile) {
+        c->tile = tile;
+        for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
+        c->graphs.clear();
+    }
+    return FD23_OK;
+}
+extern "C" int fd23_get_tile(const fd23_ctx *c) { return c ? c->tile : -1; }
+
+extern "C" double fd23_bytes_per_step(const fd23_ctx *c) {
+    if (!c) return 0;
+    const double S = c->d.nshots, N = (double)c->d.nx * c->d.nz;
+    const double wf = c->esize, wm = c->csize;
+    const double nxp = 2.0 * c->d.npml * c->d.nz;
+    const double nzp = (c->d.topabsorb ? 2.0 : 1.0) * c->d.npml * c->d.nx;
+    if (is_psv(c)) return S * (15 * wf * N + 8 * wf * (nxp + nzp)) + 5 * wm * N;
+    return S * (9 * wf * N + 4 * wf * (nxp + nzp)) + 3 * wm * N;
+}
+
+extern "C" int fd23_time_passes(fd23_ctx *c, int nsteps, double *stress_ms, double *velocity_ms) {
+    int r = ready(c);
+    if (r) return r;
+    if (nsteps < 1) return fail(FD23_EINVAL, "nsteps must be >= 1");
+    hipEvent_t e0, e1, e2;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventCreate(&e2));
+    double ts = 0, tv = 0;
+    for (int i = 0; i < nsteps; i++) {
+        hipEventRecord(e0, c->stream);
+        dispatch_pass(c, true);
+        hipEventRecord(e1, c->stream);
+        dispatch_pass(c, false);
+        hipEventRecord(e2, c->stream);
+        hipEventSynchronize(e2);
+        float a, b;
+        hipEventElapsedTime(&a, e0, e1);
+        hipEventElapsedTime(&b, e1, e2);
+        ts += a;
+        tv += b;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipEventDestroy(e2);
+    if (stress_ms) *stress_ms = ts / nsteps;
+    if (velocity_ms) *velocity_ms = tv / nsteps;
+    return check_launch();
+}
+
+// Tile autotuner: every candidate runs `steps` stress+velocity steps on the
+// context's own arrays; the state is saved first and restored afterwards so
+// tuning leaves the simulation untouched.
+extern "C" int fd23_autotune(fd23_ctx *c, int steps, double *best_ms) {
+    int r = ready(c);
+    if (r) return r;
+    if (steps < 1) steps = 4;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // snapshot fields and psi on the device
+    std::vector<void *> saved;
+    std::vector<size_t> sizes;
+    auto snap = [&](void *p, size_t n) -> int {
+        void *q = nullptr;
+        if (hipMalloc(&q, n) != hipSuccess) return fail(FD23_ENOMEM, "autotune snapshot");
+        hipMemcpyAsync(q, p, n, hipMemcpyDeviceToDevice, c->stream);
+        saved.push_back(q);
+        sizes.push_back(n);
+        return FD23_OK;
+    };
+    const size_t S = c->d.nshots;
+    for (int f = 0; f < c->nfield && !r; f++)
+        r = snap(c->field_alloc[f], ((size_t)S * c->sstride + 2 * kGuard) * c->esize);
+    const int half = c->npsi / 2;
+    for (int q = 0; q < c->npsi && !r; q++)
+        r = snap(c->psi[q], std::max<size_t>(1, S * (q < half ? c->xpsi : c->zpsi)) * c->esize);
+    if (r) {
+        for (auto p : saved) hipFree(p);
+        return r;
+    }
+    const int orig = c->tile;
+    int best = orig;
+    double bt = 1e30;
+    for (int t = 0; t < kNumTiles; t++) {
+        if (kTiles[t].spb > 1 && c->d.nshots < kTiles[t].spb) continue;
+        c->tile = t;
+        double a, b;
+        fd23_time_passes(c, 1, &a, &b); // warm-up
+        if ((r = fd23_time_passes(c, steps, &a, &b))) break;
+        if (a + b < bt) {
+            bt = a + b;
+            best = t;
+        }
+    }
+    // restore
+    size_t k = 0;
+    for (int f = 0; f < c->nfield; f++, k++)
+        hipMemcpyAsync(c->field_alloc[f], saved[k], sizes[k], hipMemcpyDeviceToDevice, c->stream);
+    for (int q = 0; q < c->npsi; q++, k++)
+        hipMemcpyAsync(c->psi[q], saved[k], sizes[k], hipMemcpyDeviceToDevice, c->stream);
+    hipStreamSynchronize(c->stream);
+    for (auto p : saved) hipFree(p);
+    c->tile = orig;
+    if (r) return r;
+    fd23_set_tile(c, best);
+    if (best_ms) *best_ms = bt;
+    return FD23_OK;
+}

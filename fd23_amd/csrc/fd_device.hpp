@@ -1,0 +1,136 @@
+// Device-side building blocks for the fd23 staggered-grid loop on gfx950.
+//
+// Layout (one array per field, all shots of a context back to back):
+//   shot s, row z, column x  ->  base + s*sstride + z*px + x
+// with px = round_up(nx + 4, 64) and 4 zero rows above row 0 and at least 4
+// below row nz-1.  Columns nx..px-1 and the extra rows are never written,
+// so every tap outside [0,nx) x [0,nz) reads 0 with no bounds test -- the
+// same "zero outside the domain" the reference gets from its zero-filled
+// LDS halo (PSV/clean_nobox/fd/cu_fd.cu:72-91).  Column -1 of row z is
+// column px-1 of row z-1, which is padding.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace fd23 {
+
+constexpr int TX = 64;  // tile width = one wavefront across x
+constexpr int BY = 4;   // wavefronts per workgroup (threadIdx.y)
+constexpr int HALO = 4; // 8th-order staggered stencil reach
+
+// ---------------------------------------------------------------------------
+// storage traits: S is the element type in HBM, C the arithmetic type.
+// Scaled types hold value/scale; scale is a power of two per array.
+// ---------------------------------------------------------------------------
+template <class S> struct Store;
+template <> struct Store<float> {
+    using C = float;
+    static constexpr bool scaled = false;
+    __device__ __forceinline__ static float ld(const float *p, long i, float) { return p[i]; }
+    __device__ __forceinline__ static void st(float *p, long i, float v, float) { p[i] = v; }
+};
+template <> struct Store<double> {
+    using C = double;
+    static constexpr bool scaled = false;
+    __device__ __forceinline__ static double ld(const double *p, long i, float) { return p[i]; }
+    __device__ __forceinline__ static void st(double *p, long i, double v, float) { p[i] = v; }
+};
+template <> struct Store<__half> {
+    using C = float;
+    static constexpr bool scaled = true;
+    __device__ __forceinline__ static float ld(const __half *p, long i, float s) {
+        return __half2float(p[i]) * s;
+    }
+    __device__ __forceinline__ static void st(__half *p, long i, float v, float inv) {
+        p[i] = __float2half_rn(v * inv);
+    }
+};
+template <> struct Store<__hip_bfloat16> {
+    using C = float;
+    static constexpr bool scaled = true;
+    __device__ __forceinline__ static float ld(const __hip_bfloat16 *p, long i, float s) {
+        return __bfloat162float(p[i]) * s;
+    }
+    __device__ __forceinline__ static void st(__hip_bfloat16 *p, long i, float v, float inv) {
+        p[i] = __float2bfloat16(v * inv);
+    }
+};
+
+// 8th-order coefficients by reduced order, PSV/clean_nobox/param/const.h:13-18
+template <class C> struct Coef {
+    __device__ __forceinline__ static void get(int ord, C &c0, C &c1, C &c2, C &c3) {
+        c0 = ord == 4 ? C(1225. / 1024.) : ord == 3 ? C(75. / 64.) : ord == 2 ? C(9. / 8.) : C(1.0);
+        c1 = ord == 4 ? C(245. / 3072.) : ord == 3 ? C(25. / 384.) : ord == 2 ? C(1. / 24.) : C(0.0);
+        c2 = ord == 4 ? C(49. / 5120.) : ord == 3 ? C(3. / 640.) : C(0.0);
+        c3 = ord == 4 ? C(5. / 7168.) : C(0.0);
+    }
+};
+
+// reduced order of the reference (cu_fd.cu:165-169): the same scalar for
+// both derivative directions, so the top rows run at low order in x too.
+__device__ __forceinline__ int order_at(int ix, int iz, int nx, int nz) {
+    int o = min(ix, iz);
+    o = min(o, nx - 1 - ix);
+    o = min(o, nz - 1 - iz);
+    return min(o, 4);
+}
+
+// Every multiply-add in the kernels is an explicit fused mad and implicit
+// contraction is off (fd_kernels.hpp), so a cell's result does not depend on
+// how a tile shape unrolls: all tile shapes and batch sizes are bitwise
+// identical.
+__device__ __forceinline__ float mad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double mad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Derivative from a strided LDS window: "m" form centres between taps i-1
+// and i (UDIFF ux, DXDZTXZ dxTxz, DZTZZ), "p" form between i and i+1
+// (WDIFF wx, DXTXX, DZTXZ).  Same association as the reference macros,
+// ((c0 d0 - c1 d1) + c2 d2) - c3 d3, each step one fused mad.
+template <class C>
+__device__ __forceinline__ C dm(const C *f, int s, C c0, C c1, C c2, C c3) {
+    C r = c0 * (f[0] - f[-s]);
+    r = mad(-c1, f[s] - f[-2 * s], r);
+    r = mad(c2, f[2 * s] - f[-3 * s], r);
+    return mad(-c3, f[3 * s] - f[-4 * s], r);
+}
+template <class C>
+__device__ __forceinline__ C dp(const C *f, int s, C c0, C c1, C c2, C c3) {
+    C r = c0 * (f[s] - f[0]);
+    r = mad(-c1, f[2 * s] - f[-s], r);
+    r = mad(c2, f[3 * s] - f[-2 * s], r);
+    return mad(-c3, f[4 * s] - f[-3 * s], r);
+}
+
+// float bits ordered as unsigned for atomicMax of |v|
+__device__ __forceinline__ void amax_update(unsigned *slot, float v) {
+    atomicMax(slot, __float_as_uint(fabsf(v)));
+}
+
+// ---------------------------------------------------------------------------
+// argument blocks
+// ---------------------------------------------------------------------------
+struct Grid {
+    int nx, nz, px;     // domain and row pitch (elements)
+    long sstride;       // elements between shots in a field array
+    int npml, top;      // C-PML width, top absorbing flag
+    int nshots;
+    long xpsi_stride;   // elements between shots in an x-psi array (2*npml*nz)
+    long zpsi_stride;   // ... in a z-psi array (2*npml*nx)
+};
+
+template <class S> struct PsvFields {
+    S *U, *W, *Txx, *Tzz, *Txz; // point at shot 0, row 0, column 0
+};
+template <class S> struct ShFields {
+    S *V, *Tx, *Tz;
+};
+// C-PML profile triple (b, c, k=1/kappa)
+template <class C> struct Prof {
+    const C *b, *c, *k;
+};
+
+} // namespace fd23

@@ -1,0 +1,227 @@
+/*
+ * fd23.h -- C ABI of the MI355X-native (gfx950) staggered-grid elastic
+ * time-stepping library.
+ *
+ * The library replaces, behind plain C entry points, the calls that the
+ * reference's per-GPU driver `singlegpu()` makes into its CUDA code
+ * (PSV/clean_nobox/main.cpp:41-181, SH/main.cpp:38-142).  Each entry point
+ * below names the reference function it stands in for.
+ *
+ * Conventions
+ *   - Every function returns FD23_OK (0) or a negative status; none exits.
+ *     fd23_last_error() returns the message of the last failure on the
+ *     calling thread.
+ *   - The caller owns host buffers; a context owns its device buffers.
+ *   - A context is bound to one HIP stream and is not thread-safe; several
+ *     contexts may run concurrently on separate streams.
+ *   - Field values are physical velocities / stresses as in the reference;
+ *     material arrays are pre-scaled by dt/h as the reference's get_model
+ *     leaves them.
+ */
+#ifndef FD23_H
+#define FD23_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define FD23_OK 0
+#define FD23_EINVAL -1   /* bad argument or call order                   */
+#define FD23_ENOMEM -2   /* device allocation failed / over the HBM cap  */
+#define FD23_EHIP -3     /* HIP runtime error                            */
+#define FD23_EIO -4      /* file missing or unreadable                   */
+#define FD23_ESTAB -5    /* CFL stability number > 1                     */
+
+/* ---- enumerations ---------------------------------------------------- */
+#define FD23_WAVE_PSV 0 /* vx=U, vz=W, sxx=Txx, szz=Tzz, sxz=Txz         */
+#define FD23_WAVE_SH 1  /* vy=V, sxy=Tx, szy=Tz                          */
+
+#define FD23_STORE_F32 0
+#define FD23_STORE_F16 1  /* FP16 storage, per-field power-of-two scale  */
+#define FD23_STORE_BF16 2 /* BF16 storage, per-field power-of-two scale  */
+#define FD23_STORE_F64 3  /* double fields and material                  */
+
+/* source kinds (main.cpp:124-136 srctype, SH/main.cpp:100-112 stype)    */
+#define FD23_SRC_LINE 0  /* '1' / stype 3: stf[it] while it < lsrc       */
+#define FD23_SRC_P 1     /* 'p' / stype 13: running-sum stf, clamped     */
+#define FD23_SRC_FORCE 2 /* SH point force on V (cu_point_force)         */
+
+/* field ids for fd23_get_field / fd23_set_field                         */
+#define FD23_F_U 0
+#define FD23_F_W 1
+#define FD23_F_TXX 2
+#define FD23_F_TXZ 3
+#define FD23_F_TZZ 4
+#define FD23_F_V 0
+#define FD23_F_TX 1
+#define FD23_F_TZ 2
+
+typedef struct fd23_ctx fd23_ctx;
+
+typedef struct fd23_desc {
+    int wave;       /* FD23_WAVE_*                                        */
+    int storage;    /* FD23_STORE_*                                       */
+    int nx, nz;     /* grid; any size >= 16 (no nx%32 / nz%16 rule)       */
+    int npml;       /* C-PML width in cells                              */
+    int topabsorb;  /* 1 = C-PML on top instead of the free surface       */
+    int nshots;     /* shots sharing one model, stepped in one launch     */
+    int max_frames; /* capacity of the on-device seismogram buffer        */
+    int max_rec;    /* receivers per component per shot                   */
+    void *stream;   /* hipStream_t to run on, or NULL for an own stream   */
+} fd23_desc;
+
+/* ---- context --------------------------------------------------------
+ * Replaces FIELD::init_gpu_full (PSV/clean_nobox/field/cu_field.cu:7-23),
+ * MATERIAL::init_gpu_full (material/cu_material.cu:8-50), the CPML device
+ * constructor (cpml/cu_cpml.cu:5-90) and RECORD::cu_init_record
+ * (record/cu_record.cu:79-110).  Fields start at zero.                  */
+int fd23_ctx_create(const fd23_desc *desc, fd23_ctx **out);
+int fd23_ctx_destroy(fd23_ctx *ctx);
+const char *fd23_last_error(void);
+/* device bytes the context holds */
+size_t fd23_ctx_device_bytes(const fd23_ctx *ctx);
+/* device bytes fd23_ctx_create would allocate for desc (no allocation)  */
+size_t fd23_desc_device_bytes(const fd23_desc *desc);
+
+/* Material, already scaled by dt/h (material.cpp:203-211).  P-SV order
+ * BU, BW, MU, MUA, LAM; SH order BV, MUX, MUZ.  Each nx*nz, x fastest,
+ * float (or double for FD23_STORE_F64).  Replaces init_gpu_full's H2D.  */
+int fd23_set_material(fd23_ctx *ctx, const void *const *arrays);
+
+/* C-PML profiles as b, c, k(=1/kappa) triples, double, packed in order
+ * P-SV: Txx_x, Txz_x, U_x, W_x (nx each), Tzz_z, Txz_z, U_z, W_z (nz each);
+ * SH:   Tx_x, V_x (nx each), Tz_z, V_z (nz each);
+ * each profile stored as b[n], c[n], k[n].  Replaces CPML(deviceid, ...). */
+int fd23_set_cpml(fd23_ctx *ctx, const double *packed);
+
+/* Point source of one shot.  P-SV M = {Mxx, Mxz, Mzz}, SH M = {Mxy, Mzy},
+ * already multiplied by the reference's factor (cu_point_source.cu:37-40).
+ * stf has lsrc samples (the running sum for FD23_SRC_P, as main.cpp:94-99
+ * passes psrc).  Replaces cu_point_source / cu_point_source_p /
+ * cu_point_force (PSV/clean_nobox/point_source/point_source.h:6-7,
+ * SH/point_source/point_source.h:6-8).                                  */
+int fd23_set_source(fd23_ctx *ctx, int shot, int kind, int xs, int zs,
+                    const double *stf, int lsrc, const double *M);
+
+/* Receivers of one shot and component (P-SV 0=U 1=W, SH 0=V), sampled at
+ * the flat index rx + rz*nx (cu_record.cu:142).  Replaces cu_init_record. */
+int fd23_set_receivers(fd23_ctx *ctx, int shot, int comp, const int *rx,
+                       const int *rz, int n);
+
+/* ---- stepping --------------------------------------------------------
+ * One stage each, in the reference's order (main.cpp:121-150):
+ * cu_step_stress (fd/cu_fd.cu:391-483), source injection, cu_step_velocity
+ * (:485-588), RECORD::cu_record (record/cu_record.cu:154-170).  All shots
+ * of the context advance together.  Asynchronous on the context stream. */
+int fd23_step_stress(fd23_ctx *ctx);
+int fd23_inject(fd23_ctx *ctx, int it);
+int fd23_step_velocity(fd23_ctx *ctx);
+int fd23_record(fd23_ctx *ctx, int it, int itrecord);
+
+/* The whole loop for it in [it0, it0+nt): stress, inject, velocity,
+ * record every itrecord steps -- captured in HIP graphs and replayed.
+ * Equivalent to the singlegpu() loop body (main.cpp:115-168).           */
+int fd23_run(fd23_ctx *ctx, int it0, int nt, int itrecord);
+int fd23_sync(fd23_ctx *ctx);
+
+/* ---- outputs ---------------------------------------------------------
+ * Seismograms of one shot/component, time-major [nframes][n] floats
+ * (the raw {output}_U file layout, cu_record.cu:148-152).               */
+int fd23_get_seismograms(fd23_ctx *ctx, int shot, int comp, float *out,
+                         int nframes);
+/* One field of one shot as nx*nz values (float, or double for F64).
+ * Replaces cpy_backhost (record/cu_record.cu:119-135).                  */
+int fd23_get_field(fd23_ctx *ctx, int shot, int field, void *out);
+int fd23_set_field(fd23_ctx *ctx, int shot, int field, const void *in);
+/* Checkpoint in the state%06d layout (main.cpp:253-318): fields then the
+ * x-psi and z-psi arrays; values are float (double for F64).
+ * Replaces FIELD/CPML::cu_save_state and cu_load_restart.               */
+size_t fd23_state_bytes(const fd23_ctx *ctx);
+int fd23_get_state(fd23_ctx *ctx, int shot, void *buf, size_t bytes);
+int fd23_set_state(fd23_ctx *ctx, int shot, const void *buf, size_t bytes);
+
+/* ---- hybrid box coupling ---------------------------------------------
+ * Ring add/sub + residual sub-grid of PSV/psv_clean_double/box/box.cpp,
+ * generalised to a bx*bz rectangle whose drive is an analytic plane wave
+ * (incident=0 P, 1 SV; Ricker f0, delay t0; angle from vertical, waves
+ * travelling upward).  Box cells [x0, x0+bx) x [z0, z0+bz).  P-SV only. */
+int fd23_box_configure(fd23_ctx *ctx, int shot, int x0, int z0, int bx, int bz,
+                       int incident, double angle_deg, double f0, double t0,
+                       double amp, double vp, double vs, double rho,
+                       double h, double dt);
+
+/* ---- tuning and introspection ---------------------------------------- */
+/* tile shape index (0 = default); fd23_autotune times the candidates on
+ * this context's own arrays and keeps the fastest.  State is preserved. */
+int fd23_set_tile(fd23_ctx *ctx, int tile);
+int fd23_get_tile(const fd23_ctx *ctx);
+int fd23_num_tiles(void);
+int fd23_autotune(fd23_ctx *ctx, int steps, double *best_ms);
+/* algorithmic HBM bytes of one stress+velocity step for this context   */
+double fd23_bytes_per_step(const fd23_ctx *ctx);
+/* Time nsteps of stress+velocity passes with HIP events on the context
+ * stream; fills per-kernel average milliseconds.                        */
+int fd23_time_passes(fd23_ctx *ctx, int nsteps, double *stress_ms,
+                     double *velocity_ms);
+
+/* ---- host precompute (boundary code, no GPU) --------------------------
+ * C-PML profile (cpml.cpp:7-83), computed in double.                    */
+int fd23_cpml_profile(int n, int npml, double startpos, double pml_dt,
+                      double pml_r, double pml_v, double pml_fc, double *b,
+                      double *c, double *k);
+/* packed profiles for fd23_set_cpml, P-SV or SH (cpml.cpp:84-186)       */
+int fd23_cpml_pack(int wave, int nx, int nz, int npml, double pml_dt,
+                   double pml_r, double pml_v, double pml_fc, double *out);
+size_t fd23_cpml_pack_len(int wave, int nx, int nz);
+/* P-SV material (PSV/clean_nobox/material/material.cpp:35-240), float   */
+int fd23_psv_material(int nx, int nz, double h, double dt, const float *vp,
+                      const float *vs, const float *den, float *BU, float *BW,
+                      float *MU, float *MUA, float *LAM, double *stab);
+int fd23_psv_material_f64(int nx, int nz, double h, double dt,
+                          const double *vp, const double *vs,
+                          const double *den, double *BU, double *BW,
+                          double *MU, double *MUA, double *LAM, double *stab);
+/* SH material (SH/material/material.cpp:31-135)                         */
+int fd23_sh_material(int nx, int nz, double h, double dt, const float *vs,
+                     const float *den, float *BV, float *MUX, float *MUZ,
+                     double *stab);
+/* moment-tensor components incl. scaling (cu_point_source.cu:16-40);
+ * explosion=1 gives the -DEXPLOSION_SOURCE tensor                       */
+int fd23_psv_moment(int kind, int explosion, double strike, double dip,
+                    double rake, double azimuth, double h, double dt,
+                    double *M);
+int fd23_sh_moment(int kind, double strike, double dip, double rake,
+                   double azimuth, double h, double dt, double *M);
+/* source-time functions (source/waveform.cpp:8-47) and a Ricker        */
+int fd23_stf_gaussian(int alpha, double *out, int cap, int *lsrc);
+int fd23_stf_trapezoid(double dt, double trap1, double trap2, double trap3,
+                       double *out, int cap, int *lsrc);
+int fd23_stf_ricker(int n, double dt, double f0, double t0, double *out);
+/* model: "{name}.mdl" polygons (SH/model/model.cpp:175-469) or raster
+ * "{name}.vp/.vs/.den" float32 files (SH/model/util.cpp:12-29)          */
+int fd23_model_read(const char *name, int nx, int nz, double h, float *vp,
+                    float *vs, float *den);
+/* ISIS trace file (PSV/clean_nobox/record/isis.cpp:8-61): raw is
+ * time-major [ntrecord][nrec]                                           */
+int fd23_isis_write(const char *path, const float *raw, int nrec,
+                    int ntrecord, const int *rx, const int *rz, double shiftx,
+                    double shiftz, double h, double dt, int itrecord, int xs,
+                    int zs);
+
+/* ---- par files (Caltech getpar semantics, getpar/getpar.c) ----------- */
+typedef struct fd23_par fd23_par;
+int fd23_par_open(int argc, const char *const *argv, fd23_par **out);
+void fd23_par_close(fd23_par *p);
+/* return 1 if found (last occurrence wins), 0 if not, <0 on error       */
+int fd23_par_int(fd23_par *p, const char *name, int *val);
+int fd23_par_float(fd23_par *p, const char *name, float *val);
+int fd23_par_double(fd23_par *p, const char *name, double *val);
+int fd23_par_string(fd23_par *p, const char *name, char *buf, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FD23_H */

@@ -1,0 +1,109 @@
+"""Shared set-up for parity tests: one problem description, built once by
+the library's host code, fed both to the GPU context and to the oracle."""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle as O
+from fd23_amd import _lib, host
+from fd23_amd.core import Context
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
+
+
+def layered_model(nx, nz, seed, vmin=1800.0, vmax=6000.0, nlayer=6, sigma=0.03):
+    rng = np.random.default_rng(seed)
+    depth = np.sort(rng.integers(1, nz, nlayer - 1))
+    vps = rng.uniform(vmin, vmax, nlayer)
+    vp = np.empty((nz, nx), np.float64)
+    edges = [0] + list(depth) + [nz]
+    for i in range(nlayer):
+        vp[edges[i]:edges[i + 1], :] = vps[i]
+    vp *= 1.0 + sigma * rng.standard_normal((nz, nx))
+    vs = vp / 1.732
+    den = 310.0 * vp ** 0.25
+    return vp.astype(np.float32), vs.astype(np.float32), den.astype(np.float32)
+
+
+def unpack_cpml(packed, wave, nx, nz, dtype=np.float32):
+    names = (O.PSV_CPML_X + O.PSV_CPML_Z) if wave == _lib.WAVE_PSV else ("Tx_x", "V_x", "Tz_z", "V_z")
+    half = len(names) // 2
+    out, off = {}, 0
+    for i, name in enumerate(names):
+        n = nx if i < half else nz
+        out[name] = tuple(packed[off + j * n: off + (j + 1) * n].astype(dtype) for j in range(3))
+        off += 3 * n
+    return out
+
+
+class Problem:
+    """A single- or multi-shot problem with library-precomputed inputs."""
+
+    def __init__(self, wave, nx, nz, npml, h, dt, nt, seed=1, topabsorb=False, kind=_lib.SRC_LINE,
+                 explosion=False, sources=None, rec_z=5, f0=None, f64=False, vmax=6000.0):
+        self.wave, self.nx, self.nz, self.npml = wave, nx, nz, npml
+        self.h, self.dt, self.nt = h, dt, nt
+        self.topabsorb, self.kind, self.f64 = topabsorb, kind, f64
+        vp, vs, den = layered_model(nx, nz, seed, vmax=vmax)
+        if f64:
+            vp, vs, den = (a.astype(np.float64) for a in (vp, vs, den))
+        if wave == _lib.WAVE_PSV:
+            self.mat, self.stab = host.psv_material(vp, vs, den, h, dt, f64=f64)
+        else:
+            self.mat, self.stab = host.sh_material(vs, den, h, dt)
+        pml_v = float(vp.max()) / h
+        f0 = f0 or 12.0
+        self.packed = host.cpml_pack(wave, nx, nz, npml, dt, 1e-6, pml_v, f0)
+        stf = host.ricker(nt, dt, f0)
+        if kind == _lib.SRC_P:
+            stf = np.cumsum(stf.astype(np.float32), dtype=np.float32).astype(np.float64)
+        self.stf = stf
+        if wave == _lib.WAVE_PSV:
+            self.M = host.psv_moment(kind, 30.0, 60.0, 45.0, 10.0, h, dt, explosion=explosion)
+        else:
+            self.M = host.sh_moment(kind if kind != _lib.SRC_FORCE else _lib.SRC_LINE, 30.0, 60.0, 45.0,
+                                    10.0, h, dt)
+        self.sources = sources or [(nx // 2, 10)]
+        self.rx = np.arange(1, nx, 2, dtype=np.int32)
+        self.rz = np.full(self.rx.size, rec_z, np.int32)
+
+    @property
+    def nshots(self):
+        return len(self.sources)
+
+    def context(self, storage=None, max_frames=None):
+        if storage is None:
+            storage = _lib.STORE_F64 if self.f64 else _lib.STORE_F32
+        ctx = Context(self.wave, self.nx, self.nz, self.npml, nshots=self.nshots, storage=storage,
+                      topabsorb=self.topabsorb, max_frames=self.nt if max_frames is None else max_frames,
+                      max_rec=self.rx.size)
+        ctx.set_material(self.mat)
+        ctx.set_cpml(self.packed)
+        ncomp = 2 if self.wave == _lib.WAVE_PSV else 1
+        for s, (xs, zs) in enumerate(self.sources):
+            ctx.set_source(s, self.kind, xs, zs, self.stf, self.M)
+            for c in range(ncomp):
+                ctx.set_receivers(s, c, self.rx, self.rz)
+        return ctx
+
+    def oracle(self, shot=0):
+        dt_ = np.float64 if self.f64 else np.float32
+        cp = unpack_cpml(self.packed, self.wave, self.nx, self.nz, dt_)
+        xs, zs = self.sources[shot]
+        if self.wave == _lib.WAVE_PSV:
+            mat = dict(zip(("BU", "BW", "MU", "MUA", "LAM"), self.mat))
+            sim = O.PSV(self.nx, self.nz, self.npml, mat, cp, self.topabsorb, dt_)
+            okind = {(_lib.SRC_LINE, False): 0, (_lib.SRC_P, False): 1}.get((self.kind, False), 0)
+            u, w = O.run_psv(sim, self.nt, self.stf.astype(dt_), self.M.astype(dt_), xs, zs, kind=okind,
+                             rx=self.rx, rz=self.rz)
+            return sim, [u, w]
+        mat = dict(zip(("BV", "MUX", "MUZ"), self.mat))
+        sim = O.SH(self.nx, self.nz, self.npml, mat, cp, self.topabsorb, dt_)
+        v = O.run_sh(sim, self.nt, self.stf.astype(dt_), self.M.astype(dt_), xs, zs, kind=self.kind,
+                     rx=self.rx, rz=self.rz)
+        return sim, [v]

@@ -1,0 +1,210 @@
+"""HIP path vs the oracle and vs the reference's own checkpoint fixtures.
+
+All calls go through libfd23.so's C ABI.  Tolerances: 1e-5 relative L2 on
+seismograms and fields for FP32 (the bar the north-star sets), 1e-6 on
+the reference's step-0/1 checkpoints, 1e-10 for FP64 storage vs the FP64
+oracle.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from fd23_amd import _lib, host
+from fd23_amd.core import Context
+from helpers import Problem, rel
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref")
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# reference checkpoints: PSV/clean_nobox/TEST/tt (96x96, npml 32, top C-PML)
+# ---------------------------------------------------------------------------
+def _tt_context():
+    n, npml, h, dt = 96, 32, 0.3, 0.01
+    vp, vs, den = host.model_read(os.path.join(GOLD, "psv_tt", "homo_6.8.mdl"), n, n, h)
+    mat, stab = host.psv_material(vp, vs, den, h, dt)
+    packed = host.cpml_pack(_lib.WAVE_PSV, n, n, npml, 0.005, 1e-11, 30.0, 2.0)
+    ctx = Context(_lib.WAVE_PSV, n, n, npml, topabsorb=True, max_frames=1000, max_rec=2)
+    ctx.set_material(mat)
+    ctx.set_cpml(packed)
+    stf = host.gaussian(-30)
+    M = host.psv_moment(_lib.SRC_LINE, 0, 20, 90, 45, h, dt)
+    ctx.set_source(0, _lib.SRC_LINE, 33, 33, stf, M)
+    return ctx
+
+
+def _state(name):
+    return np.fromfile(os.path.join(GOLD, "psv_tt", name), np.float32)
+
+
+def test_reference_checkpoints_step0_step1():
+    _print("case: tt checkpoints step 0/1")
+    with _tt_context() as ctx:
+        for it, name in ((0, "state000000"), (1, "state000001")):
+            ctx.step_stress()
+            ctx.inject(it)
+            ctx.step_velocity()
+            got = np.frombuffer(ctx.get_state(), np.float32)
+            err = rel(got, _state(name))
+            _print(name, err)
+            assert err < 1e-6
+
+
+def test_reference_checkpoint_500_steps_and_restart():
+    _print("case: tt 500 steps + restart")
+    with _tt_context() as ctx:
+        ctx.run(0, 500, 1)
+        got = np.frombuffer(ctx.get_state(), np.float32)
+        err = rel(got, _state("state000499"))
+        _print("state000499", err)
+        assert err < 1e-5
+    with _tt_context() as ctx:
+        ctx.set_state(_state("state000499").tobytes())
+        ctx.run(500, 500, 1)
+        got = np.frombuffer(ctx.get_state(), np.float32)
+        err = rel(got, _state("state000999"))
+        _print("state000999", err)
+        assert err < 1e-5
+
+
+# ---------------------------------------------------------------------------
+# oracle parity at non-multiple sizes, all edge branches
+# ---------------------------------------------------------------------------
+PSV_CASES = [
+    # nx, nz, npml, top, kind, explosion
+    (150, 90, 12, False, _lib.SRC_LINE, False),
+    (131, 77, 10, True, _lib.SRC_LINE, False),
+    (200, 64, 16, False, _lib.SRC_P, False),
+    (97, 120, 8, True, _lib.SRC_LINE, True),
+]
+
+
+@pytest.mark.parametrize("nx,nz,npml,top,kind,expl", PSV_CASES)
+def test_psv_vs_oracle(nx, nz, npml, top, kind, expl):
+    _print("case: psv", nx, nz, npml, top, kind, expl)
+    # receivers below the frozen top C-PML rows when the top absorbs
+    p = Problem(_lib.WAVE_PSV, nx, nz, npml, 10.0, 7e-4, 400, seed=nx, topabsorb=top, kind=kind,
+                explosion=expl, sources=[(nx // 3, 6)], rec_z=npml + 3 if top else 3)
+    with p.context() as ctx:
+        ctx.run(0, p.nt, 1)
+        gu, gw = ctx.seismograms(0, 0, p.nt), ctx.seismograms(0, 1, p.nt)
+        fields = {n: ctx.field(n) for n in ("U", "W", "Txx", "Txz", "Tzz")}
+        st = np.frombuffer(ctx.get_state(), np.float32)
+    sim, (ou, ow) = p.oracle()
+    eu, ew = rel(gu, ou), rel(gw, ow)
+    _print("seis", eu, ew)
+    assert eu < 1e-5 and ew < 1e-5
+    for n, f in fields.items():
+        e = rel(f, getattr(sim, n))
+        _print(n, e)
+        assert e < 1e-5
+    ost = np.frombuffer(sim.state_bytes(), np.float32)
+    assert rel(st, ost) < 1e-5
+
+
+SH_CASES = [
+    (150, 90, 12, False, _lib.SRC_LINE),
+    (131, 77, 10, True, _lib.SRC_P),
+    (120, 100, 9, True, _lib.SRC_FORCE),
+]
+
+
+@pytest.mark.parametrize("nx,nz,npml,top,kind", SH_CASES)
+def test_sh_vs_oracle(nx, nz, npml, top, kind):
+    _print("case: sh", nx, nz, npml, top, kind)
+    p = Problem(_lib.WAVE_SH, nx, nz, npml, 10.0, 7e-4, 400, seed=nx, topabsorb=top, kind=kind,
+                sources=[(nx // 2, 8)], rec_z=npml + 2 if top else 2)
+    with p.context() as ctx:
+        ctx.run(0, p.nt, 1)
+        gv = ctx.seismograms(0, 0, p.nt)
+        fields = {n: ctx.field(n) for n in ("V", "Tx", "Tz")}
+    sim, (ov,) = p.oracle()
+    e = rel(gv, ov)
+    _print("seis", e)
+    assert e < 1e-5
+    for n, f in fields.items():
+        e = rel(f, getattr(sim, n))
+        _print(n, e)
+        assert e < 1e-5
+
+
+def test_psv_f64_storage_vs_f64_oracle():
+    _print("case: psv f64")
+    p = Problem(_lib.WAVE_PSV, 140, 80, 10, 10.0, 7e-4, 150, seed=3, f64=True, sources=[(70, 6)])
+    with p.context() as ctx:
+        ctx.run(0, p.nt, 1)
+        gu, gw = ctx.seismograms(0, 0, p.nt), ctx.seismograms(0, 1, p.nt)
+    sim, (ou, ow) = p.oracle()
+    _print(rel(gu, ou), rel(gw, ow))
+    assert rel(gu, ou) < 1e-10 and rel(gw, ow) < 1e-10
+
+
+# ---------------------------------------------------------------------------
+# internal consistency: graph loop == staged calls, tiles agree, batch == solo
+# ---------------------------------------------------------------------------
+def test_graph_run_equals_staged_calls():
+    _print("case: graph vs staged")
+    p = Problem(_lib.WAVE_PSV, 130, 70, 10, 10.0, 7e-4, 70, seed=5, sources=[(60, 6)])
+    with p.context() as a, p.context() as b:
+        a.run(0, p.nt, 2)
+        for it in range(p.nt):
+            b.step_stress()
+            b.inject(it)
+            b.step_velocity()
+            b.record(it, 2)
+        assert np.array_equal(a.get_state(), b.get_state())
+        fr = (p.nt + 1) // 2
+        assert np.array_equal(a.seismograms(0, 0, fr), b.seismograms(0, 0, fr))
+
+
+def test_all_tiles_bitwise_equal():
+    _print("case: tiles")
+    p = Problem(_lib.WAVE_PSV, 150, 90, 12, 10.0, 7e-4, 40, seed=9, sources=[(40, 6), (100, 9)])
+    ref = None
+    ntile = _lib.load().fd23_num_tiles()
+    for t in range(ntile):
+        with p.context() as ctx:
+            ctx.tile = t
+            ctx.run(0, p.nt, 1)
+            st = ctx.get_state(0) + ctx.get_state(1)
+        if ref is None:
+            ref = st
+        _print("tile", t, st == ref)
+        assert st == ref
+
+
+def test_batch_equals_single_shots():
+    _print("case: batch vs solo")
+    srcs = [(20 + 13 * k, 6 + k % 3) for k in range(9)]
+    p = Problem(_lib.WAVE_PSV, 160, 80, 10, 10.0, 7e-4, 60, seed=11, sources=srcs)
+    with p.context() as ctx:
+        ctx.tile = 4  # 4 shots per workgroup, 9 shots: ragged last group
+        ctx.run(0, p.nt, 1)
+        batch = [ctx.get_state(s) for s in range(len(srcs))]
+        bseis = [ctx.seismograms(s, 1, p.nt) for s in range(len(srcs))]
+    for s, src in enumerate(srcs):
+        q = Problem(_lib.WAVE_PSV, 160, 80, 10, 10.0, 7e-4, 60, seed=11, sources=[src])
+        with q.context() as ctx:
+            ctx.run(0, q.nt, 1)
+            assert ctx.get_state(0) == batch[s]
+            assert np.array_equal(ctx.seismograms(0, 1, q.nt), bseis[s])
+
+
+def test_autotune_preserves_state():
+    _print("case: autotune")
+    p = Problem(_lib.WAVE_PSV, 150, 90, 12, 10.0, 7e-4, 50, seed=2, sources=[(70, 6)] * 4)
+    with p.context() as a, p.context() as b:
+        a.run(0, 25, 1)
+        b.run(0, 25, 1)
+        tile, ms = a.autotune(2)
+        _print("best tile", tile, ms)
+        a.run(25, 25, 1)
+        b.run(25, 25, 1)
+        assert a.get_state(2) == b.get_state(2)
