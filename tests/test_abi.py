@@ -1,0 +1,52 @@
+"""The C ABI library loads and exports every entry point include/fd23.h
+declares (CPU: no compute calls)."""
+import ctypes
+import os
+import re
+
+from fd23_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "fd23.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fd23_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "fd23_ctx_create" in names and "fd23_run" in names and len(names) > 40
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header():
+    names = set(declared_functions())
+    assert names <= set(_lib.SIGNATURES), sorted(names - set(_lib.SIGNATURES))
+
+
+def test_invalid_descriptor_is_rejected_without_gpu_work():
+    lib = _lib.load()
+    d = _lib.Desc(7, 0, 100, 100, 10, 0, 1, 0, 0, None)
+    h = ctypes.c_void_p()
+    assert lib.fd23_ctx_create(ctypes.byref(d), ctypes.byref(h)) == _lib.EINVAL
+    assert b"wave" in lib.fd23_last_error()
+    d = _lib.Desc(0, 0, 100, 100, 60, 0, 1, 0, 0, None)
+    assert lib.fd23_ctx_create(ctypes.byref(d), ctypes.byref(h)) == _lib.EINVAL
+
+
+def test_device_bytes_estimate():
+    from fd23_amd.core import desc_device_bytes
+
+    one = desc_device_bytes(_lib.WAVE_PSV, 1024, 512, 32)
+    many = desc_device_bytes(_lib.WAVE_PSV, 1024, 512, 32, nshots=64)
+    half = desc_device_bytes(_lib.WAVE_PSV, 1024, 512, 32, nshots=64, storage=_lib.STORE_F16)
+    assert 5 * 1024 * 512 * 4 < one < 20 * 1024 * 512 * 4
+    # the material is shared by the shots of a context
+    assert 30 * one < many < 64 * one and half < 0.6 * many
