@@ -56,6 +56,7 @@ SIGNATURES = {
     "fd23_set_state": (c_int, [c_vp, c_int, c_vp, c_sz]),
     "fd23_box_configure": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_dbl, c_dbl, c_dbl,
                                    c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl]),
+    "fd23_box_get_residual": (c_int, [c_vp, c_int, c_vp]),
     "fd23_set_tile": (c_int, [c_vp, c_int]),
     "fd23_get_tile": (c_int, [c_vp]),
     "fd23_num_tiles": (c_int, []),
@@ -84,7 +85,6 @@ SIGNATURES = {
     "fd23_par_float": (c_int, [c_vp, ctypes.c_char_p, P_flt]),
     "fd23_par_double": (c_int, [c_vp, ctypes.c_char_p, P_dbl]),
     "fd23_par_string": (c_int, [c_vp, ctypes.c_char_p, ctypes.c_char_p, c_int]),
-    # internal (not in the public header): FP16/BF16 scale control
     "fd23_set_scales": (c_int, [c_vp, c_vp, c_int]),
     "fd23_get_scales": (c_int, [c_vp, c_vp, c_vp, c_int]),
 }

@@ -152,9 +152,18 @@ class Context:
         check(self.lib.fd23_set_state(self.h, shot, _ptr(buf), buf.size))
 
     # -- box ---------------------------------------------------------------
-    def box_configure(self, shot, x0, z0, bx, bz, incident, angle_deg, f0, t0, amp, vp, vs, rho, h, dt):
-        check(self.lib.fd23_box_configure(self.h, shot, x0, z0, bx, bz, incident, angle_deg, f0, t0, amp,
+    def box_configure(self, x0, z0, bx, bz, incident, angle_deg, f0, t0, amp, vp, vs, rho, h, dt):
+        """Plane-wave box on every shot (incident 0 = P, 1 = SV)."""
+        check(self.lib.fd23_box_configure(self.h, -1, x0, z0, bx, bz, incident, angle_deg, f0, t0, amp,
                                           vp, vs, rho, h, dt))
+        self._box_shape = (bz, bx)
+
+    def box_residual(self, shot=0):
+        """Residual grid of the box: dict of U W Txx Tzz Txz (bz x bx)."""
+        bz, bx = self._box_shape
+        out = np.zeros((5, bz, bx), self.cdtype)
+        check(self.lib.fd23_box_get_residual(self.h, shot, _ptr(out)))
+        return dict(zip(("U", "W", "Txx", "Tzz", "Txz"), out))
 
     # -- tuning / measurement ----------------------------------------------
     @property

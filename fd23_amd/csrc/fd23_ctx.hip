@@ -87,7 +87,10 @@ struct fd23_ctx {
     int *d_compfield = nullptr;
     float *scl = nullptr;
     unsigned *amax = nullptr;
+    float *resc = nullptr;
     int *clock = nullptr;
+    bool scales_init = false;
+    double bmax = 0;   // largest buoyancy-type material value (scale estimate)
 
     // sources
     std::vector<HostSrc> src;
@@ -96,6 +99,16 @@ struct fd23_ctx {
     double *d_stf = nullptr;
     bool src_dirty = true;
     size_t taps_cap = 0, stf_cap = 0;
+    std::vector<Tap> h_taps;
+    std::vector<SrcInfo> h_srcinfo;
+
+    // per-tile lists of the fused graph loop
+    int *d_tap_start = nullptr, *d_rec_start = nullptr;
+    TapT *d_tapsT = nullptr;
+    RecT *d_recs = nullptr;
+    size_t tap_start_cap = 0, rec_start_cap = 0, tapsT_cap = 0, recs_cap = 0;
+    bool events_dirty = true;
+    int events_tile = -1;
 
     // receivers
     long *d_roff = nullptr;
@@ -115,10 +128,11 @@ struct fd23_ctx {
 };
 
 // implemented in fd23_box.hip
-int fd23_box_launch_pre_stress(fd23_ctx *c);
-int fd23_box_launch_post_stress(fd23_ctx *c);
-int fd23_box_launch_post_velocity(fd23_ctx *c);
+int fd23_box_launch_pre_stress(fd23_ctx *c, int k);
+int fd23_box_launch_post_stress(fd23_ctx *c, int k);
+int fd23_box_launch_post_velocity(fd23_ctx *c, int k);
 void fd23_box_free(fd23_ctx *c);
+void fd23_box_set_step(fd23_ctx *c, int it);
 
 // accessors used by the box module
 hipStream_t fd23_ctx_stream(fd23_ctx *c) { return c->stream; }
@@ -176,8 +190,9 @@ static void free_ctx(fd23_ctx *c) {
     for (auto p : c->field_alloc) if (p) hipFree(p);
     for (auto p : c->psi) if (p) hipFree(p);
     for (auto p : c->mat) if (p) hipFree(p);
-    void *others[] = {c->prof, c->d_fieldtab, c->d_comptab, c->d_compfield, c->scl, c->amax, c->clock,
-                      c->d_taps, c->d_src, c->d_stf, c->d_roff, c->d_nrec, c->seis};
+    void *others[] = {c->prof, c->d_fieldtab, c->d_comptab, c->d_compfield, c->scl, c->amax, c->resc, c->clock,
+                      c->d_taps, c->d_src, c->d_stf, c->d_roff, c->d_nrec, c->seis, c->d_tap_start,
+                      c->d_rec_start, c->d_tapsT, c->d_recs};
     for (auto p : others) if (p) hipFree(p);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -260,6 +275,7 @@ extern "C" int fd23_ctx_create(const fd23_desc *desc, fd23_ctx **out) {
     ALLOC(c->d_compfield, sizeof(int) * 2);
     ALLOC(c->scl, sizeof(float) * 16);
     ALLOC(c->amax, sizeof(unsigned) * 16);
+    ALLOC(c->resc, sizeof(float) * 16);
     ALLOC(c->clock, sizeof(int) * 2);
     ALLOC(c->d_src, sizeof(SrcInfo) * S);
     ALLOC(c->d_nrec, sizeof(int) * S * c->ncomp);
@@ -275,6 +291,7 @@ extern "C" int fd23_ctx_create(const fd23_desc *desc, fd23_ctx **out) {
     hipMemcpy(c->d_compfield, cf, sizeof cf, hipMemcpyHostToDevice);
     std::vector<float> ones(16, 1.0f);
     hipMemcpy(c->scl, ones.data(), sizeof(float) * 16, hipMemcpyHostToDevice);
+    hipMemcpy(c->resc, ones.data(), sizeof(float) * 16, hipMemcpyHostToDevice);
     hipMemcpy(c->d_roff, c->h_roff.data(), sizeof(long) * c->h_roff.size(), hipMemcpyHostToDevice);
     c->tile = (d.nshots >= 4) ? 4 : 0;
     hipError_t e = hipDeviceSynchronize();
@@ -304,6 +321,14 @@ extern "C" int fd23_set_material(fd23_ctx *c, const void *const *arrays) {
                                 (size_t)c->d.nx * c->csize, c->d.nz, hipMemcpyHostToDevice, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
+    // largest BU/BW (P-SV) or BV (SH): a velocity is about bmax times a stress
+    const int nb = is_psv(c) ? 2 : 1;
+    double bm = 0;
+    const size_t N = (size_t)c->d.nx * c->d.nz;
+    for (int m = 0; m < nb; m++)
+        for (size_t q = 0; q < N; q++)
+            bm = std::max(bm, std::fabs(c->csize == 8 ? ((const double *)arrays[m])[q] : (double)((const float *)arrays[m])[q]));
+    c->bmax = bm;
     c->have_mat = true;
     return FD23_OK;
 }
@@ -329,6 +354,7 @@ extern "C" int fd23_set_scales(fd23_ctx *c, const float *scales, int n) {
     if (!c || !scales || n < 1 || n > 16) return fail(FD23_EINVAL, "bad scales");
     HIPCHK(hipMemcpyAsync(c->scl, scales, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->scales_init = true;
     return FD23_OK;
 }
 extern "C" int fd23_get_scales(fd23_ctx *c, float *scales, unsigned *amax_bits, int n) {
@@ -428,7 +454,9 @@ static int upload_sources(fd23_ctx *c) {
         si.ntap = (int)taps.size() - si.tap0;
         info[s] = si;
     }
-    if (taps.size() > c->taps_cap) {
+    c->h_taps = taps;
+    c->h_srcinfo = info;
+    if (taps.size() > c->taps_cap || !c->d_taps) {
         if (c->d_taps) hipFree(c->d_taps);
         c->taps_cap = std::max<size_t>(taps.size(), 16);
         HIPCHK(hipMalloc((void **)&c->d_taps, sizeof(Tap) * c->taps_cap));
@@ -438,10 +466,6 @@ static int upload_sources(fd23_ctx *c) {
         c->stf_cap = std::max<size_t>(stf.size(), 16);
         HIPCHK(hipMalloc((void **)&c->d_stf, sizeof(double) * c->stf_cap));
     }
-    if (!c->d_taps) {
-        HIPCHK(hipMalloc((void **)&c->d_taps, sizeof(Tap) * 16));
-        c->taps_cap = 16;
-    }
     if (!taps.empty())
         HIPCHK(hipMemcpyAsync(c->d_taps, taps.data(), sizeof(Tap) * taps.size(), hipMemcpyHostToDevice, c->stream));
     if (!stf.empty())
@@ -449,9 +473,7 @@ static int upload_sources(fd23_ctx *c) {
     HIPCHK(hipMemcpyAsync(c->d_src, info.data(), sizeof(SrcInfo) * info.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->src_dirty = false;
-    // a graph holds the old pointers if the buffers moved
-    for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
-    c->graphs.clear();
+    c->events_dirty = true;
     return FD23_OK;
 }
 
@@ -481,6 +503,81 @@ extern "C" int fd23_set_receivers(fd23_ctx *c, int shot, int comp, const int *rx
     HIPCHK(hipMemcpyAsync(c->d_nrec, c->h_nrec.data(), sizeof(int) * c->h_nrec.size(), hipMemcpyHostToDevice,
                           c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->events_dirty = true;
+    return FD23_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-tile event lists for the fused injection / gather of the graph loop
+// ---------------------------------------------------------------------------
+static int tile_rows(const fd23_ctx *c) { return BY * kTiles[c->tile].rz; }
+
+template <class T> static int upload_vec(fd23_ctx *c, T *&dev, size_t &cap, const std::vector<T> &v) {
+    if (v.size() > cap || !dev) {
+        if (dev) hipFree(dev);
+        cap = std::max<size_t>(v.size(), 16);
+        HIPCHK(hipMalloc((void **)&dev, sizeof(T) * cap));
+    }
+    if (!v.empty()) HIPCHK(hipMemcpyAsync(dev, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, c->stream));
+    return FD23_OK;
+}
+
+static int build_events(fd23_ctx *c) {
+    if (!c->events_dirty && c->events_tile == c->tile) return FD23_OK;
+    const int TZ = tile_rows(c);
+    const int ntx = (c->d.nx + TX - 1) / TX, ntz = (c->d.nz + TZ - 1) / TZ;
+    const int S = c->d.nshots;
+    const size_t nkey = (size_t)ntx * ntz * S;
+    std::vector<std::vector<TapT>> tb(nkey);
+    std::vector<std::vector<RecT>> rb(nkey);
+    auto locate = [&](long off, int &key_tile, int &cell) {
+        const int z = (int)(off / c->px), x = (int)(off % c->px);
+        const int tx_ = x / TX, tz_ = z / TZ;
+        key_tile = tz_ * ntx + tx_;
+        cell = (z - tz_ * TZ) * TX + (x - tx_ * TX);
+    };
+    for (int s = 0; s < S; s++) {
+        const SrcInfo &si = c->h_srcinfo[s];
+        for (int i = 0; i < si.ntap; i++) {
+            const Tap &tp = c->h_taps[si.tap0 + i];
+            int tile, cell;
+            locate(tp.off, tile, cell);
+            tb[(size_t)tile * S + s].push_back(TapT{cell, tp.field, tp.m, tp.q, tp.mode});
+        }
+        for (int comp = 0; comp < c->ncomp; comp++) {
+            const int n = c->h_nrec[(size_t)s * c->ncomp + comp];
+            const long base = ((long)s * c->ncomp + comp) * c->d.max_rec;
+            for (int r = 0; r < n; r++) {
+                const long off = c->h_roff[base + r];
+                if (off < 0) continue; // outside the grid: the frame keeps 0
+                int tile, cell;
+                locate(off, tile, cell);
+                rb[(size_t)tile * S + s].push_back(RecT{cell, comp, (int)(base + r)});
+            }
+        }
+    }
+    std::vector<int> ts(nkey + 1), rs(nkey + 1);
+    std::vector<TapT> taps;
+    std::vector<RecT> recs;
+    for (size_t k = 0; k < nkey; k++) {
+        ts[k] = (int)taps.size();
+        rs[k] = (int)recs.size();
+        taps.insert(taps.end(), tb[k].begin(), tb[k].end());
+        recs.insert(recs.end(), rb[k].begin(), rb[k].end());
+    }
+    ts[nkey] = (int)taps.size();
+    rs[nkey] = (int)recs.size();
+    int r;
+    if ((r = upload_vec(c, c->d_tap_start, c->tap_start_cap, ts))) return r;
+    if ((r = upload_vec(c, c->d_rec_start, c->rec_start_cap, rs))) return r;
+    if ((r = upload_vec(c, c->d_tapsT, c->tapsT_cap, taps))) return r;
+    if ((r = upload_vec(c, c->d_recs, c->recs_cap, recs))) return r;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->events_dirty = false;
+    c->events_tile = c->tile;
+    // graphs hold the old list pointers
+    for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
     return FD23_OK;
 }
 
@@ -507,114 +604,144 @@ template <class C> static Prof<C> prof_of(const fd23_ctx *c, int p) {
     return Prof<C>{base, base + n, base + 2 * n};
 }
 
-template <class S, int RZ, int SPB> static void launch_stress_t(fd23_ctx *c) {
+static Work make_work(const fd23_ctx *c, int RZ, int SPB, dim3 &grid) {
+    Work w;
+    w.ntx = (c->d.nx + TX - 1) / TX;
+    w.ntz = (c->d.nz + BY * RZ - 1) / (BY * RZ);
+    w.ngroups = (c->d.nshots + SPB - 1) / SPB;
+    w.nwork = w.ntx * w.ntz * w.ngroups;
+    grid = dim3(round_up(w.nwork, 8), 1, 1);
+    return w;
+}
+
+// fused events for step k of a captured chunk (k < 0: none)
+static Events make_events(const fd23_ctx *c, int k, int itrecord) {
+    Events e{};
+    if (k < 0) return e;
+    e.clock = c->clock;
+    e.k = k;
+    e.itrecord = itrecord;
+    e.tap_start = c->d_tap_start;
+    e.taps = c->d_tapsT;
+    e.src = c->d_src;
+    e.stf = c->d_stf;
+    e.rec_start = c->d_rec_start;
+    e.recs = c->d_recs;
+    e.seis = c->seis;
+    e.max_frames = c->seis ? c->d.max_frames : 0;
+    e.frame_size = (long)c->d.nshots * c->ncomp * c->d.max_rec;
+    return e;
+}
+
+template <class S, int RZ, int SPB> static void launch_stress_t(fd23_ctx *c, const Events &ev) {
     using C = typename Store<S>::C;
     const Grid g = make_grid(c);
+    dim3 grid;
+    const Work w = make_work(c, RZ, SPB, grid);
     dim3 block(TX, BY);
-    dim3 grid((g.nx + TX - 1) / TX, (g.nz + BY * RZ - 1) / (BY * RZ), (g.nshots + SPB - 1) / SPB);
     if (is_psv(c)) {
         PsvFields<S> f{(S *)c->fields[0], (S *)c->fields[1], (S *)c->fields[2], (S *)c->fields[3],
                        (S *)c->fields[4]};
         PsvStressMat<C> m{(const C *)c->mat[4], (const C *)c->mat[2], (const C *)c->mat[3]};
         PsvStressPsi<S> ps{(S *)c->psi[2], (S *)c->psi[3], (S *)c->psi[6], (S *)c->psi[7]};
         PsvStressProf<C> pr{prof_of<C>(c, 2), prof_of<C>(c, 3), prof_of<C>(c, 6), prof_of<C>(c, 7)};
-        hipLaunchKernelGGL((k_psv_stress<S, RZ, SPB>), grid, block, 0, c->stream, f, m, ps, pr, g, c->scl,
+        hipLaunchKernelGGL((k_psv_stress<S, RZ, SPB>), grid, block, 0, c->stream, f, m, ps, pr, g, w, ev, c->scl,
                            c->amax);
     } else {
         ShFields<S> f{(S *)c->fields[0], (S *)c->fields[1], (S *)c->fields[2]};
         ShStressMat<C> m{(const C *)c->mat[1], (const C *)c->mat[2]};
         ShPsi2<S> ps{(S *)c->psi[1], (S *)c->psi[3]};
         ShProf2<C> pr{prof_of<C>(c, 1), prof_of<C>(c, 3)};
-        hipLaunchKernelGGL((k_sh_stress<S, RZ, SPB>), grid, block, 0, c->stream, f, m, ps, pr, g, c->scl,
+        hipLaunchKernelGGL((k_sh_stress<S, RZ, SPB>), grid, block, 0, c->stream, f, m, ps, pr, g, w, ev, c->scl,
                            c->amax);
     }
 }
 
-template <class S, int RZ, int SPB> static void launch_velocity_t(fd23_ctx *c) {
+template <class S, int RZ, int SPB> static void launch_velocity_t(fd23_ctx *c, const Events &ev) {
     using C = typename Store<S>::C;
     const Grid g = make_grid(c);
+    dim3 grid;
+    const Work w = make_work(c, RZ, SPB, grid);
     dim3 block(TX, BY);
-    dim3 grid((g.nx + TX - 1) / TX, (g.nz + BY * RZ - 1) / (BY * RZ), (g.nshots + SPB - 1) / SPB);
     if (is_psv(c)) {
         PsvFields<S> f{(S *)c->fields[0], (S *)c->fields[1], (S *)c->fields[2], (S *)c->fields[3],
                        (S *)c->fields[4]};
         PsvVelMat<C> m{(const C *)c->mat[0], (const C *)c->mat[1]};
         PsvVelPsi<S> ps{(S *)c->psi[0], (S *)c->psi[1], (S *)c->psi[5], (S *)c->psi[4]};
         PsvVelProf<C> pr{prof_of<C>(c, 0), prof_of<C>(c, 1), prof_of<C>(c, 5), prof_of<C>(c, 4)};
-        hipLaunchKernelGGL((k_psv_velocity<S, RZ, SPB>), grid, block, 0, c->stream, f, m, ps, pr, g, c->scl,
-                           c->amax);
+        hipLaunchKernelGGL((k_psv_velocity<S, RZ, SPB>), grid, block, 0, c->stream, f, m, ps, pr, g, w, ev,
+                           c->scl, c->amax);
     } else {
         ShFields<S> f{(S *)c->fields[0], (S *)c->fields[1], (S *)c->fields[2]};
         ShPsi2<S> ps{(S *)c->psi[0], (S *)c->psi[2]};
         ShProf2<C> pr{prof_of<C>(c, 0), prof_of<C>(c, 2)};
         hipLaunchKernelGGL((k_sh_velocity<S, RZ, SPB>), grid, block, 0, c->stream, f, (const C *)c->mat[0], ps,
-                           pr, g, c->scl, c->amax);
+                           pr, g, w, ev, c->scl, c->amax);
     }
 }
 
-template <class S> static void launch_pass(fd23_ctx *c, bool stress) {
+template <class S> static void launch_pass(fd23_ctx *c, bool stress, const Events &ev) {
     const TileCfg t = kTiles[c->tile];
-#define TILE_CASE(R, P)                                                        \
-    if (t.rz == R && t.spb == P) {                                             \
-        if (stress) launch_stress_t<S, R, P>(c); else launch_velocity_t<S, R, P>(c); \
-        return;                                                                \
+#define TILE_CASE(R, P)                                                                        \
+    if (t.rz == R && t.spb == P) {                                                             \
+        if (stress) launch_stress_t<S, R, P>(c, ev); else launch_velocity_t<S, R, P>(c, ev); \
+        return;                                                                                \
     }
     TILE_CASE(4, 1) TILE_CASE(2, 1) TILE_CASE(8, 1) TILE_CASE(4, 2) TILE_CASE(4, 4) TILE_CASE(4, 8)
     TILE_CASE(8, 2) TILE_CASE(8, 4) TILE_CASE(2, 4) TILE_CASE(2, 8)
 #undef TILE_CASE
 }
 
-static void dispatch_pass(fd23_ctx *c, bool stress) {
+static void dispatch_pass(fd23_ctx *c, bool stress, const Events &ev) {
     switch (c->d.storage) {
-    case FD23_STORE_F16: launch_pass<__half>(c, stress); break;
-    case FD23_STORE_BF16: launch_pass<__hip_bfloat16>(c, stress); break;
-    case FD23_STORE_F64: launch_pass<double>(c, stress); break;
-    default: launch_pass<float>(c, stress); break;
+    case FD23_STORE_F16: launch_pass<__half>(c, stress, ev); break;
+    case FD23_STORE_BF16: launch_pass<__hip_bfloat16>(c, stress, ev); break;
+    case FD23_STORE_F64: launch_pass<double>(c, stress, ev); break;
+    default: launch_pass<float>(c, stress, ev); break;
     }
 }
 
-template <class S> static void launch_inject_t(fd23_ctx *c, int it, bool use_clock) {
-    // one workgroup: its thread 0 advances the step clock after every
-    // shot's taps are done
+template <class S> static void launch_inject_t(fd23_ctx *c, int it) {
     const int nthr = std::min(256, round_up(c->d.nshots, 64));
-    hipLaunchKernelGGL((k_inject<S>), dim3(1), dim3(nthr), 0, c->stream, (S *const *)c->d_fieldtab,
-                       c->nfield, c->sstride, c->d_taps, c->d_src, c->d_stf, c->d.nshots, it,
-                       use_clock ? c->clock : nullptr, c->scl);
+    hipLaunchKernelGGL((k_inject<S>), dim3(1), dim3(nthr), 0, c->stream, (S *const *)c->d_fieldtab, c->sstride,
+                       c->d_taps, c->d_src, c->d_stf, c->d.nshots, it, c->scl);
 }
 
-static void dispatch_inject(fd23_ctx *c, int it, bool use_clock) {
+static void dispatch_inject(fd23_ctx *c, int it) {
     switch (c->d.storage) {
-    case FD23_STORE_F16: launch_inject_t<__half>(c, it, use_clock); break;
-    case FD23_STORE_BF16: launch_inject_t<__hip_bfloat16>(c, it, use_clock); break;
-    case FD23_STORE_F64: launch_inject_t<double>(c, it, use_clock); break;
-    default: launch_inject_t<float>(c, it, use_clock); break;
+    case FD23_STORE_F16: launch_inject_t<__half>(c, it); break;
+    case FD23_STORE_BF16: launch_inject_t<__hip_bfloat16>(c, it); break;
+    case FD23_STORE_F64: launch_inject_t<double>(c, it); break;
+    default: launch_inject_t<float>(c, it); break;
     }
 }
 
-template <class S> static void launch_record_t(fd23_ctx *c, int it, bool use_clock, int itrecord) {
+template <class S> static void launch_record_t(fd23_ctx *c, int it, int itrecord) {
     using C = typename Store<S>::C;
     const long total = (long)c->d.nshots * c->ncomp * c->d.max_rec;
     if (total == 0 || !c->seis) return;
     const int nblk = (int)std::min<long>((total + 255) / 256, 1024);
     hipLaunchKernelGGL((k_record<S>), dim3(nblk), dim3(256), 0, c->stream, (S *const *)c->d_comptab, c->ncomp,
                        c->sstride, c->d_roff, c->d_nrec, c->d.max_rec, c->d.nshots, (C *)c->seis,
-                       c->d.max_frames, it, use_clock ? c->clock : nullptr, itrecord, c->scl, c->d_compfield);
+                       c->d.max_frames, it, itrecord, c->scl, c->d_compfield);
 }
 
-static void dispatch_record(fd23_ctx *c, int it, bool use_clock, int itrecord) {
+static void dispatch_record(fd23_ctx *c, int it, int itrecord) {
     switch (c->d.storage) {
-    case FD23_STORE_F16: launch_record_t<__half>(c, it, use_clock, itrecord); break;
-    case FD23_STORE_BF16: launch_record_t<__hip_bfloat16>(c, it, use_clock, itrecord); break;
-    case FD23_STORE_F64: launch_record_t<double>(c, it, use_clock, itrecord); break;
-    default: launch_record_t<float>(c, it, use_clock, itrecord); break;
+    case FD23_STORE_F16: launch_record_t<__half>(c, it, itrecord); break;
+    case FD23_STORE_BF16: launch_record_t<__hip_bfloat16>(c, it, itrecord); break;
+    case FD23_STORE_F64: launch_record_t<double>(c, it, itrecord); break;
+    default: launch_record_t<float>(c, it, itrecord); break;
     }
 }
 
+static int init_scales(fd23_ctx *c);
 static int ready(fd23_ctx *c) {
     if (!c) return fail(FD23_EINVAL, "null context");
     if (!c->have_mat) return fail(FD23_EINVAL, "material not set");
     if (!c->have_cpml) return fail(FD23_EINVAL, "C-PML profiles not set");
-    return upload_sources(c);
+    int r = upload_sources(c);
+    return r ? r : init_scales(c);
 }
 
 static int check_launch() {
@@ -623,44 +750,120 @@ static int check_launch() {
     return FD23_OK;
 }
 
+static const Events kNoEvents{};
+
+// ---------------------------------------------------------------------------
+// FP16/BF16 scales
+// ---------------------------------------------------------------------------
+static bool scaled_storage(const fd23_ctx *c) {
+    return c->d.storage == FD23_STORE_F16 || c->d.storage == FD23_STORE_BF16;
+}
+static const int kScaleEvery = 8;            // steps between scale checks
+static const int kScaleLo = 7, kScaleHi = 13, kScaleTarget = 10; // stored max band (log2)
+
+static float pow2_scale(double est) {
+    if (!(est > 0) || !std::isfinite(est)) return 1.0f;
+    int e;
+    std::frexp(est, &e);
+    return std::ldexp(1.0f, e - kScaleTarget);
+}
+
+// First scales from the sources: stresses are of the order of the running
+// sum of the injected samples times |M|, velocities bmax times that.  The
+// control kernel corrects them within kScaleEvery steps.
+static int init_scales(fd23_ctx *c) {
+    if (!scaled_storage(c) || c->scales_init) return FD23_OK;
+    double t_est = 0;
+    for (const HostSrc &h : c->src) {
+        if (h.kind < 0) continue;
+        double run = 0, peak = 0;
+        for (double v : h.stf) {
+            run = h.kind == FD23_SRC_P ? v : run + v;
+            peak = std::max(peak, std::fabs(run));
+        }
+        const double mm = std::max(std::fabs(h.M[0]), std::max(std::fabs(h.M[1]), std::fabs(h.M[2])));
+        t_est = std::max(t_est, peak * mm);
+    }
+    const double v_est = t_est * std::max(c->bmax, 1e-30);
+    float sc[16];
+    for (int i = 0; i < 16; i++) sc[i] = 1.0f;
+    if (is_psv(c)) {
+        sc[SC_U] = sc[SC_W] = pow2_scale(v_est);
+        sc[SC_TXX] = sc[SC_TZZ] = sc[SC_TXZ] = pow2_scale(t_est);
+        sc[SC_PTXX_X] = sc[SC_PTXZ_X] = sc[SC_PTZZ_Z] = sc[SC_PTXZ_Z] = pow2_scale(t_est);
+        sc[SC_PU_X] = sc[SC_PW_X] = sc[SC_PU_Z] = sc[SC_PW_Z] = pow2_scale(v_est);
+    } else {
+        // a point force drives V directly
+        sc[SC_V] = pow2_scale(std::max(v_est, t_est));
+        sc[SC_TX] = sc[SC_TZ] = pow2_scale(t_est);
+        sc[SC_PTX_X] = sc[SC_PTZ_Z] = pow2_scale(t_est);
+        sc[SC_PV_X] = sc[SC_PV_Z] = pow2_scale(std::max(v_est, t_est));
+    }
+    HIPCHK(hipMemcpyAsync(c->scl, sc, sizeof sc, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->scales_init = true;
+    return FD23_OK;
+}
+
+static void enqueue_scale_control(fd23_ctx *c) {
+    hipLaunchKernelGGL(k_scale_control, dim3(1), dim3(64), 0, c->stream, c->scl, c->amax, c->resc, c->nscale,
+                       kScaleLo, kScaleHi, kScaleTarget);
+    RescaleTable tab{};
+    const size_t S = c->d.nshots;
+    for (int f = 0; f < c->nfield; f++) {
+        tab.ptr[f] = c->field_alloc[f];
+        tab.n[f] = (long)(S * c->sstride + 2 * kGuard);
+    }
+    const int half = c->npsi / 2;
+    for (int q = 0; q < c->npsi; q++) {
+        tab.ptr[c->nfield + q] = c->psi[q];
+        tab.n[c->nfield + q] = (long)(S * (q < half ? c->xpsi : c->zpsi));
+    }
+    const dim3 grid(1024, c->nscale);
+    if (c->d.storage == FD23_STORE_F16)
+        hipLaunchKernelGGL((k_rescale<__half>), grid, dim3(256), 0, c->stream, tab, (const float *)c->resc);
+    else
+        hipLaunchKernelGGL((k_rescale<__hip_bfloat16>), grid, dim3(256), 0, c->stream, tab, (const float *)c->resc);
+}
+
 extern "C" int fd23_step_stress(fd23_ctx *c) {
     int r = ready(c);
     if (r) return r;
-    if (c->box && (r = fd23_box_launch_pre_stress(c))) return r;
-    dispatch_pass(c, true);
-    if (c->box && (r = fd23_box_launch_post_stress(c))) return r;
+    if (c->box && (r = fd23_box_launch_pre_stress(c, -1))) return r;
+    dispatch_pass(c, true, kNoEvents);
+    if (c->box && (r = fd23_box_launch_post_stress(c, -1))) return r;
     return check_launch();
 }
 extern "C" int fd23_inject(fd23_ctx *c, int it) {
     int r = ready(c);
     if (r) return r;
-    dispatch_inject(c, it, false);
+    dispatch_inject(c, it);
     return check_launch();
 }
 extern "C" int fd23_step_velocity(fd23_ctx *c) {
     int r = ready(c);
     if (r) return r;
-    dispatch_pass(c, false);
-    if (c->box && (r = fd23_box_launch_post_velocity(c))) return r;
+    dispatch_pass(c, false, kNoEvents);
+    if (c->box && (r = fd23_box_launch_post_velocity(c, -1))) return r;
     return check_launch();
 }
 extern "C" int fd23_record(fd23_ctx *c, int it, int itrecord) {
     if (!c) return fail(FD23_EINVAL, "null context");
     if (itrecord < 1) return fail(FD23_EINVAL, "itrecord must be >= 1");
-    dispatch_record(c, it, false, itrecord);
+    dispatch_record(c, it, itrecord);
     return check_launch();
 }
 
-// one step with the device clock (graph body)
-static int enqueue_step(fd23_ctx *c, int itrecord) {
+// step k of a chunk: the step index is clock[0] + k on the device
+static int enqueue_step(fd23_ctx *c, int k, int itrecord) {
     int r;
-    if (c->box && (r = fd23_box_launch_pre_stress(c))) return r;
-    dispatch_pass(c, true);
-    if (c->box && (r = fd23_box_launch_post_stress(c))) return r;
-    dispatch_inject(c, 0, true);
-    dispatch_pass(c, false);
-    if (c->box && (r = fd23_box_launch_post_velocity(c))) return r;
-    dispatch_record(c, 0, true, itrecord);
+    const Events ev = make_events(c, k, itrecord);
+    if (c->box && (r = fd23_box_launch_pre_stress(c, k))) return r;
+    dispatch_pass(c, true, ev);
+    if (c->box && (r = fd23_box_launch_post_stress(c, k))) return r;
+    dispatch_pass(c, false, ev);
+    if (c->box && (r = fd23_box_launch_post_velocity(c, k))) return r;
+    if (scaled_storage(c) && (k + 1) % kScaleEvery == 0) enqueue_scale_control(c);
     return FD23_OK;
 }
 
@@ -676,7 +879,8 @@ static int get_graph(fd23_ctx *c, int itrecord, int steps, hipGraphExec_t *out) 
     hipGraph_t graph;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int r = FD23_OK;
-    for (int s = 0; s < steps && r == FD23_OK; s++) r = enqueue_step(c, itrecord);
+    for (int s = 0; s < steps && r == FD23_OK; s++) r = enqueue_step(c, s, itrecord);
+    if (r == FD23_OK) hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c->stream, c->clock, steps);
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (r) return r;
     if (e != hipSuccess) return fail(FD23_EHIP, "graph capture: %s", hipGetErrorString(e));
@@ -693,6 +897,8 @@ extern "C" int fd23_run(fd23_ctx *c, int it0, int nt, int itrecord) {
     int r = ready(c);
     if (r) return r;
     if (nt < 0 || it0 < 0 || itrecord < 1) return fail(FD23_EINVAL, "bad run range");
+    if ((r = build_events(c))) return r;
+    if ((r = init_scales(c))) return r;
     int clk[2] = {it0, it0};
     HIPCHK(hipMemcpyAsync(c->clock, clk, sizeof clk, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream)); // clk is a stack buffer
@@ -702,8 +908,12 @@ extern "C" int fd23_run(fd23_ctx *c, int it0, int nt, int itrecord) {
         if ((r = get_graph(c, itrecord, kGraphSteps, &g))) return r;
         for (int i = 0; i < full; i++) HIPCHK(hipGraphLaunch(g, c->stream));
     }
-    for (int i = 0; i < rest; i++)
-        if ((r = enqueue_step(c, itrecord))) return r;
+    if (rest > 0) {
+        for (int i = 0; i < rest; i++)
+            if ((r = enqueue_step(c, i, itrecord))) return r;
+        hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c->stream, c->clock, rest);
+    }
+    if (c->box) fd23_box_set_step(c, it0 + nt);
     return check_launch();
 }
 
@@ -889,9 +1099,9 @@ extern "C" int fd23_time_passes(fd23_ctx *c, int nsteps, double *stress_ms, doub
     double ts = 0, tv = 0;
     for (int i = 0; i < nsteps; i++) {
         hipEventRecord(e0, c->stream);
-        dispatch_pass(c, true);
+        dispatch_pass(c, true, kNoEvents);
         hipEventRecord(e1, c->stream);
-        dispatch_pass(c, false);
+        dispatch_pass(c, false, kNoEvents);
         hipEventRecord(e2, c->stream);
         hipEventSynchronize(e2);
         float a, b;

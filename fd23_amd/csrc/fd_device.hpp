@@ -105,9 +105,14 @@ __device__ __forceinline__ C dp(const C *f, int s, C c0, C c1, C c2, C c3) {
     return mad(-c3, f[4 * s] - f[-3 * s], r);
 }
 
-// float bits ordered as unsigned for atomicMax of |v|
+// Running max of |v| per array (scaled storage): reduced across the 64
+// lanes of the wavefront, then one atomicMax on the float bits (non-negative
+// floats order like unsigned integers).
 __device__ __forceinline__ void amax_update(unsigned *slot, float v) {
-    atomicMax(slot, __float_as_uint(fabsf(v)));
+    v = fabsf(v);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0 && v > 0.f) atomicMax(slot, __float_as_uint(v));
 }
 
 // ---------------------------------------------------------------------------

@@ -7,15 +7,29 @@
 // Tiling: a workgroup is 64 (x, one wavefront per row) x 4 (waves) threads
 // and owns a TX x TZ tile, TZ = 4*RZ; each thread updates RZ consecutive
 // cells of its column.  The fields whose derivatives are needed are staged
-// once per tile in LDS with their 4-cell halos (x halo only for the rows of
-// the tile, z halo only for the 64 tile columns).  A workgroup steps SPB
-// shots of the same model with the material held in registers, so the
-// material is read once per SPB shots.
+// in LDS with their 4-cell halos (x halo only for the rows of the tile, z
+// halo only for the 64 tile columns).  Every global load of a tile -- the
+// staged fields, the arrays the pass updates, the material -- is issued
+// before the first LDS barrier, so a workgroup waits on memory once.  A
+// workgroup steps SPB shots of the same model with the material held in
+// registers, so the material is read once per SPB shots.
+//
+// Grid: one dimension; workgroup b runs logical tile (b % 8)*q + b/8
+// (q = padded grid / 8).  Workgroups are dispatched round-robin over the 8
+// XCDs, so each XCD (own L2) sweeps one contiguous band of tiles and the
+// halo rows/columns a tile shares with its neighbours stay in that L2.
 //
 // Each tile is classified once: tiles that touch no C-PML strip, no edge
 // band of reduced order and not the free surface take a branch-free path
 // with the full 8th-order coefficients; all others evaluate the
 // reference's per-cell branch structure exactly.
+//
+// In the graph-captured loop the source injection is fused into the stress
+// pass and the receiver gather into the velocity pass (per-tile lists built
+// on the host); the stand-alone k_inject / k_record kernels serve the
+// staged entry points.  Fusing is exact: the stress update of a cell reads
+// only U and W, so adding the source to a cell's stress right after its own
+// update equals adding it after the whole pass.
 #pragma once
 #include "fd_device.hpp"
 
@@ -38,36 +52,141 @@ __device__ __forceinline__ long zpsi_idx(int ix, int iz, int nx, int nz, int npm
     return topcell ? (long)iz * nx + ix : (long)(npml + nz - 1 - iz) * nx + ix;
 }
 
-// Cooperative LDS staging of one field tile.
-//  ZH: stage the 4-row halos above/below (all TX columns)
-//  XH: stage the 4-column halos left/right (tile rows only)
-// LDS row r <-> global row z0-4+r; LDS column c <-> global column x0-4+c.
-template <class S, int TZ, bool ZH, bool XH>
-__device__ __forceinline__ void stage(typename Store<S>::C *sm, const S *__restrict__ f, long base,
-                                      int px, int x0, int z0, float scl) {
-    constexpr int LW = TX + 2 * HALO;
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    constexpr int rbeg = ZH ? 0 : HALO, rend = ZH ? TZ + 2 * HALO : TZ + HALO;
-    static_assert((rend - rbeg) % BY == 0, "tile rows must split evenly over the waves");
-#pragma unroll
-    for (int i = 0; i < (rend - rbeg) / BY; i++) {
-        const int r = rbeg + ty + i * BY;
-        sm[r * LW + tx + HALO] = Store<S>::ld(f, base + (long)(z0 - HALO + r) * px + x0 + tx, scl);
+// ---------------------------------------------------------------------------
+// work mapping
+// ---------------------------------------------------------------------------
+struct Work {
+    int ntx, ntz, ngroups; // tiles in x, in z, shot groups
+    int nwork;             // ntx*ntz*ngroups (logical workgroups)
+};
+struct TileId {
+    int tile, group, x0, z0;
+};
+template <int TZ> __device__ __forceinline__ bool map_tile(const Work &w, TileId &t) {
+    const int q = gridDim.x >> 3; // grid padded to a multiple of 8
+    const int L = (int)(blockIdx.x & 7) * q + (int)(blockIdx.x >> 3);
+    if (L >= w.nwork) return false;
+    const int ntiles = w.ntx * w.ntz;
+    t.group = L / ntiles;
+    t.tile = L - t.group * ntiles;
+    const int tz = t.tile / w.ntx;
+    t.x0 = (t.tile - tz * w.ntx) * TX;
+    t.z0 = tz * TZ;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// fused events (graph loop): step = clock[0] + k
+// ---------------------------------------------------------------------------
+struct TapT {
+    int cell;   // lz*TX + lx inside the tile
+    int field;  // internal field index (P-SV U W Txx Tzz Txz, SH V Tx Tz)
+    double m;   // moment component (already scaled)
+    double q;   // multiplier: +-1, +-0.25 (P-SV) or +-0.5 (SH)
+    int mode;   // 0: float product then add (P-SV), 1: double sum (SH)
+};
+struct RecT {
+    int cell;   // lz*TX + lx inside the tile
+    int comp;
+    int out;    // index inside one seismogram frame
+};
+struct SrcInfo {
+    int ntap, tap0; // taps [tap0, tap0+ntap) of the stand-alone tap table
+    int lsrc;
+    int clamp;      // 1: it >= lsrc uses stf[lsrc-1] ('p' / stype 13)
+    long stf0;      // offset of this shot's samples in the stf table
+};
+struct Events {
+    const int *clock;     // nullptr: no fused events
+    int k;                // step offset inside the captured chunk
+    int itrecord;
+    const int *tap_start; // CSR over (tile*nshots + shot)
+    const TapT *taps;
+    const SrcInfo *src;
+    const double *stf;
+    const int *rec_start; // CSR over (tile*nshots + shot)
+    const RecT *recs;
+    void *seis;           // [max_frames][frame_size] compute type
+    int max_frames;
+    long frame_size;
+};
+
+// source sample of a shot at step it, or false when inactive (it >= lsrc
+// without clamping), PSV/clean_nobox/point_source/cu_point_source.cu:47,110-112
+__device__ __forceinline__ bool src_sample(const SrcInfo &si, const double *stf, int it, double &sv) {
+    if (si.lsrc <= 0) return false;
+    int t = it;
+    if (t >= si.lsrc) {
+        if (!si.clamp) return false;
+        t = si.lsrc - 1;
     }
-    if (XH) {
-        const int tid = ty * TX + tx;
-        constexpr int NE = TZ * 2 * HALO;
+    sv = stf[si.stf0 + t];
+    return true;
+}
+
+template <class C> __device__ __forceinline__ C apply_tap(C v, const TapT &tp, double sv) {
+    if (tp.mode == 0) {
+        const C prod = C(sv) * C(tp.m);
+        return v + prod * C(tp.q);
+    }
+    return C((double)v + tp.q * sv * tp.m);
+}
+
+// ---------------------------------------------------------------------------
+// tile staging through registers: all loads first, LDS writes afterwards
+//  ZH: the 4-row halos above/below (all TX columns)
+//  XH: the 4-column halos left/right (tile rows only)
+// LDS row r <-> global row z0-4+r; LDS column c <-> global column x0-4+c.
+// ---------------------------------------------------------------------------
+template <class S, int TZ, bool ZH, bool XH> struct Stage {
+    using C = typename Store<S>::C;
+    static constexpr int LW = TX + 2 * HALO;
+    static constexpr int RB = ZH ? 0 : HALO, RE = ZH ? TZ + 2 * HALO : TZ + HALO;
+    static constexpr int NR = (RE - RB) / BY;
+    static constexpr int NE = XH ? TZ * 2 * HALO : 0;
+    static constexpr int NH = XH ? (NE + TX * BY - 1) / (TX * BY) : 0;
+    static_assert((RE - RB) % BY == 0, "tile rows must split evenly over the waves");
+    C r[NR];
+    C h[NH > 0 ? NH : 1];
+
+    __device__ __forceinline__ void load(const S *__restrict__ f, long base, int px, int x0, int z0, float scl) {
+        const int tx = threadIdx.x, ty = threadIdx.y;
 #pragma unroll
-        for (int i = 0; i < (NE + TX * BY - 1) / (TX * BY); i++) {
-            const int e = tid + i * TX * BY;
-            if (NE % (TX * BY) != 0 && e >= NE) break;
-            const int r = e >> 3, c = e & 7;
-            const int col = c < HALO ? c - HALO : TX + c - HALO;
-            sm[(r + HALO) * LW + col + HALO] =
-                Store<S>::ld(f, base + (long)(z0 + r) * px + x0 + col, scl);
+        for (int i = 0; i < NR; i++) {
+            const int row = RB + ty + i * BY;
+            r[i] = Store<S>::ld(f, base + (long)(z0 - HALO + row) * px + x0 + tx, scl);
+        }
+        if (XH) {
+            const int tid = ty * TX + tx;
+#pragma unroll
+            for (int i = 0; i < NH; i++) {
+                const int e = tid + i * TX * BY;
+                if (NE % (TX * BY) == 0 || e < NE) {
+                    const int row = e >> 3, c = e & 7;
+                    const int col = c < HALO ? c - HALO : TX + c - HALO;
+                    h[i] = Store<S>::ld(f, base + (long)(z0 + row) * px + x0 + col, scl);
+                }
+            }
         }
     }
-}
+    __device__ __forceinline__ void store(C *sm) const {
+        const int tx = threadIdx.x, ty = threadIdx.y;
+#pragma unroll
+        for (int i = 0; i < NR; i++) sm[(RB + ty + i * BY) * LW + tx + HALO] = r[i];
+        if (XH) {
+            const int tid = ty * TX + tx;
+#pragma unroll
+            for (int i = 0; i < NH; i++) {
+                const int e = tid + i * TX * BY;
+                if (NE % (TX * BY) == 0 || e < NE) {
+                    const int row = e >> 3, c = e & 7;
+                    const int col = c < HALO ? c - HALO : TX + c - HALO;
+                    sm[(row + HALO) * LW + col + HALO] = h[i];
+                }
+            }
+        }
+    }
+};
 
 // ===========================================================================
 // P-SV stress pass (cu_fd.cu:96-245)
@@ -89,8 +208,8 @@ enum { SC_U = 0, SC_W, SC_TXX, SC_TZZ, SC_TXZ, SC_PTXX_X, SC_PTXZ_X, SC_PU_X, SC
 template <class S, int RZ, int SPB>
 __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStressMat<typename Store<S>::C> m,
                                                        PsvStressPsi<S> ps,
-                                                       PsvStressProf<typename Store<S>::C> pr, Grid g,
-                                                       const float *__restrict__ scl,
+                                                       PsvStressProf<typename Store<S>::C> pr, Grid g, Work w,
+                                                       Events ev, const float *__restrict__ scl,
                                                        unsigned *__restrict__ amax) {
     using C = typename Store<S>::C;
     using ST = Store<S>;
@@ -99,12 +218,15 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
     __shared__ C sU[(TZ + 2 * HALO) * LW];
     __shared__ C sW[(TZ + 2 * HALO) * LW];
 
+    TileId t;
+    if (!map_tile<TZ>(w, t)) return;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int x0 = blockIdx.x * TX, z0 = blockIdx.y * TZ;
+    const int x0 = t.x0, z0 = t.z0;
     const int nx = g.nx, nz = g.nz, px = g.px, npml = g.npml;
     const int gx = x0 + tx;
     const int lz0 = ty * RZ;
     const bool interior = tile_interior(g, x0, z0, TZ);
+    const int it = ev.clock ? ev.clock[0] + ev.k : 0;
 
     float sc_u = 1.f, sc_w = 1.f, sc_xx = 1.f, sc_zz = 1.f, sc_xz = 1.f;
     if (ST::scaled) {
@@ -112,6 +234,7 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
         sc_xx = scl[SC_TXX]; sc_zz = scl[SC_TZZ]; sc_xz = scl[SC_TXZ];
     }
     float mx_xx = 0.f, mx_zz = 0.f, mx_xz = 0.f;
+    float mp[4] = {0.f, 0.f, 0.f, 0.f}; // psi U_x W_x W_z U_z
 
     // material for this thread's cells, reused across the SPB shots
     C lam[RZ], mu[RZ], mua[RZ];
@@ -125,14 +248,25 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
         mua[r] = live ? m.MUA[q] : C(0);
     }
 
-    const int sh0 = blockIdx.z * SPB;
+    const int sh0 = t.group * SPB;
     for (int s = 0; s < SPB; s++) {
         const int shot = sh0 + s;
         if (shot >= g.nshots) break;
         const long base = (long)shot * g.sstride;
+        Stage<S, TZ, true, true> rU, rW;
+        rU.load(f.U, base, px, x0, z0, sc_u);
+        rW.load(f.W, base, px, x0, z0, sc_w);
+        C txx[RZ], tzz[RZ], txz[RZ];
+#pragma unroll
+        for (int r = 0; r < RZ; r++) {
+            const long q = base + (long)(z0 + lz0 + r) * px + gx;
+            txx[r] = ST::ld(f.Txx, q, sc_xx);
+            tzz[r] = ST::ld(f.Tzz, q, sc_zz);
+            txz[r] = ST::ld(f.Txz, q, sc_xz);
+        }
         if (s > 0) __syncthreads();
-        stage<S, TZ, true, true>(sU, f.U, base, px, x0, z0, sc_u);
-        stage<S, TZ, true, true>(sW, f.W, base, px, x0, z0, sc_w);
+        rU.store(sU);
+        rW.store(sW);
         __syncthreads();
 
         if (interior) {
@@ -147,18 +281,8 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
                 const C wx = dp(pw, 1, c0, c1, c2, c3);
                 const C wz = dp(pw, LW, c0, c1, c2, c3);
                 const C gam = lam[r] + C(2.0) * mu[r];
-                const long q = base + (long)(z0 + lz) * px + gx;
-                C txx = ST::ld(f.Txx, q, sc_xx), tzz = ST::ld(f.Tzz, q, sc_zz), txz = ST::ld(f.Txz, q, sc_xz);
-                txx = mad(gam, ux, txx); tzz = mad(lam[r], ux, tzz); txz = mad(mua[r], wx, txz);
-                txx = mad(lam[r], wz, txx); tzz = mad(gam, wz, tzz); txz = mad(mua[r], uz, txz);
-                ST::st(f.Txx, q, txx, 1.f / sc_xx);
-                ST::st(f.Tzz, q, tzz, 1.f / sc_zz);
-                ST::st(f.Txz, q, txz, 1.f / sc_xz);
-                if (ST::scaled) {
-                    mx_xx = fmaxf(mx_xx, fabsf((float)txx));
-                    mx_zz = fmaxf(mx_zz, fabsf((float)tzz));
-                    mx_xz = fmaxf(mx_xz, fabsf((float)txz));
-                }
+                txx[r] = mad(gam, ux, txx[r]); tzz[r] = mad(lam[r], ux, tzz[r]); txz[r] = mad(mua[r], wx, txz[r]);
+                txx[r] = mad(lam[r], wz, txx[r]); tzz[r] = mad(gam, wz, tzz[r]); txz[r] = mad(mua[r], uz, txz[r]);
             }
         } else {
             float sp_ux = 1.f, sp_wx = 1.f, sp_uz = 1.f, sp_wz = 1.f;
@@ -180,24 +304,23 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
                 const C wz = dp(pw, LW, c0, c1, c2, c3);
                 const C lm = lam[r], mm = mu[r], ma = mua[r];
                 const C gam = lm + C(2.0) * mm;
-                const long q = base + (long)iz * px + ix;
-                C txx = ST::ld(f.Txx, q, sc_xx), tzz = ST::ld(f.Tzz, q, sc_zz), txz = ST::ld(f.Txz, q, sc_xz);
+                C a = txx[r], b = tzz[r], cc = txz[r];
                 if (ix < npml || ix >= nx - npml) {
                     const long pi = (long)shot * g.xpsi_stride + xpsi_idx(ix, iz, nx, npml);
                     C p1 = mad(pr.Ux.b[ix], ST::ld(ps.Ux, pi, sp_ux), pr.Ux.c[ix] * ux);
                     ST::st(ps.Ux, pi, p1, 1.f / sp_ux);
                     const C e1 = mad(ux, pr.Ux.k[ix], p1);
-                    txx = mad(gam, e1, txx);
-                    tzz = mad(lm, e1, tzz);
+                    a = mad(gam, e1, a);
+                    b = mad(lm, e1, b);
                     C p2 = mad(pr.Wx.b[ix], ST::ld(ps.Wx, pi, sp_wx), pr.Wx.c[ix] * wx);
                     ST::st(ps.Wx, pi, p2, 1.f / sp_wx);
-                    txz = mad(ma, mad(wx, pr.Wx.k[ix], p2), txz);
+                    cc = mad(ma, mad(wx, pr.Wx.k[ix], p2), cc);
                     if (ST::scaled) {
-                        atomicMax(&amax[SC_PU_X], __float_as_uint(fabsf((float)p1)));
-                        atomicMax(&amax[SC_PW_X], __float_as_uint(fabsf((float)p2)));
+                        mp[0] = fmaxf(mp[0], fabsf((float)p1));
+                        mp[1] = fmaxf(mp[1], fabsf((float)p2));
                     }
                 } else if (iz != 0) {
-                    txx = mad(gam, ux, txx); tzz = mad(lm, ux, tzz); txz = mad(ma, wx, txz);
+                    a = mad(gam, ux, a); b = mad(lm, ux, b); cc = mad(ma, wx, cc);
                 }
                 const bool topcell = iz < npml && g.top;
                 if (topcell || iz >= nz - npml) {
@@ -205,30 +328,58 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
                     C p1 = mad(pr.Wz.b[iz], ST::ld(ps.Wz, pi, sp_wz), pr.Wz.c[iz] * wz);
                     ST::st(ps.Wz, pi, p1, 1.f / sp_wz);
                     const C e1 = mad(wz, pr.Wz.k[iz], p1);
-                    txx = mad(lm, e1, txx);
-                    tzz = mad(gam, e1, tzz);
+                    a = mad(lm, e1, a);
+                    b = mad(gam, e1, b);
                     C p2 = mad(pr.Uz.b[iz], ST::ld(ps.Uz, pi, sp_uz), pr.Uz.c[iz] * uz);
                     ST::st(ps.Uz, pi, p2, 1.f / sp_uz);
-                    txz = mad(ma, mad(uz, pr.Uz.k[iz], p2), txz);
+                    cc = mad(ma, mad(uz, pr.Uz.k[iz], p2), cc);
                     if (ST::scaled) {
-                        atomicMax(&amax[SC_PW_Z], __float_as_uint(fabsf((float)p1)));
-                        atomicMax(&amax[SC_PU_Z], __float_as_uint(fabsf((float)p2)));
+                        mp[2] = fmaxf(mp[2], fabsf((float)p1));
+                        mp[3] = fmaxf(mp[3], fabsf((float)p2));
                     }
                 } else if (iz == 0) {
-                    txx = mad(C(2.0) * mm, ux, txx);
-                    tzz = C(0);
-                    txz = C(0);
+                    a = mad(C(2.0) * mm, ux, a);
+                    b = C(0);
+                    cc = C(0);
                 } else {
-                    txx = mad(lm, wz, txx); tzz = mad(gam, wz, tzz); txz = mad(ma, uz, txz);
+                    a = mad(lm, wz, a); b = mad(gam, wz, b); cc = mad(ma, uz, cc);
                 }
-                ST::st(f.Txx, q, txx, 1.f / sc_xx);
-                ST::st(f.Tzz, q, tzz, 1.f / sc_zz);
-                ST::st(f.Txz, q, txz, 1.f / sc_xz);
-                if (ST::scaled) {
-                    mx_xx = fmaxf(mx_xx, fabsf((float)txx));
-                    mx_zz = fmaxf(mx_zz, fabsf((float)tzz));
-                    mx_xz = fmaxf(mx_xz, fabsf((float)txz));
+                txx[r] = a; tzz[r] = b; txz[r] = cc;
+            }
+        }
+        // fused source injection: the owner of a tap cell adds it after the
+        // cell's own update, taps in the reference's order
+        if (ev.clock) {
+            const int key = t.tile * g.nshots + shot;
+            const int e0 = ev.tap_start[key], e1 = ev.tap_start[key + 1];
+            double sv;
+            if (e1 > e0 && src_sample(ev.src[shot], ev.stf, it, sv)) {
+                for (int e = e0; e < e1; e++) {
+                    const TapT tp = ev.taps[e];
+                    const int lx = tp.cell % TX, lz = tp.cell / TX;
+                    if (lx != tx || lz < lz0 || lz >= lz0 + RZ) continue;
+#pragma unroll
+                    for (int r = 0; r < RZ; r++) {
+                        if (lz != lz0 + r) continue;
+                        if (tp.field == 2) txx[r] = apply_tap(txx[r], tp, sv);
+                        else if (tp.field == 3) tzz[r] = apply_tap(tzz[r], tp, sv);
+                        else txz[r] = apply_tap(txz[r], tp, sv);
+                    }
                 }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RZ; r++) {
+            const int gz = z0 + lz0 + r;
+            if (gx >= nx || gz >= nz) continue;
+            const long q = base + (long)gz * px + gx;
+            ST::st(f.Txx, q, txx[r], 1.f / sc_xx);
+            ST::st(f.Tzz, q, tzz[r], 1.f / sc_zz);
+            ST::st(f.Txz, q, txz[r], 1.f / sc_xz);
+            if (ST::scaled) {
+                mx_xx = fmaxf(mx_xx, fabsf((float)txx[r]));
+                mx_zz = fmaxf(mx_zz, fabsf((float)tzz[r]));
+                mx_xz = fmaxf(mx_xz, fabsf((float)txz[r]));
             }
         }
     }
@@ -236,6 +387,34 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
         amax_update(&amax[SC_TXX], mx_xx);
         amax_update(&amax[SC_TZZ], mx_zz);
         amax_update(&amax[SC_TXZ], mx_xz);
+        amax_update(&amax[SC_PU_X], mp[0]);
+        amax_update(&amax[SC_PW_X], mp[1]);
+        amax_update(&amax[SC_PW_Z], mp[2]);
+        amax_update(&amax[SC_PU_Z], mp[3]);
+    }
+}
+
+// gather fused into a velocity pass: the new values of the tile are parked
+// in LDS (the staging buffers are free once every wave has computed), then
+// the tile's receivers read them.  Uniform per workgroup.
+template <class C, int RZ, int NC>
+__device__ __forceinline__ void fused_record(const Events &ev, int it, int key, C *park0, C *park1,
+                                             const C (&val)[NC][RZ], int lz0) {
+    const int e0 = ev.rec_start[key], e1 = ev.rec_start[key + 1];
+    if (e1 <= e0 || it % ev.itrecord != 0) return;
+    const int frame = it / ev.itrecord;
+    if (frame >= ev.max_frames) return;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RZ; r++) {
+        park0[(lz0 + r) * TX + threadIdx.x] = val[0][r];
+        if (NC > 1) park1[(lz0 + r) * TX + threadIdx.x] = val[NC - 1][r];
+    }
+    __syncthreads();
+    C *seis = (C *)ev.seis + (long)frame * ev.frame_size;
+    for (int e = e0 + threadIdx.y * TX + threadIdx.x; e < e1; e += TX * BY) {
+        const RecT rc = ev.recs[e];
+        seis[rc.out] = rc.comp == 0 ? park0[rc.cell] : park1[rc.cell];
     }
 }
 
@@ -255,8 +434,8 @@ template <class C> struct PsvVelProf {
 template <class S, int RZ, int SPB>
 __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelMat<typename Store<S>::C> m,
                                                          PsvVelPsi<S> ps,
-                                                         PsvVelProf<typename Store<S>::C> pr, Grid g,
-                                                         const float *__restrict__ scl,
+                                                         PsvVelProf<typename Store<S>::C> pr, Grid g, Work w,
+                                                         Events ev, const float *__restrict__ scl,
                                                          unsigned *__restrict__ amax) {
     using C = typename Store<S>::C;
     using ST = Store<S>;
@@ -266,12 +445,15 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
     __shared__ C sZZ[(TZ + 2 * HALO) * LW];
     __shared__ C sXZ[(TZ + 2 * HALO) * LW];
 
+    TileId t;
+    if (!map_tile<TZ>(w, t)) return;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int x0 = blockIdx.x * TX, z0 = blockIdx.y * TZ;
+    const int x0 = t.x0, z0 = t.z0;
     const int nx = g.nx, nz = g.nz, px = g.px, npml = g.npml;
     const int gx = x0 + tx;
     const int lz0 = ty * RZ;
     const bool interior = tile_interior(g, x0, z0, TZ);
+    const int it = ev.clock ? ev.clock[0] + ev.k : 0;
 
     float sc_u = 1.f, sc_w = 1.f, sc_xx = 1.f, sc_zz = 1.f, sc_xz = 1.f;
     if (ST::scaled) {
@@ -279,6 +461,7 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
         sc_xx = scl[SC_TXX]; sc_zz = scl[SC_TZZ]; sc_xz = scl[SC_TXZ];
     }
     float mx_u = 0.f, mx_w = 0.f;
+    float mp[4] = {0.f, 0.f, 0.f, 0.f}; // psi Txx_x Txz_x Txz_z Tzz_z
 
     C bu[RZ], bw[RZ];
 #pragma unroll
@@ -290,15 +473,28 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
         bw[r] = live ? m.BW[q] : C(0);
     }
 
-    const int sh0 = blockIdx.z * SPB;
+    const int sh0 = t.group * SPB;
     for (int s = 0; s < SPB; s++) {
         const int shot = sh0 + s;
         if (shot >= g.nshots) break;
         const long base = (long)shot * g.sstride;
+        Stage<S, TZ, false, true> rXX;
+        Stage<S, TZ, true, false> rZZ;
+        Stage<S, TZ, true, true> rXZ;
+        rXX.load(f.Txx, base, px, x0, z0, sc_xx);
+        rZZ.load(f.Tzz, base, px, x0, z0, sc_zz);
+        rXZ.load(f.Txz, base, px, x0, z0, sc_xz);
+        C nv[2][RZ];
+#pragma unroll
+        for (int r = 0; r < RZ; r++) {
+            const long q = base + (long)(z0 + lz0 + r) * px + gx;
+            nv[0][r] = ST::ld(f.U, q, sc_u);
+            nv[1][r] = ST::ld(f.W, q, sc_w);
+        }
         if (s > 0) __syncthreads();
-        stage<S, TZ, false, true>(sXX, f.Txx, base, px, x0, z0, sc_xx);
-        stage<S, TZ, true, false>(sZZ, f.Tzz, base, px, x0, z0, sc_zz);
-        stage<S, TZ, true, true>(sXZ, f.Txz, base, px, x0, z0, sc_xz);
+        rXX.store(sXX);
+        rZZ.store(sZZ);
+        rXZ.store(sXZ);
         __syncthreads();
 
         if (interior) {
@@ -311,16 +507,10 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
                 const C dzTzz = dm(&sZZ[o], LW, c0, c1, c2, c3);
                 const C dxTxz = dm(&sXZ[o], 1, c0, c1, c2, c3);
                 const C dzTxz = dp(&sXZ[o], LW, c0, c1, c2, c3);
-                const long q = base + (long)(z0 + lz) * px + gx;
-                C u = ST::ld(f.U, q, sc_u), w = ST::ld(f.W, q, sc_w);
-                u = mad(bu[r], dxTxx, u); w = mad(bw[r], dxTxz, w);
-                u = mad(bu[r], dzTxz, u); w = mad(bw[r], dzTzz, w);
-                ST::st(f.U, q, u, 1.f / sc_u);
-                ST::st(f.W, q, w, 1.f / sc_w);
-                if (ST::scaled) {
-                    mx_u = fmaxf(mx_u, fabsf((float)u));
-                    mx_w = fmaxf(mx_w, fabsf((float)w));
-                }
+                C u = nv[0][r], ww = nv[1][r];
+                u = mad(bu[r], dxTxx, u); ww = mad(bw[r], dxTxz, ww);
+                u = mad(bu[r], dzTxz, u); ww = mad(bw[r], dzTzz, ww);
+                nv[0][r] = u; nv[1][r] = ww;
             }
         } else {
             float sp_xx = 1.f, sp_xzx = 1.f, sp_xzz = 1.f, sp_zz = 1.f;
@@ -340,8 +530,7 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
                 const C dxTxz = dm(&sXZ[o], 1, c0, c1, c2, c3);
                 const C dzTxz = dp(&sXZ[o], LW, c0, c1, c2, c3);
                 const C b_u = bu[r], b_w = bw[r];
-                const long q = base + (long)iz * px + ix;
-                C u = ST::ld(f.U, q, sc_u), w = ST::ld(f.W, q, sc_w);
+                C u = nv[0][r], ww = nv[1][r];
                 if (ix < npml || ix >= nx - npml) {
                     const long pi = (long)shot * g.xpsi_stride + xpsi_idx(ix, iz, nx, npml);
                     C p1 = mad(pr.Txx_x.b[ix], ST::ld(ps.Txx_x, pi, sp_xx), pr.Txx_x.c[ix] * dxTxx);
@@ -349,14 +538,14 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
                     u = mad(b_u, mad(dxTxx, pr.Txx_x.k[ix], p1), u);
                     C p2 = mad(pr.Txz_x.b[ix], ST::ld(ps.Txz_x, pi, sp_xzx), pr.Txz_x.c[ix] * dxTxz);
                     ST::st(ps.Txz_x, pi, p2, 1.f / sp_xzx);
-                    w = mad(b_w, mad(dxTxz, pr.Txz_x.k[ix], p2), w);
+                    ww = mad(b_w, mad(dxTxz, pr.Txz_x.k[ix], p2), ww);
                     if (ST::scaled) {
-                        atomicMax(&amax[SC_PTXX_X], __float_as_uint(fabsf((float)p1)));
-                        atomicMax(&amax[SC_PTXZ_X], __float_as_uint(fabsf((float)p2)));
+                        mp[0] = fmaxf(mp[0], fabsf((float)p1));
+                        mp[1] = fmaxf(mp[1], fabsf((float)p2));
                     }
                 } else if (iz != 0) {
                     u = mad(b_u, dxTxx, u);
-                    w = mad(b_w, dxTxz, w);
+                    ww = mad(b_w, dxTxz, ww);
                 }
                 const bool topcell = iz < npml && g.top;
                 if (topcell || iz >= nz - npml) {
@@ -366,30 +555,42 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
                     ST::st(ps.Txz_z, pi, p1, 1.f / sp_xzz);
                     ST::st(ps.Tzz_z, pi, p2, 1.f / sp_zz);
                     u = mad(b_u, mad(dzTxz, pr.Txz_z.k[iz], p1), u);
-                    w = mad(b_w, mad(dzTzz, pr.Tzz_z.k[iz], p2), w);
+                    ww = mad(b_w, mad(dzTzz, pr.Tzz_z.k[iz], p2), ww);
                     if (ST::scaled) {
-                        atomicMax(&amax[SC_PTXZ_Z], __float_as_uint(fabsf((float)p1)));
-                        atomicMax(&amax[SC_PTZZ_Z], __float_as_uint(fabsf((float)p2)));
+                        mp[2] = fmaxf(mp[2], fabsf((float)p1));
+                        mp[3] = fmaxf(mp[3], fabsf((float)p2));
                     }
                 } else if (iz == 0) {
                     u = mad(b_u, dxTxx + dzTxz, u);
-                    w = C(0);
+                    ww = C(0);
                 } else {
                     u = mad(b_u, dzTxz, u);
-                    w = mad(b_w, dzTzz, w);
+                    ww = mad(b_w, dzTzz, ww);
                 }
-                ST::st(f.U, q, u, 1.f / sc_u);
-                ST::st(f.W, q, w, 1.f / sc_w);
-                if (ST::scaled) {
-                    mx_u = fmaxf(mx_u, fabsf((float)u));
-                    mx_w = fmaxf(mx_w, fabsf((float)w));
-                }
+                nv[0][r] = u; nv[1][r] = ww;
             }
         }
+#pragma unroll
+        for (int r = 0; r < RZ; r++) {
+            const int gz = z0 + lz0 + r;
+            if (gx >= nx || gz >= nz) continue;
+            const long q = base + (long)gz * px + gx;
+            ST::st(f.U, q, nv[0][r], 1.f / sc_u);
+            ST::st(f.W, q, nv[1][r], 1.f / sc_w);
+            if (ST::scaled) {
+                mx_u = fmaxf(mx_u, fabsf((float)nv[0][r]));
+                mx_w = fmaxf(mx_w, fabsf((float)nv[1][r]));
+            }
+        }
+        if (ev.clock) fused_record<C, RZ, 2>(ev, it, t.tile * g.nshots + shot, sXX, sZZ, nv, lz0);
     }
     if (ST::scaled) {
         amax_update(&amax[SC_U], mx_u);
         amax_update(&amax[SC_W], mx_w);
+        amax_update(&amax[SC_PTXX_X], mp[0]);
+        amax_update(&amax[SC_PTXZ_X], mp[1]);
+        amax_update(&amax[SC_PTXZ_Z], mp[2]);
+        amax_update(&amax[SC_PTZZ_Z], mp[3]);
     }
 }
 
@@ -411,8 +612,8 @@ template <class C> struct ShProf2 {
 
 template <class S, int RZ, int SPB>
 __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat<typename Store<S>::C> m,
-                                                      ShPsi2<S> ps, ShProf2<typename Store<S>::C> pr,
-                                                      Grid g, const float *__restrict__ scl,
+                                                      ShPsi2<S> ps, ShProf2<typename Store<S>::C> pr, Grid g,
+                                                      Work w, Events ev, const float *__restrict__ scl,
                                                       unsigned *__restrict__ amax) {
     using C = typename Store<S>::C;
     using ST = Store<S>;
@@ -420,16 +621,20 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
     constexpr int LW = TX + 2 * HALO;
     __shared__ C sV[(TZ + 2 * HALO) * LW];
 
+    TileId t;
+    if (!map_tile<TZ>(w, t)) return;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int x0 = blockIdx.x * TX, z0 = blockIdx.y * TZ;
+    const int x0 = t.x0, z0 = t.z0;
     const int nx = g.nx, nz = g.nz, px = g.px, npml = g.npml;
     const int gx = x0 + tx;
     const int lz0 = ty * RZ;
     const bool interior = tile_interior(g, x0, z0, TZ);
+    const int it = ev.clock ? ev.clock[0] + ev.k : 0;
 
     float sc_v = 1.f, sc_x = 1.f, sc_z = 1.f;
     if (ST::scaled) { sc_v = scl[SC_V]; sc_x = scl[SC_TX]; sc_z = scl[SC_TZ]; }
     float mx_x = 0.f, mx_z = 0.f;
+    float mp[2] = {0.f, 0.f}; // psi V_x V_z
 
     C mux[RZ], muz[RZ];
 #pragma unroll
@@ -441,13 +646,22 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
         muz[r] = live ? m.MUZ[q] : C(0);
     }
 
-    const int sh0 = blockIdx.z * SPB;
+    const int sh0 = t.group * SPB;
     for (int s = 0; s < SPB; s++) {
         const int shot = sh0 + s;
         if (shot >= g.nshots) break;
         const long base = (long)shot * g.sstride;
+        Stage<S, TZ, true, true> rV;
+        rV.load(f.V, base, px, x0, z0, sc_v);
+        C vx[RZ], vz[RZ];
+#pragma unroll
+        for (int r = 0; r < RZ; r++) {
+            const long q = base + (long)(z0 + lz0 + r) * px + gx;
+            vx[r] = ST::ld(f.Tx, q, sc_x);
+            vz[r] = ST::ld(f.Tz, q, sc_z);
+        }
         if (s > 0) __syncthreads();
-        stage<S, TZ, true, true>(sV, f.V, base, px, x0, z0, sc_v);
+        rV.store(sV);
         __syncthreads();
         if (interior) {
             const C c0 = C(1225. / 1024.), c1 = C(245. / 3072.), c2 = C(49. / 5120.), c3 = C(5. / 7168.);
@@ -457,15 +671,8 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
                 const C *pv = &sV[(lz + HALO) * LW + tx + HALO];
                 const C dxV = dp(pv, 1, c0, c1, c2, c3);
                 const C dzV = dm(pv, LW, c0, c1, c2, c3);
-                const long q = base + (long)(z0 + lz) * px + gx;
-                C tx_ = mad(mux[r], dxV, ST::ld(f.Tx, q, sc_x));
-                C tz_ = mad(muz[r], dzV, ST::ld(f.Tz, q, sc_z));
-                ST::st(f.Tx, q, tx_, 1.f / sc_x);
-                ST::st(f.Tz, q, tz_, 1.f / sc_z);
-                if (ST::scaled) {
-                    mx_x = fmaxf(mx_x, fabsf((float)tx_));
-                    mx_z = fmaxf(mx_z, fabsf((float)tz_));
-                }
+                vx[r] = mad(mux[r], dxV, vx[r]);
+                vz[r] = mad(muz[r], dzV, vz[r]);
             }
         } else {
             float sp_x = 1.f, sp_z = 1.f;
@@ -481,14 +688,13 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
                 const C *pv = &sV[(lz + HALO) * LW + tx + HALO];
                 const C dxV = dp(pv, 1, c0, c1, c2, c3);
                 const C dzV = dm(pv, LW, c0, c1, c2, c3);
-                const long q = base + (long)iz * px + ix;
-                C tx_ = ST::ld(f.Tx, q, sc_x), tz_ = ST::ld(f.Tz, q, sc_z);
+                C tx_ = vx[r], tz_ = vz[r];
                 if (ix < npml || ix >= nx - npml) {
                     const long pi = (long)shot * g.xpsi_stride + xpsi_idx(ix, iz, nx, npml);
                     C p = mad(pr.x.b[ix], ST::ld(ps.x, pi, sp_x), pr.x.c[ix] * dxV);
                     ST::st(ps.x, pi, p, 1.f / sp_x);
                     tx_ = mad(mux[r], mad(dxV, pr.x.k[ix], p), tx_);
-                    if (ST::scaled) atomicMax(&amax[SC_PV_X], __float_as_uint(fabsf((float)p)));
+                    if (ST::scaled) mp[0] = fmaxf(mp[0], fabsf((float)p));
                 } else {
                     tx_ = mad(mux[r], dxV, tx_);
                 }
@@ -498,22 +704,51 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
                     C p = mad(pr.z.b[iz], ST::ld(ps.z, pi, sp_z), pr.z.c[iz] * dzV);
                     ST::st(ps.z, pi, p, 1.f / sp_z);
                     tz_ = mad(muz[r], mad(dzV, pr.z.k[iz], p), tz_);
-                    if (ST::scaled) atomicMax(&amax[SC_PV_Z], __float_as_uint(fabsf((float)p)));
+                    if (ST::scaled) mp[1] = fmaxf(mp[1], fabsf((float)p));
                 } else {
                     tz_ = mad(muz[r], dzV, tz_);
                 }
-                ST::st(f.Tx, q, tx_, 1.f / sc_x);
-                ST::st(f.Tz, q, tz_, 1.f / sc_z);
-                if (ST::scaled) {
-                    mx_x = fmaxf(mx_x, fabsf((float)tx_));
-                    mx_z = fmaxf(mx_z, fabsf((float)tz_));
+                vx[r] = tx_; vz[r] = tz_;
+            }
+        }
+        // fused SH source, stress taps (Tx, Tz); a point force on V is added
+        // by the velocity pass, which is the only pass that writes V
+        if (ev.clock) {
+            const int key = t.tile * g.nshots + shot;
+            const int e0 = ev.tap_start[key], e1 = ev.tap_start[key + 1];
+            double sv;
+            if (e1 > e0 && src_sample(ev.src[shot], ev.stf, it, sv)) {
+                for (int e = e0; e < e1; e++) {
+                    const TapT tp = ev.taps[e];
+                    const int lx = tp.cell % TX, lz = tp.cell / TX;
+                    if (tp.field == 0 || lx != tx || lz < lz0 || lz >= lz0 + RZ) continue;
+#pragma unroll
+                    for (int r = 0; r < RZ; r++) {
+                        if (lz != lz0 + r) continue;
+                        if (tp.field == 1) vx[r] = apply_tap(vx[r], tp, sv);
+                        else vz[r] = apply_tap(vz[r], tp, sv);
+                    }
                 }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RZ; r++) {
+            const int gz = z0 + lz0 + r;
+            if (gx >= nx || gz >= nz) continue;
+            const long q = base + (long)gz * px + gx;
+            ST::st(f.Tx, q, vx[r], 1.f / sc_x);
+            ST::st(f.Tz, q, vz[r], 1.f / sc_z);
+            if (ST::scaled) {
+                mx_x = fmaxf(mx_x, fabsf((float)vx[r]));
+                mx_z = fmaxf(mx_z, fabsf((float)vz[r]));
             }
         }
     }
     if (ST::scaled) {
         amax_update(&amax[SC_TX], mx_x);
         amax_update(&amax[SC_TZ], mx_z);
+        amax_update(&amax[SC_PV_X], mp[0]);
+        amax_update(&amax[SC_PV_Z], mp[1]);
     }
 }
 
@@ -523,8 +758,8 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
 // ===========================================================================
 template <class S, int RZ, int SPB>
 __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typename Store<S>::C *__restrict__ BV,
-                                                        ShPsi2<S> ps, ShProf2<typename Store<S>::C> pr,
-                                                        Grid g, const float *__restrict__ scl,
+                                                        ShPsi2<S> ps, ShProf2<typename Store<S>::C> pr, Grid g,
+                                                        Work w, Events ev, const float *__restrict__ scl,
                                                         unsigned *__restrict__ amax) {
     using C = typename Store<S>::C;
     using ST = Store<S>;
@@ -533,16 +768,20 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
     __shared__ C sX[(TZ + 2 * HALO) * LW];
     __shared__ C sZ[(TZ + 2 * HALO) * LW];
 
+    TileId t;
+    if (!map_tile<TZ>(w, t)) return;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int x0 = blockIdx.x * TX, z0 = blockIdx.y * TZ;
+    const int x0 = t.x0, z0 = t.z0;
     const int nx = g.nx, nz = g.nz, px = g.px, npml = g.npml;
     const int gx = x0 + tx;
     const int lz0 = ty * RZ;
     const bool interior = tile_interior(g, x0, z0, TZ);
+    const int it = ev.clock ? ev.clock[0] + ev.k : 0;
 
     float sc_v = 1.f, sc_x = 1.f, sc_z = 1.f;
     if (ST::scaled) { sc_v = scl[SC_V]; sc_x = scl[SC_TX]; sc_z = scl[SC_TZ]; }
     float mx_v = 0.f;
+    float mp[2] = {0.f, 0.f}; // psi Tx_x Tz_z
 
     C bv[RZ];
 #pragma unroll
@@ -552,26 +791,51 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
         bv[r] = live ? BV[(long)gz * px + gx] : C(0);
     }
 
-    const int sh0 = blockIdx.z * SPB;
+    const int sh0 = t.group * SPB;
     for (int s = 0; s < SPB; s++) {
         const int shot = sh0 + s;
         if (shot >= g.nshots) break;
         const long base = (long)shot * g.sstride;
-        if (s > 0) __syncthreads();
-        stage<S, TZ, false, true>(sX, f.Tx, base, px, x0, z0, sc_x);
-        stage<S, TZ, true, false>(sZ, f.Tz, base, px, x0, z0, sc_z);
+        Stage<S, TZ, false, true> rX;
+        Stage<S, TZ, true, false> rZ;
+        rX.load(f.Tx, base, px, x0, z0, sc_x);
+        rZ.load(f.Tz, base, px, x0, z0, sc_z);
+        C nv[1][RZ];
+#pragma unroll
+        for (int r = 0; r < RZ; r++) nv[0][r] = ST::ld(f.V, base + (long)(z0 + lz0 + r) * px + gx, sc_v);
         // stress imaging at the surface: the LDS copy of row 0 (LDS row
-        // HALO - z0) takes -Tz[row 1], and its owner writes it back.
-        if (z0 - HALO <= 0 && z0 + TZ + HALO > 0) {
+        // HALO - z0) takes -Tz[row 1], and the tile owning row 0 writes it back
+        const bool img = z0 - HALO <= 0 && z0 + TZ + HALO > 0;
+        C img_v = C(0);
+        if (img && ty == 0) img_v = -ST::ld(f.Tz, base + (long)px + gx, sc_z);
+        if (s > 0) __syncthreads();
+        rX.store(sX);
+        rZ.store(sZ);
+        if (img) {
             __syncthreads();
             if (ty == 0) {
-                const int r0 = HALO - z0; // LDS row of global row 0
-                const C v = -ST::ld(f.Tz, base + (long)px + gx, sc_z);
-                sZ[r0 * LW + tx + HALO] = v;
-                if (z0 == 0 && gx < nx) ST::st(f.Tz, base + gx, v, 1.f / sc_z);
+                sZ[(HALO - z0) * LW + tx + HALO] = img_v;
+                if (z0 == 0 && gx < nx) ST::st(f.Tz, base + gx, img_v, 1.f / sc_z);
             }
         }
         __syncthreads();
+        // fused SH point force (cu_point_force): V += source before the
+        // cell's velocity update, as the reference injects between passes
+        if (ev.clock) {
+            const int key = t.tile * g.nshots + shot;
+            const int e0 = ev.tap_start[key], e1 = ev.tap_start[key + 1];
+            double sv;
+            if (e1 > e0 && src_sample(ev.src[shot], ev.stf, it, sv)) {
+                for (int e = e0; e < e1; e++) {
+                    const TapT tp = ev.taps[e];
+                    const int lx = tp.cell % TX, lz = tp.cell / TX;
+                    if (tp.field != 0 || lx != tx || lz < lz0 || lz >= lz0 + RZ) continue;
+#pragma unroll
+                    for (int r = 0; r < RZ; r++)
+                        if (lz == lz0 + r) nv[0][r] = apply_tap(nv[0][r], tp, sv);
+                }
+            }
+        }
         if (interior) {
             const C c0 = C(1225. / 1024.), c1 = C(245. / 3072.), c2 = C(49. / 5120.), c3 = C(5. / 7168.);
 #pragma unroll
@@ -580,12 +844,10 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
                 const int o = (lz + HALO) * LW + tx + HALO;
                 const C Tx_x = dm(&sX[o], 1, c0, c1, c2, c3);
                 const C Tz_z = dp(&sZ[o], LW, c0, c1, c2, c3);
-                const long q = base + (long)(z0 + lz) * px + gx;
-                C v = ST::ld(f.V, q, sc_v);
+                C v = nv[0][r];
                 v = mad(bv[r], Tx_x, v);
                 v = mad(bv[r], Tz_z, v);
-                ST::st(f.V, q, v, 1.f / sc_v);
-                if (ST::scaled) mx_v = fmaxf(mx_v, fabsf((float)v));
+                nv[0][r] = v;
             }
         } else {
             float sp_x = 1.f, sp_z = 1.f;
@@ -600,14 +862,13 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
                 const int o = (lz + HALO) * LW + tx + HALO;
                 const C Tx_x = dm(&sX[o], 1, c0, c1, c2, c3);
                 const C Tz_z = dp(&sZ[o], LW, c0, c1, c2, c3);
-                const long q = base + (long)iz * px + ix;
-                C v = ST::ld(f.V, q, sc_v);
+                C v = nv[0][r];
                 if (ix < npml || ix >= nx - npml) {
                     const long pi = (long)shot * g.xpsi_stride + xpsi_idx(ix, iz, nx, npml);
                     C p = mad(pr.x.b[ix], ST::ld(ps.x, pi, sp_x), pr.x.c[ix] * Tx_x);
                     ST::st(ps.x, pi, p, 1.f / sp_x);
                     v = mad(bv[r], mad(Tx_x, pr.x.k[ix], p), v);
-                    if (ST::scaled) atomicMax(&amax[SC_PTX_X], __float_as_uint(fabsf((float)p)));
+                    if (ST::scaled) mp[0] = fmaxf(mp[0], fabsf((float)p));
                 } else {
                     v = mad(bv[r], Tx_x, v);
                 }
@@ -617,95 +878,75 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
                     C p = mad(pr.z.b[iz], ST::ld(ps.z, pi, sp_z), pr.z.c[iz] * Tz_z);
                     ST::st(ps.z, pi, p, 1.f / sp_z);
                     v = mad(bv[r], mad(Tz_z, pr.z.k[iz], p), v);
-                    if (ST::scaled) atomicMax(&amax[SC_PTZ_Z], __float_as_uint(fabsf((float)p)));
+                    if (ST::scaled) mp[1] = fmaxf(mp[1], fabsf((float)p));
                 } else {
                     v = mad(bv[r], Tz_z, v);
                 }
-                ST::st(f.V, q, v, 1.f / sc_v);
-                if (ST::scaled) mx_v = fmaxf(mx_v, fabsf((float)v));
+                nv[0][r] = v;
             }
         }
+#pragma unroll
+        for (int r = 0; r < RZ; r++) {
+            const int gz = z0 + lz0 + r;
+            if (gx >= nx || gz >= nz) continue;
+            ST::st(f.V, base + (long)gz * px + gx, nv[0][r], 1.f / sc_v);
+            if (ST::scaled) mx_v = fmaxf(mx_v, fabsf((float)nv[0][r]));
+        }
+        if (ev.clock) fused_record<C, RZ, 1>(ev, it, t.tile * g.nshots + shot, sX, sX, nv, lz0);
     }
-    if (ST::scaled) amax_update(&amax[SC_V], mx_v);
+    if (ST::scaled) {
+        amax_update(&amax[SC_V], mx_v);
+        amax_update(&amax[SC_PTX_X], mp[0]);
+        amax_update(&amax[SC_PTZ_Z], mp[1]);
+    }
 }
 
 // ===========================================================================
-// Source injection (one workgroup): one thread per shot walks its taps in the
-// reference's order (PSV/clean_nobox/point_source/cu_point_source.cu:45-70,
-// :115-158; SH/point_source/cu_point_source.cu:36-64, :100-158, :192-198).
+// Stand-alone source injection (staged entry point fd23_inject): one thread
+// per shot walks its taps in the reference's order
+// (PSV/clean_nobox/point_source/cu_point_source.cu:45-70, :115-158;
+// SH/point_source/cu_point_source.cu:36-64, :100-158, :192-198).
 // ===========================================================================
 struct Tap {
     int field;  // index into the context's field-pointer table
     long off;   // element offset inside the shot (row*px + col)
-    double m;   // moment component (already scaled)
-    double q;   // multiplier: +-1, +-0.25 (P-SV) or +-0.5 (SH)
-    int mode;   // 0: float product then add (P-SV), 1: double sum (SH)
-};
-struct SrcInfo {
-    int ntap, tap0; // taps [tap0, tap0+ntap) of the tap table
-    int lsrc;
-    int clamp;      // 1: it >= lsrc uses stf[lsrc-1] ('p' / stype 13)
-    long stf0;      // offset of this shot's samples in the stf table
+    double m;
+    double q;
+    int mode;
 };
 
 template <class S>
-__global__ void k_inject(S *const *fields, int nfields_per_shot, long sstride, const Tap *__restrict__ taps,
-                         const SrcInfo *__restrict__ src, const double *__restrict__ stf, int nshots,
-                         int it_arg, int *clock, const float *__restrict__ scl) {
-    using C = typename Store<S>::C;
+__global__ void k_inject(S *const *fields, long sstride, const Tap *__restrict__ taps,
+                         const SrcInfo *__restrict__ src, const double *__restrict__ stf, int nshots, int it,
+                         const float *__restrict__ scl) {
     using ST = Store<S>;
-    int it = it_arg;
-    if (clock) it = clock[0];
     for (int shot = threadIdx.x; shot < nshots; shot += blockDim.x) {
         const SrcInfo si = src[shot];
-        int t = it;
-        bool act = si.ntap > 0 && si.lsrc > 0;
-        if (t >= si.lsrc) {
-            if (si.clamp) t = si.lsrc - 1;
-            else act = false;
-        }
-        if (act) {
-            const double sv = stf[si.stf0 + t];
-            for (int i = 0; i < si.ntap; i++) {
-                const Tap tp = taps[si.tap0 + i];
-                S *fp = fields[tp.field];
-                const float sc = ST::scaled ? scl[tp.field] : 1.f;
-                const long q = (long)shot * sstride + tp.off;
-                C v = ST::ld(fp, q, sc);
-                if (tp.mode == 0) {
-                    const C prod = C(sv) * C(tp.m);
-                    v += prod * C(tp.q);
-                } else {
-                    v = C((double)v + tp.q * sv * tp.m);
-                }
-                ST::st(fp, q, v, 1.f / sc);
-            }
-        }
-    }
-    (void)nfields_per_shot;
-    if (clock) {
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            clock[1] = it;
-            clock[0] = it + 1;
+        double sv;
+        if (si.ntap == 0 || !src_sample(si, stf, it, sv)) continue;
+        for (int i = 0; i < si.ntap; i++) {
+            const Tap tp = taps[si.tap0 + i];
+            S *fp = fields[tp.field];
+            const float sc = ST::scaled ? scl[tp.field] : 1.f;
+            const long q = (long)shot * sstride + tp.off;
+            const TapT tt{0, tp.field, tp.m, tp.q, tp.mode};
+            ST::st(fp, q, apply_tap(ST::ld(fp, q, sc), tt, sv), 1.f / sc);
         }
     }
 }
 
 // ===========================================================================
-// Receiver gather (SH/record/cu_record.cu:136-144) into the on-device
-// seismogram buffer [frame][shot][comp][max_rec]; offset -1 reads 0.
+// Stand-alone receiver gather (staged entry point fd23_record),
+// SH/record/cu_record.cu:136-144, into [frame][shot][comp][max_rec];
+// offset -1 reads 0 (flat index outside the grid).
 // ===========================================================================
 template <class S>
 __global__ void k_record(S *const *comps, int ncomp, long sstride, const long *__restrict__ roff,
                          const int *__restrict__ nrec, int max_rec, int nshots,
-                         typename Store<S>::C *__restrict__ seis, int max_frames, int it_arg,
-                         const int *clock, int itrecord, const float *__restrict__ scl,
-                         const int *__restrict__ comp_field) {
+                         typename Store<S>::C *__restrict__ seis, int max_frames, int it, int itrecord,
+                         const float *__restrict__ scl, const int *__restrict__ comp_field) {
     using C = typename Store<S>::C;
     using ST = Store<S>;
-    int it = it_arg;
-    if (clock) it = clock[1];
     if (it % itrecord != 0) return;
     const int frame = it / itrecord;
     if (frame >= max_frames) return;
@@ -723,6 +964,53 @@ __global__ void k_record(S *const *comps, int ncomp, long sstride, const long *_
         }
         seis[(long)frame * total + i] = v;
     }
+}
+
+// ===========================================================================
+// Per-array power-of-two scales of FP16/BF16 storage.  Every few steps the
+// control kernel turns each array's running max |value| into a new scale
+// when the stored max has left [2^lo, 2^hi], aiming at 2^target, and the
+// rescale kernel multiplies the stored data by the (exact) power-of-two
+// ratio of old to new scale.
+// ===========================================================================
+__global__ void k_scale_control(float *scl, unsigned *amax, float *resc, int n, int lo, int hi, int target) {
+    const int f = threadIdx.x;
+    if (f >= n) return;
+    const float m = __uint_as_float(amax[f]);
+    amax[f] = 0u;
+    float r = 1.f;
+    if (m > 0.f && isfinite(m)) {
+        const float cur = scl[f];
+        const float stored = m / cur;
+        if (stored > ldexpf(1.f, hi) || stored < ldexpf(1.f, lo)) {
+            int e;
+            frexpf(m, &e); // m in [2^(e-1), 2^e)
+            const float nw = ldexpf(1.f, e - target);
+            r = cur / nw;
+            scl[f] = nw;
+        }
+    }
+    resc[f] = r;
+}
+
+struct RescaleTable {
+    void *ptr[16];
+    long n[16];
+};
+template <class S>
+__global__ void k_rescale(RescaleTable tab, const float *__restrict__ resc) {
+    const int a = blockIdx.y;
+    const float r = resc[a];
+    if (r == 1.f) return;
+    S *p = (S *)tab.ptr[a];
+    const long n = tab.n[a];
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        Store<S>::st(p, i, Store<S>::ld(p, i, 1.f) * r, 1.f);
+}
+
+// advance the step clock after a captured chunk of steps
+__global__ void k_tick(int *clock, int steps) {
+    if (threadIdx.x == 0) clock[0] += steps;
 }
 
 } // namespace fd23

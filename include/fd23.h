@@ -152,6 +152,9 @@ int fd23_box_configure(fd23_ctx *ctx, int shot, int x0, int z0, int bx, int bz,
                        int incident, double angle_deg, double f0, double t0,
                        double amp, double vp, double vs, double rho,
                        double h, double dt);
+/* residual-grid fields of one shot, U W Txx Tzz Txz (bz*bx each; float, or
+ * double for F64).  Replaces reading BOX's residual FIELD back.          */
+int fd23_box_get_residual(fd23_ctx *ctx, int shot, void *out);
 
 /* ---- tuning and introspection ---------------------------------------- */
 /* tile shape index (0 = default); fd23_autotune times the candidates on
@@ -160,6 +163,13 @@ int fd23_set_tile(fd23_ctx *ctx, int tile);
 int fd23_get_tile(const fd23_ctx *ctx);
 int fd23_num_tiles(void);
 int fd23_autotune(fd23_ctx *ctx, int steps, double *best_ms);
+/* FP16/BF16 storage: per-array power-of-two scales (stored = value/scale);
+ * P-SV order U W Txx Tzz Txz then psi Txx_x Txz_x U_x W_x Tzz_z Txz_z U_z
+ * W_z; SH order V Tx Tz then psi Tx_x V_x Tz_z V_z.  Set before stepping
+ * to override the estimate from the sources; get returns the scales and
+ * the running max |value| per array since the last scale check.        */
+int fd23_set_scales(fd23_ctx *ctx, const float *scales, int n);
+int fd23_get_scales(fd23_ctx *ctx, float *scales, unsigned *amax_bits, int n);
 /* algorithmic HBM bytes of one stress+velocity step for this context   */
 double fd23_bytes_per_step(const fd23_ctx *ctx);
 /* Time nsteps of stress+velocity passes with HIP events on the context

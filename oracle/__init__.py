@@ -481,3 +481,108 @@ def run_sh(sim, nt, src, M, xs, zs, kind=0, rx=None, rz=None, itrecord=1):
         if rx is not None and it % itrecord == 0:
             sv.append(sim.record("V", rx, rz))
     return np.array(sv) if rx is not None else None
+
+
+# --------------------------------------------------------------------------
+# hybrid box coupling (PSV/psv_clean_double/box/box.cpp:10-210 ring
+# arithmetic, fd/cu_fd.cu:93-207 residual grid, main.cpp:153-200 order),
+# with a rectangular box and an analytic plane-wave drive s = -incident
+# --------------------------------------------------------------------------
+class PlaneWaveBox:
+    """Box cells [x0, x0+bx) x [z0, z0+bz); incident 0 = P, 1 = SV."""
+
+    def __init__(self, sim, x0, z0, bx, bz, incident, angle_deg, f0, t0, amp, vp, vs, rho, h, dt):
+        self.sim, self.x0, self.z0, self.bx, self.bz = sim, x0, z0, bx, bz
+        self.h, self.dt, self.f0, self.t0, self.amp = h, dt, f0, t0, amp
+        th = np.deg2rad(angle_deg)
+        self.n = np.array([np.sin(th), -np.cos(th)])
+        mu = rho * vs * vs
+        lam = rho * vp * vp - 2.0 * mu
+        if incident == 0:
+            self.q, self.c = self.n.copy(), vp
+            nx_, nz_ = self.n
+            self.sfac = {"Txx": -(lam + 2 * mu * nx_ * nx_) / vp, "Tzz": -(lam + 2 * mu * nz_ * nz_) / vp,
+                         "Txz": -(2 * mu * nx_ * nz_) / vp}
+        else:
+            self.q, self.c = np.array([np.cos(th), np.sin(th)]), vs
+            (qx, qz), (nx_, nz_) = self.q, self.n
+            self.sfac = {"Txx": -2 * mu * qx * nx_ / vs, "Tzz": -2 * mu * qz * nz_ / vs,
+                         "Txz": -mu * (qx * nz_ + qz * nx_) / vs}
+        corners = [self.n[0] * (x0 + cx * (bx - 1)) * h + self.n[1] * (z0 + cz * (bz - 1)) * h
+                   for cx in (0, 1) for cz in (0, 1)]
+        self.nref = min(corners) - 1.0 * h
+        dt_ = sim.dtype
+        fac = dt / h
+        self.lam, self.mu, self.b = (dt_.type(lam * fac), dt_.type(mu * fac), dt_.type(fac / rho))
+        sl = (slice(z0, z0 + bz), slice(x0, x0 + bx))
+        for k, v in (("BU", self.b), ("BW", self.b), ("MU", self.mu), ("MUA", self.mu), ("LAM", self.lam)):
+            sim.mat[k][sl] = v
+        self.r = {k: np.zeros((bz, bx), dt_) for k in ("U", "W", "Txx", "Tzz", "Txz")}
+        j, i = np.mgrid[0:bz, 0:bx]
+        d = np.minimum(np.minimum(i, j), np.minimum(bx - 1 - i, bz - 1 - j))
+        self.sub, self.add = (d <= 3), (d >= 4) & (d <= 7)
+        self.i, self.j = i, j
+        self.rstress = getattr(lib(), "or_box_stress" + _sfx(dt_))
+        self.rvel = getattr(lib(), "or_box_velocity" + _sfx(dt_))
+        self.rstress.restype = self.rvel.restype = None
+
+    OFF = {"U": (0.5, 0.0), "W": (0.0, -0.5), "Txx": (0.0, 0.0), "Tzz": (0.0, 0.0), "Txz": (0.5, -0.5)}
+
+    def drive(self, name, t):
+        ox, oz = self.OFF[name]
+        x = (self.x0 + self.i + ox) * self.h
+        z = (self.z0 + self.j + oz) * self.h
+        tau = t - (self.n[0] * x + self.n[1] * z - self.nref) / self.c - self.t0
+        a = (np.pi * self.f0 * tau) ** 2
+        g = -self.amp * (1.0 - 2.0 * a) * np.exp(-a)
+        if name == "U":
+            v = g * self.q[0]
+        elif name == "W":
+            v = g * self.q[1]
+        else:
+            v = g * self.sfac[name]
+        return v.astype(self.sim.dtype)
+
+    def _ring(self, names, t, add):
+        sl = (slice(self.z0, self.z0 + self.bz), slice(self.x0, self.x0 + self.bx))
+        m = self.add if add else self.sub
+        for nm in names:
+            s = self.drive(nm, t)
+            a = getattr(self.sim, nm)[sl]
+            if add:
+                a[m] = self.r[nm][m] + s[m]
+            else:
+                self.r[nm][m] = a[m] - s[m]
+
+    def pre_stress(self, it):
+        self._ring(("U", "W"), (it + 0.5) * self.dt, True)
+
+    def post_stress(self, it):
+        self._ring(("U", "W"), (it + 0.5) * self.dt, False)
+        r = self.r
+        self.rstress(ctypes.c_int(self.bx), ctypes.c_int(self.bz), _p(r["U"]), _p(r["W"]), _p(r["Txx"]),
+                     _p(r["Tzz"]), _p(r["Txz"]), _p(np.full((self.bz, self.bx), self.lam)),
+                     _p(np.full((self.bz, self.bx), self.mu)), _p(np.full((self.bz, self.bx), self.mu)))
+        self._ring(("Txx", "Tzz", "Txz"), (it + 1.0) * self.dt, True)
+
+    def post_velocity(self, it):
+        self._ring(("Txx", "Tzz", "Txz"), (it + 1.0) * self.dt, False)
+        r = self.r
+        full = np.full((self.bz, self.bx), self.b)
+        self.rvel(ctypes.c_int(self.bx), ctypes.c_int(self.bz), _p(r["Txx"]), _p(r["Tzz"]), _p(r["Txz"]),
+                  _p(r["U"]), _p(r["W"]), _p(full), _p(full))
+
+
+def run_psv_box(sim, box, nt, rx=None, rz=None, it0=0):
+    """Box loop of PSV/psv_clean_double/main.cpp:153-200 (no point source)."""
+    su, sw = [], []
+    for it in range(it0, it0 + nt):
+        box.pre_stress(it)
+        sim.stress()
+        box.post_stress(it)
+        sim.velocity()
+        if rx is not None:
+            su.append(sim.record("U", rx, rz))
+            sw.append(sim.record("W", rx, rz))
+        box.post_velocity(it)
+    return (np.array(su), np.array(sw)) if rx is not None else (None, None)

@@ -1,0 +1,130 @@
+"""Hybrid box coupling on the GPU vs the oracle's box loop.
+
+The oracle follows PSV/psv_clean_double/main.cpp:153-200 (ring add/sub of
+box/box.cpp:198-210, residual passes of fd/cu_fd.cu:93-207) with the
+plane-wave drive; both sides get the same heterogeneous model, so the
+global grid carries a scattered field.  Tolerances: 1e-5 relative L2 for
+FP32, 1e-10 for FP64.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from fd23_amd import _lib, host
+from fd23_amd.core import Context
+from helpers import layered_model, rel, unpack_cpml
+
+pytestmark = pytest.mark.gpu
+
+NX, NZ, NPML, H, DT = 240, 180, 20, 10.0, 8e-4
+BOX = dict(x0=60, z0=50, bx=120, bz=90)
+MED = dict(vp=3000.0, vs=1732.0, rho=2300.0)
+
+
+def _setup(f64, incident, angle):
+    vp, vs, den = layered_model(NX, NZ, 77, vmin=2000.0, vmax=4500.0)
+    if f64:
+        vp, vs, den = (a.astype(np.float64) for a in (vp, vs, den))
+    mat, _ = host.psv_material(vp, vs, den, H, DT, f64=f64)
+    packed = host.cpml_pack(_lib.WAVE_PSV, NX, NZ, NPML, DT, 1e-6, float(vp.max()) / H, 12.0)
+    wave = dict(incident=incident, angle_deg=angle, f0=12.0, t0=1.5 / 12.0, amp=1.0)
+    return mat, packed, wave
+
+
+def _oracle(mat, packed, wave, f64, nt, rx, rz):
+    dt_ = np.float64 if f64 else np.float32
+    cp = unpack_cpml(packed, _lib.WAVE_PSV, NX, NZ, dt_)
+    sim = O.PSV(NX, NZ, NPML, dict(zip(("BU", "BW", "MU", "MUA", "LAM"), [m.copy() for m in mat])), cp,
+                dtype=dt_)
+    box = O.PlaneWaveBox(sim, BOX["x0"], BOX["z0"], BOX["bx"], BOX["bz"], wave["incident"], wave["angle_deg"],
+                         wave["f0"], wave["t0"], wave["amp"], MED["vp"], MED["vs"], MED["rho"], H, DT)
+    u, w = O.run_psv_box(sim, box, nt, rx, rz)
+    return sim, box, u, w
+
+
+def _gpu(mat, packed, wave, f64, nt, rx, rz, staged):
+    storage = _lib.STORE_F64 if f64 else _lib.STORE_F32
+    ctx = Context(_lib.WAVE_PSV, NX, NZ, NPML, storage=storage, max_frames=nt, max_rec=rx.size)
+    ctx.set_material(mat)
+    ctx.set_cpml(packed)
+    for c in range(2):
+        ctx.set_receivers(0, c, rx, rz)
+    ctx.box_configure(BOX["x0"], BOX["z0"], BOX["bx"], BOX["bz"], wave["incident"], wave["angle_deg"],
+                      wave["f0"], wave["t0"], wave["amp"], MED["vp"], MED["vs"], MED["rho"], H, DT)
+    if staged:
+        for it in range(nt):
+            ctx.step_stress()
+            ctx.step_velocity()
+            ctx.record(it)
+    else:
+        ctx.run(0, nt)
+    ctx.sync()
+    return ctx
+
+
+@pytest.mark.parametrize("f64,incident,angle,staged", [
+    (False, 0, 20.0, False),
+    (False, 1, -15.0, False),
+    (False, 0, 0.0, True),
+    (True, 1, 25.0, False),
+])
+def test_box_matches_oracle(f64, incident, angle, staged):
+    print("case: box f64=%s incident=%d angle=%g staged=%s" % (f64, incident, angle, staged), flush=True)
+    nt = 300
+    rx = np.arange(2, NX - 2, 3, dtype=np.int32)
+    rz = np.full(rx.size, 25, np.int32)
+    mat, packed, wave = _setup(f64, incident, angle)
+    sim, box, u, w = _oracle(mat, packed, wave, f64, nt, rx, rz)
+    tol = 1e-10 if f64 else 1e-5
+    with _gpu(mat, packed, wave, f64, nt, rx, rz, staged) as ctx:
+        for name in ("U", "W", "Txx", "Tzz", "Txz"):
+            e = rel(ctx.field(name), getattr(sim, name))
+            print(name, e, flush=True)
+            assert e < tol, name
+        res = ctx.box_residual()
+        for name in ("U", "W", "Txx", "Tzz", "Txz"):
+            e = rel(res[name], box.r[name])
+            print("resid", name, e, flush=True)
+            assert e < tol, name
+        eu = rel(ctx.seismograms(0, 0, nt), u)
+        ew = rel(ctx.seismograms(0, 1, nt), w)
+        print("seis", eu, ew, flush=True)
+        assert eu < tol and ew < tol
+        assert np.abs(u).max() > 0
+
+
+def test_box_homogeneous_plane_wave():
+    """In the box medium everywhere, the residual grid carries the incident
+    plane wave and almost nothing leaks into the global grid."""
+    print("case: box homogeneous plane wave", flush=True)
+    nt = 300
+    vp = np.full((NZ, NX), MED["vp"], np.float32)
+    vs = np.full((NZ, NX), MED["vs"], np.float32)
+    den = np.full((NZ, NX), MED["rho"], np.float32)
+    mat, _ = host.psv_material(vp, vs, den, H, DT)
+    packed = host.cpml_pack(_lib.WAVE_PSV, NX, NZ, NPML, DT, 1e-6, MED["vp"] / H, 12.0)
+    rx = np.array([10], np.int32)
+    rz = np.array([10], np.int32)
+    for incident in (0, 1):
+        wave = dict(incident=incident, angle_deg=20.0, f0=12.0, t0=1.5 / 12.0, amp=1.0)
+        with _gpu(mat, packed, wave, False, nt, rx, rz, False) as ctx:
+            res = ctx.box_residual()
+            inner = (slice(10, -10), slice(10, -10))
+            peak = np.abs(res["U"]).max()
+            x0, z0, bx, bz = BOX["x0"], BOX["z0"], BOX["bx"], BOX["bz"]
+            outside = np.abs(ctx.field("U"))
+            outside[z0:z0 + bz, x0:x0 + bx] = 0
+            print("incident", incident, "peak", peak, "leak", outside.max() / peak, flush=True)
+            assert peak > 0.1
+            assert outside.max() / peak < 1e-2
+
+
+def test_box_rejects_bad_geometry():
+    with Context(_lib.WAVE_PSV, NX, NZ, NPML) as ctx:
+        with pytest.raises(_lib.Fd23Error):
+            ctx.box_configure(1, 50, 120, 90, 0, 0.0, 12.0, 0.1, 1.0, 3000.0, 1732.0, 2300.0, H, DT)
+        with pytest.raises(_lib.Fd23Error):
+            ctx.box_configure(60, 50, 8, 90, 0, 0.0, 12.0, 0.1, 1.0, 3000.0, 1732.0, 2300.0, H, DT)
+    with Context(_lib.WAVE_SH, NX, NZ, NPML) as ctx:
+        with pytest.raises(_lib.Fd23Error):
+            ctx.box_configure(60, 50, 120, 90, 0, 0.0, 12.0, 0.1, 1.0, 3000.0, 1732.0, 2300.0, H, DT)
