@@ -113,12 +113,122 @@ def cpu_baseline(wl, budget_s=12.0):
                 sample="%d steps of the %dx%d P-SV FP32 loop (oracle C restatement, OpenMP)" % (k, wl.nx, wl.nz))
 
 
+STORAGE = {"f32": 0, "f16": 1, "bf16": 2, "f64": 3}
+
+
+def make_workload(name, storage=None):
+    from fd23_amd import _lib, workloads as W
+
+    if name == "psv2000":
+        wl = W.config_psv_2000x1000()
+    elif name == "batched":
+        wl = W.config_psv_batched(64)
+    elif name == "single1024":
+        wl = W.config_psv_batched(1)
+    elif name == "c4":
+        wl = W.config_psv_precision(_lib.STORE_F32)
+    elif name in ("f16", "bf16"):
+        wl = W.config_psv_precision(STORAGE[name])
+    else:
+        wl = W.config_sh_400x200()
+    if storage is not None:
+        wl.storage = STORAGE[storage]
+    return wl
+
+
+def accuracy_vs_f64(name, storage, nt):
+    """Seismograms of the workload at its storage type against an FP64
+    storage run of the same inputs (FP64 on the GPU, pinned to the FP64
+    oracle by the parity tests): relative L2 and peak error over all shots
+    and components."""
+    out = {}
+    for st in (storage, "f64"):
+        wl = make_workload(name, st).build()
+        ctx = wl.context(max_frames=nt)
+        ctx.run(0, nt, 1)
+        ctx.sync()
+        out[st] = np.stack([ctx.seismograms(s, c, nt).astype(np.float64) for s in range(wl.nshots)
+                            for c in range(ctx.ncomp)])
+        ctx.close()
+    a, b = out[storage], out["f64"]
+    return {"steps": nt, "rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+            "peak": float(np.abs(a - b).max() / np.abs(b).max()), "finite": bool(np.isfinite(a).all())}
+
+
+def calibrate_copy(nbytes=1 << 30, reps=10):
+    """A plain device copy of known size, for scaling HBM counters."""
+    import torch
+
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_bytes_each_way": nbytes, "reps": reps, "ms": el * 1e3,
+                      "GBps_read_plus_write": 2 * nbytes / el / 1e9}), flush=True)
+
+
+def bench_serve(nreq, max_active):
+    """Replay the mixed request stream through the serving engine."""
+    from fd23_amd import _lib
+    from fd23_amd.serve import Server, replay, serving_stream
+
+    stream = serving_stream(nreq)
+    with Server(max_active=max_active, chunk_steps=256, max_batch_shots=64) as srv:
+        recs, wall = replay(srv, stream)
+        st = srv.stats()
+    ok = [r for r in recs if r.get("ok")]
+    lat = np.array([r["latency"] for r in ok])
+    que = np.array([r["queue"] for r in ok])
+    cells = sum(r["cells"] for r in ok)
+    offered = sum(s.cells for _, s in stream) / max(stream[-1][0], 1e-9)
+    return {
+        "metric": "Gcell-updates/s",
+        "value": round(cells / wall / 1e9, 3),
+        "unit": "Gcell/s",
+        "n_gpus": 1,
+        "steps": None,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic request stream (Poisson 40/s, seeded layered + GRF models)",
+        "config": {"workload": "serving_mixed_sh_psv", "requests": nreq, "max_active": max_active,
+                   "chunk_steps": 256},
+        "requests_per_s": round(len(ok) / wall, 3),
+        "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
+                      "max": round(float(lat.max()), 4)},
+        "queue_s": {"p50": round(float(np.percentile(que, 50)), 4), "p99": round(float(np.percentile(que, 99)), 4)},
+        "offered_gcell_per_s": round(offered / 1e9, 2),
+        "wall_s": round(wall, 3),
+        "completed": len(ok),
+        "failed": len(recs) - len(ok),
+        "finite": all(r["finite"] for r in ok),
+        "engine": {k: st[k] for k in ("launches", "merged", "peak_active")},
+        "mix": {"psv": sum(r["wave"] == _lib.WAVE_PSV for r in recs), "box": sum(r["box"] for r in recs),
+                "f16": sum(r["storage"] == _lib.STORE_F16 for r in recs)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="psv2000", choices=["psv2000", "batched", "f16", "bf16", "sh400"])
+    ap.add_argument("--workload", default="psv2000",
+                    choices=["psv2000", "batched", "single1024", "c4", "f16", "bf16", "sh400", "serve"])
+    ap.add_argument("--requests", type=int, default=40, help="serve: requests in the stream")
+    ap.add_argument("--max-active", type=int, default=8, help="serve: resident contexts")
+    ap.add_argument("--storage", choices=sorted(STORAGE), default=None)
+    ap.add_argument("--accuracy", action="store_true",
+                    help="also report seismogram error against an FP64 run of the same inputs")
+    ap.add_argument("--accuracy-steps", type=int, default=0, help="steps of the accuracy runs (default nt)")
+    ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
@@ -126,20 +236,16 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (shots and requests shard "
               "across GPUs with no communication); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+    if args.workload == "serve":
+        print(json.dumps(bench_serve(args.requests, args.max_active)), flush=True)
+        return
 
-    from fd23_amd import _lib, workloads as W
+    from fd23_amd import _lib
 
-    if args.workload == "psv2000":
-        wl = W.config_psv_2000x1000()
-    elif args.workload == "batched":
-        wl = W.config_psv_batched(64)
-    elif args.workload == "f16":
-        wl = W.config_psv_precision(_lib.STORE_F16)
-    elif args.workload == "bf16":
-        wl = W.config_psv_precision(_lib.STORE_BF16)
-    else:
-        wl = W.config_sh_400x200()
-    wl.build()
+    wl = make_workload(args.workload, args.storage).build()
     r = run_gpu(wl, args.steps, args.warmup, autotune=not args.no_autotune)
     cells = float(wl.nshots) * wl.nx * wl.nz * args.steps
     gcell = cells / r["elapsed"] / 1e9
@@ -151,6 +257,7 @@ def main():
         name, nbytes, kms = "velocity", vb, r["vel_ms"]
     achieved = nbytes / (kms * 1e-3) / 1e9
     step_bytes = sb + vb
+    sname = {v: k for k, v in STORAGE.items()}[wl.storage]
     out = {
         "metric": "Gcell-updates/s",
         "value": round(gcell, 3),
@@ -162,8 +269,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {_lib.STORE_F32: "f32", _lib.STORE_F16: "f32 (f16 storage)",
-                  _lib.STORE_BF16: "f32 (bf16 storage)", _lib.STORE_F64: "f64"}[wl.storage],
+        "dtype": "f32" if sname in ("f32", "f16", "bf16") else "f64",
+        "storage": sname,
         "data": "synthetic (seeded layered + GRF model, Ricker source)",
         "config": {"workload": wl.name, "nx": wl.nx, "nz": wl.nz, "nshots": wl.nshots, "npml": wl.npml,
                    "dx": wl.dx, "dt": wl.dt, "nt": wl.nt, "tile": r["tile"]},
@@ -171,9 +278,12 @@ def main():
         "hbm_fraction_step": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel_ms": round(kms, 5), "algorithmic_bytes": nbytes},
+                     "kernel_ms": round(kms, 5), "algorithmic_bytes": nbytes,
+                     "stress_ms": round(r["stress_ms"], 5), "velocity_ms": round(r["vel_ms"], 5)},
         "finite": r["finite"],
     }
+    if args.accuracy:
+        out["error_vs_fp64"] = accuracy_vs_f64(args.workload, sname, args.accuracy_steps or wl.nt)
     if not args.no_cpu and wl.wave == _lib.WAVE_PSV and wl.nshots == 1:
         out["cpu_baseline"] = cpu_baseline(wl)
     print(json.dumps(out), flush=True)

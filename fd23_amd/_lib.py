@@ -1,4 +1,5 @@
-"""ctypes binding of libfd23.so (the C ABI in include/fd23.h).
+"""ctypes binding of libfd23.so (the C ABI in include/fd23.h and
+include/fd23_serve.h).
 
 The library is built in-tree by ``fd23_amd.build`` (or
 ``__graft_entry__.build()``).  Loading fails loudly when it is missing:
@@ -31,6 +32,31 @@ class Desc(ctypes.Structure):
                 ("topabsorb", c_int), ("nshots", c_int), ("max_frames", c_int), ("max_rec", c_int),
                 ("stream", c_vp)]
 
+
+class ServerDesc(ctypes.Structure):
+    _fields_ = [("hbm_budget", c_sz), ("max_active", c_int), ("chunk_steps", c_int), ("max_batch_shots", c_int),
+                ("autotune", c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("wave", c_int), ("storage", c_int), ("nx", c_int), ("nz", c_int), ("npml", c_int),
+                ("topabsorb", c_int), ("nshots", c_int), ("nt", c_int), ("itrecord", c_int),
+                ("dx", c_dbl), ("dt", c_dbl), ("material", ctypes.POINTER(c_vp)), ("cpml", c_vp),
+                ("src_kind", c_int), ("xs", c_vp), ("zs", c_vp), ("stf", c_vp), ("lsrc", c_int), ("M", c_vp),
+                ("nrec", c_int), ("rx", c_vp), ("rz", c_vp), ("box", c_int), ("box_x0", c_int),
+                ("box_z0", c_int), ("box_bx", c_int), ("box_bz", c_int), ("box_incident", c_int),
+                ("box_angle", c_dbl), ("box_f0", c_dbl), ("box_t0", c_dbl), ("box_amp", c_dbl),
+                ("box_vp", c_dbl), ("box_vs", c_dbl), ("box_rho", c_dbl), ("model_key", ctypes.c_ulonglong)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong), ("failed", ctypes.c_longlong),
+                ("queued", ctypes.c_longlong), ("running", ctypes.c_longlong), ("launches", ctypes.c_longlong),
+                ("merged", ctypes.c_longlong), ("peak_active", ctypes.c_longlong), ("bytes_in_use", c_sz),
+                ("budget", c_sz), ("cell_updates", c_dbl)]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 SIGNATURES = {
     "fd23_ctx_create": (c_int, [ctypes.POINTER(Desc), ctypes.POINTER(c_vp)]),
@@ -86,6 +112,13 @@ SIGNATURES = {
     "fd23_par_double": (c_int, [c_vp, ctypes.c_char_p, P_dbl]),
     "fd23_par_string": (c_int, [c_vp, ctypes.c_char_p, ctypes.c_char_p, c_int]),
     "fd23_set_scales": (c_int, [c_vp, c_vp, c_int]),
+    "fd23_server_create": (c_int, [ctypes.POINTER(ServerDesc), ctypes.POINTER(c_vp)]),
+    "fd23_server_destroy": (c_int, [c_vp]),
+    "fd23_server_submit": (c_int, [c_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_longlong)]),
+    "fd23_server_wait": (c_int, [c_vp, ctypes.c_longlong, c_dbl, P_int]),
+    "fd23_server_result_bytes": (c_sz, [c_vp, ctypes.c_longlong]),
+    "fd23_server_result": (c_int, [c_vp, ctypes.c_longlong, c_vp, c_sz, P_dbl, P_dbl]),
+    "fd23_server_stats_get": (c_int, [c_vp, ctypes.POINTER(ServerStats)]),
     "fd23_get_scales": (c_int, [c_vp, c_vp, c_vp, c_int]),
 }
 

@@ -899,9 +899,7 @@ extern "C" int fd23_run(fd23_ctx *c, int it0, int nt, int itrecord) {
     if (nt < 0 || it0 < 0 || itrecord < 1) return fail(FD23_EINVAL, "bad run range");
     if ((r = build_events(c))) return r;
     if ((r = init_scales(c))) return r;
-    int clk[2] = {it0, it0};
-    HIPCHK(hipMemcpyAsync(c->clock, clk, sizeof clk, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream)); // clk is a stack buffer
+    hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(64), 0, c->stream, c->clock, it0);
     const int full = nt / kGraphSteps, rest = nt % kGraphSteps;
     if (full > 0) {
         hipGraphExec_t g;
@@ -1091,28 +1089,31 @@ extern "C" double fd23_bytes_per_step(const fd23_ctx *c) {
 extern "C" int fd23_time_passes(fd23_ctx *c, int nsteps, double *stress_ms, double *velocity_ms) {
     int r = ready(c);
     if (r) return r;
-    if (nsteps < 1) return fail(FD23_EINVAL, "nsteps must be >= 1");
-    hipEvent_t e0, e1, e2;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventCreate(&e2));
-    double ts = 0, tv = 0;
+    if (nsteps < 1 || nsteps > 4096) return fail(FD23_EINVAL, "nsteps must be in [1, 4096]");
+    // all passes are queued back to back with an event between each pair,
+    // and the host waits once at the end: the stream never drains between
+    // kernels, so each interval is one kernel plus its dispatch, as in the
+    // graph-replayed loop
+    std::vector<hipEvent_t> ev(2 * nsteps + 1);
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    hipEventRecord(ev[0], c->stream);
     for (int i = 0; i < nsteps; i++) {
-        hipEventRecord(e0, c->stream);
         dispatch_pass(c, true, kNoEvents);
-        hipEventRecord(e1, c->stream);
+        hipEventRecord(ev[2 * i + 1], c->stream);
         dispatch_pass(c, false, kNoEvents);
-        hipEventRecord(e2, c->stream);
-        hipEventSynchronize(e2);
-        float a, b;
-        hipEventElapsedTime(&a, e0, e1);
-        hipEventElapsedTime(&b, e1, e2);
+        hipEventRecord(ev[2 * i + 2], c->stream);
+    }
+    const hipError_t se = hipEventSynchronize(ev.back());
+    double ts = 0, tv = 0;
+    for (int i = 0; i < nsteps && se == hipSuccess; i++) {
+        float a = 0, b = 0;
+        hipEventElapsedTime(&a, ev[2 * i], ev[2 * i + 1]);
+        hipEventElapsedTime(&b, ev[2 * i + 1], ev[2 * i + 2]);
         ts += a;
         tv += b;
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    hipEventDestroy(e2);
+    for (auto &e : ev) hipEventDestroy(e);
+    if (se != hipSuccess) return fail(FD23_EHIP, "%s", hipGetErrorString(se));
     if (stress_ms) *stress_ms = ts / nsteps;
     if (velocity_ms) *velocity_ms = tv / nsteps;
     return check_launch();

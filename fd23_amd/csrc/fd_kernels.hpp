@@ -1013,4 +1013,9 @@ __global__ void k_tick(int *clock, int steps) {
     if (threadIdx.x == 0) clock[0] += steps;
 }
 
+// set the step clock in stream order (no host round trip)
+__global__ void k_set_clock(int *clock, int it0) {
+    if (threadIdx.x == 0) clock[0] = it0;
+}
+
 } // namespace fd23

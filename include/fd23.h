@@ -172,8 +172,10 @@ int fd23_set_scales(fd23_ctx *ctx, const float *scales, int n);
 int fd23_get_scales(fd23_ctx *ctx, float *scales, unsigned *amax_bits, int n);
 /* algorithmic HBM bytes of one stress+velocity step for this context   */
 double fd23_bytes_per_step(const fd23_ctx *ctx);
-/* Time nsteps of stress+velocity passes with HIP events on the context
- * stream; fills per-kernel average milliseconds.                        */
+/* Time nsteps (1..4096) stress+velocity steps with HIP events on the
+ * context stream, queued back to back; fills per-pass average
+ * milliseconds.  The passes advance the fields (no injection, no
+ * recording).                                                          */
 int fd23_time_passes(fd23_ctx *ctx, int nsteps, double *stress_ms,
                      double *velocity_ms);
 

@@ -4,7 +4,8 @@ The oracle follows PSV/psv_clean_double/main.cpp:153-200 (ring add/sub of
 box/box.cpp:198-210, residual passes of fd/cu_fd.cu:93-207) with the
 plane-wave drive; both sides get the same heterogeneous model, so the
 global grid carries a scattered field.  Tolerances: 1e-5 relative L2 for
-FP32, 1e-10 for FP64.
+FP32, 1e-10 for FP64 (global fields measured against the total-field
+scale: inside the box they hold total minus incident, a small difference).
 """
 import numpy as np
 import pytest
@@ -76,21 +77,23 @@ def test_box_matches_oracle(f64, incident, angle, staged):
     mat, packed, wave = _setup(f64, incident, angle)
     sim, box, u, w = _oracle(mat, packed, wave, f64, nt, rx, rz)
     tol = 1e-10 if f64 else 1e-5
+    errs = {}
     with _gpu(mat, packed, wave, f64, nt, rx, rz, staged) as ctx:
-        for name in ("U", "W", "Txx", "Tzz", "Txz"):
-            e = rel(ctx.field(name), getattr(sim, name))
-            print(name, e, flush=True)
-            assert e < tol, name
         res = ctx.box_residual()
         for name in ("U", "W", "Txx", "Tzz", "Txz"):
-            e = rel(res[name], box.r[name])
-            print("resid", name, e, flush=True)
-            assert e < tol, name
-        eu = rel(ctx.seismograms(0, 0, nt), u)
-        ew = rel(ctx.seismograms(0, 1, nt), w)
-        print("seis", eu, ew, flush=True)
-        assert eu < tol and ew < tol
-        assert np.abs(u).max() > 0
+            errs["resid " + name] = rel(res[name], box.r[name])
+            # inside the box the global grid holds total minus incident, a
+            # small difference of the residual grid's total field: measure
+            # it against the total-field scale, not against itself
+            g, o = ctx.field(name), getattr(sim, name)
+            scale = max(np.linalg.norm(o), np.linalg.norm(box.r[name]))
+            errs["field " + name] = float(np.linalg.norm(g.astype(np.float64) - o) / scale)
+        errs["seis U"] = rel(ctx.seismograms(0, 0, nt), u)
+        errs["seis W"] = rel(ctx.seismograms(0, 1, nt), w)
+    print(errs, flush=True)
+    assert np.abs(u).max() > 0
+    bad = {k: v for k, v in errs.items() if not v < tol}
+    assert not bad, bad
 
 
 def test_box_homogeneous_plane_wave():
@@ -109,7 +112,6 @@ def test_box_homogeneous_plane_wave():
         wave = dict(incident=incident, angle_deg=20.0, f0=12.0, t0=1.5 / 12.0, amp=1.0)
         with _gpu(mat, packed, wave, False, nt, rx, rz, False) as ctx:
             res = ctx.box_residual()
-            inner = (slice(10, -10), slice(10, -10))
             peak = np.abs(res["U"]).max()
             x0, z0, bx, bz = BOX["x0"], BOX["z0"], BOX["bx"], BOX["bz"]
             outside = np.abs(ctx.field("U"))
