@@ -105,6 +105,8 @@ SIGNATURES = {
     "fd23_model_read": (c_int, [ctypes.c_char_p, c_int, c_int, c_dbl, c_vp, c_vp, c_vp]),
     "fd23_isis_write": (c_int, [ctypes.c_char_p, c_vp, c_int, c_int, c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_dbl,
                                 c_int, c_int, c_int]),
+    "fd23_line_receivers": (c_int, [c_int, c_int, c_int, c_int, c_int, c_vp, c_vp]),
+    "fd23_station_read": (c_int, [ctypes.c_char_p, c_vp, c_vp, c_int, P_int]),
     "fd23_par_open": (c_int, [c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(c_vp)]),
     "fd23_par_close": (None, [c_vp]),
     "fd23_par_int": (c_int, [c_vp, ctypes.c_char_p, P_int]),

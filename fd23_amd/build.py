@@ -47,7 +47,32 @@ def build(verbose: bool = False) -> str:
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"]
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    build_cli(verbose)
     return LIB
+
+
+CLI_NAMES = ("nbpsv2d", "nbpsv2d_topabsorb", "nbpsv2d_exp", "nbpsv2d_exp_topabsorb", "sh2d", "sh2d_topabsorb")
+
+
+def build_cli(verbose: bool = False) -> str:
+    """The par-file driver, linked against libfd23.so, plus one link per
+    reference program name (the name picks the variant)."""
+    bindir = os.path.join(HERE, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    exe = os.path.join(bindir, "fd23")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "fd23_cli.cpp"), "-o", exe + ".tmp", "-L", HERE, "-lfd23",
+           "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(exe + ".tmp", exe)
+    for name in CLI_NAMES:
+        link = os.path.join(bindir, name)
+        if os.path.lexists(link):
+            os.remove(link)
+        os.symlink("fd23", link)
+    return exe
 
 
 if __name__ == "__main__":

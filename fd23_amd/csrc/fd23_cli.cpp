@@ -1,0 +1,408 @@
+// Command-line driver with the reference programs' interface, on top of the
+// C ABI only.  One executable; the name it is started under picks the
+// reference build variant (PSV/clean_nobox/Makefile, SH/Makefile):
+//   nbpsv2d, nbpsv2d_topabsorb, nbpsv2d_exp, nbpsv2d_exp_topabsorb, sh2d,
+//   sh2d_topabsorb; under any other name the par keys wave=psv|sh,
+//   topabsorb=0|1 and explosion=0|1 choose.
+//
+// P-SV follows PSV/clean_nobox/main.cpp:24-330 and param/param.cpp:94-193;
+// SH follows SH/main.cpp:20-200 and SH/param/param.cpp:12-84:
+//   model (.mdl polygons or raster files) -> material -> C-PML -> source
+//   wavelet (Gaussian or trapezoid; running sum for srctype=p / stype=13)
+//   -> receivers (a line or station files, pairs x and x-1) -> optional
+//   restart from state%06d -> time loop (graph-replayed chunks that end at
+//   each snapshot and progress-print step) -> optional state%06d ->
+//   raw time-major seismograms {output}_U/_W or _V -> ISIS files.
+// Keys the one-GPU build reads but does not need: gpuid, ngpu, usetable.
+// Additional key: storage=f32|f16|bf16|f64 (default f32).  usepunch=1
+// (first-arrival block skipping) is not supported and is rejected.
+#include "fd23.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <iomanip>
+#include <iterator>
+#include <set>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct Par {
+    fd23_par *p = nullptr;
+    ~Par() {
+        if (p) fd23_par_close(p);
+    }
+    bool has_i(const char *k, int &v) { return fd23_par_int(p, k, &v) == 1; }
+    bool has_f(const char *k, float &v) { return fd23_par_float(p, k, &v) == 1; }
+    bool has_s(const char *k, std::string &v) {
+        char buf[1024];
+        if (fd23_par_string(p, k, buf, sizeof buf) != 1) return false;
+        v = buf;
+        return true;
+    }
+    // mstpar: a missing key ends the program, as getpar's mstpar does
+    int mi(const char *k) {
+        int v;
+        if (!has_i(k, v)) missing(k);
+        return v;
+    }
+    float mf(const char *k) {
+        float v;
+        if (!has_f(k, v)) missing(k);
+        return v;
+    }
+    std::string ms(const char *k) {
+        std::string v;
+        if (!has_s(k, v)) missing(k);
+        return v;
+    }
+    [[noreturn]] static void missing(const char *k) {
+        fprintf(stderr, "mstpar: must specify value for '%s'\n", k);
+        exit(-1);
+    }
+};
+
+[[noreturn]] void die(const char *what, int code) {
+    const char *e = fd23_last_error();
+    fprintf(stderr, "%s failed (%d): %s\n", what, code, e ? e : "");
+    exit(-1);
+}
+#define CHECK(call)                                                                                                    \
+    do {                                                                                                               \
+        int rc_ = (call);                                                                                              \
+        if (rc_ != FD23_OK) die(#call, rc_);                                                                           \
+    } while (0)
+
+void write_file(const std::string &name, const void *data, size_t bytes) {
+    std::ofstream f(name, std::ios::binary);
+    f.write((const char *)data, (std::streamsize)bytes);
+    if (!f) {
+        fprintf(stderr, "cannot write %s\n", name.c_str());
+        exit(-1);
+    }
+}
+
+std::string basename_of(const char *p) {
+    std::string s(p ? p : "");
+    const size_t k = s.find_last_of('/');
+    return k == std::string::npos ? s : s.substr(k + 1);
+}
+
+// SH needsnap: extra snapshot steps listed in a file (SH/record/needsnap.cpp)
+std::set<int> snap_list(const std::string &file) {
+    std::set<int> s;
+    if (file.empty()) return s;
+    std::ifstream f(file);
+    if (!f.is_open()) {
+        fprintf(stderr, "%s Not exist!", file.c_str());
+        exit(-1);
+    }
+    int v;
+    while (f >> v) s.insert(v);
+    return s;
+}
+
+} // namespace
+
+int main(int argc, char **argv) {
+    const std::string prog = basename_of(argv[0]);
+    Par par;
+    CHECK(fd23_par_open(argc, (const char *const *)argv, &par.p));
+
+    bool psv = true, top = false, explosion = false;
+    if (prog.rfind("nbpsv2d", 0) == 0 || prog.rfind("sh2d", 0) == 0) {
+        psv = prog.rfind("nbpsv2d", 0) == 0;
+        top = prog.find("topabsorb") != std::string::npos;
+        explosion = prog.find("_exp") != std::string::npos;
+    } else {
+        std::string w;
+        if (par.has_s("wave", w)) psv = !(w == "sh" || w == "SH");
+        int v;
+        if (par.has_i("topabsorb", v)) top = v != 0;
+        if (par.has_i("explosion", v)) explosion = v != 0;
+    }
+    int storage = FD23_STORE_F32;
+    {
+        std::string st;
+        if (par.has_s("storage", st)) {
+            if (st == "f16") storage = FD23_STORE_F16;
+            else if (st == "bf16") storage = FD23_STORE_BF16;
+            else if (st == "f64") storage = FD23_STORE_F64;
+            else if (st != "f32") Par::missing("storage (f32|f16|bf16|f64)");
+        }
+    }
+
+    // ---- parameters (P-SV param.cpp:94-193, SH param.cpp:12-84) --------
+    int restart = 0, nt0 = 0, savestate = 0, itprint = 100, itrecord = 1, ntsnap;
+    int v;
+    if (psv) {
+        par.has_i("restart", restart);
+        if (restart) nt0 = par.mi("nt0");
+        par.has_i("savestate", savestate);
+    } else {
+        if (par.mi("usepunch")) {
+            fprintf(stderr, "usepunch=1 (first-arrival block skipping) is not supported by this build\n");
+            return -1;
+        }
+        par.mi("ngpu");
+    }
+    const int nx = par.mi("nx"), nz = par.mi("nz"), nt = par.mi("nt");
+    const float h = par.mf("h"), dt = par.mf("dt");
+    char srctype = '1';
+    int stype = 3;
+    std::string s;
+    if (psv) {
+        if (par.has_s("srctype", s) && !s.empty()) srctype = s[0];
+    } else {
+        par.has_i("stype", stype);
+    }
+    const std::string srctime = par.ms(psv ? "srctime" : "sourcetime");
+    float trap1 = 0, trap2 = 0, trap3 = 0, alpha = 0;
+    if (srctime[0] == 'T' || srctime[0] == 't') {
+        trap1 = par.mf("trap1");
+        trap2 = par.mf("trap2");
+        trap3 = par.mf("trap3");
+    }
+    if (srctime[0] == 'G' || srctime[0] == 'g') alpha = par.mf("alpha");
+    const float strike = par.mf("strike"), dip = par.mf("dip"), rake = par.mf("rake"), azimuth = par.mf("azimuth");
+    const int xs = par.mi("xs"), zs = par.mi("zs");
+    par.has_i("itprint", itprint);
+    const int npml = par.mi("npml");
+    const float pml_dt = par.mf("pml_dt"), pml_v = par.mf("pml_v"), pml_r = par.mf("pml_r"),
+                pml_fc = par.mf("pml_fc");
+    if (psv)
+        par.has_i("itrecord", itrecord);
+    else
+        itrecord = par.mi("itrecord");
+    const std::string output = par.ms("output");
+    // receivers per component
+    std::vector<std::vector<int>> rx(psv ? 2 : 1), rz(psv ? 2 : 1);
+    std::string sta[2];
+    bool use_sta = false;
+    if (psv) {
+        par.has_s("U_sta_file", sta[0]);
+        par.has_s("W_sta_file", sta[1]);
+        use_sta = !(sta[0].empty() && sta[1].empty());
+    } else {
+        par.has_s("V_sta_file", sta[0]);
+        use_sta = !sta[0].empty();
+    }
+    for (size_t c = 0; c < rx.size(); c++) {
+        if (use_sta) {
+            int n = 0;
+            CHECK(fd23_station_read(sta[c].c_str(), nullptr, nullptr, 0, &n));
+            rx[c].resize(n);
+            rz[c].resize(n);
+            CHECK(fd23_station_read(sta[c].c_str(), rx[c].data(), rz[c].data(), n, &n));
+        }
+    }
+    if (!use_sta) {
+        const int nrec = par.mi("nrec"), ixrec0 = par.mi("ixrec0");
+        int iz0[2];
+        if (psv) {
+            iz0[0] = par.mi("izrec0_u");
+            iz0[1] = par.mi("izrec0_w");
+        } else {
+            iz0[0] = par.mi("izrec0_v");
+        }
+        const int idxrec = par.mi("idxrec"), idzrec = par.mi("idzrec");
+        for (size_t c = 0; c < rx.size(); c++) {
+            rx[c].resize(2 * nrec);
+            rz[c].resize(2 * nrec);
+            CHECK(fd23_line_receivers(nrec, ixrec0, iz0[c], idxrec, idzrec, rx[c].data(), rz[c].data()));
+        }
+    }
+    const std::string model = par.ms("model");
+    if (psv) par.has_i("usetable", v);
+    ntsnap = par.mi("ntsnap");
+    std::string snapfile;
+    if (!psv) par.has_s("ntsnapfile", snapfile);
+    if (psv && par.has_i("usepunch", v) && v == 1) {
+        fprintf(stderr, "usepunch=1 (first-arrival block skipping) is not supported by this build\n");
+        return -1;
+    }
+    const std::set<int> extra_snaps = snap_list(snapfile);
+
+    // ---- model, material, C-PML -----------------------------------------
+    const size_t N = (size_t)nx * nz;
+    std::vector<float> vp(N), vs(N), den(N);
+    CHECK(fd23_model_read(model.c_str(), nx, nz, h, vp.data(), vs.data(), den.data()));
+    const bool f64 = storage == FD23_STORE_F64;
+    const int nmat = psv ? 5 : 3;
+    std::vector<std::vector<float>> matf(nmat, std::vector<float>(N));
+    std::vector<std::vector<double>> matd;
+    double stab = 0;
+    int mr;
+    if (psv && f64) {
+        matd.assign(nmat, std::vector<double>(N));
+        std::vector<double> a(vp.begin(), vp.end()), b(vs.begin(), vs.end()), c(den.begin(), den.end());
+        mr = fd23_psv_material_f64(nx, nz, h, dt, a.data(), b.data(), c.data(), matd[0].data(), matd[1].data(),
+                                   matd[2].data(), matd[3].data(), matd[4].data(), &stab);
+    } else if (psv) {
+        mr = fd23_psv_material(nx, nz, h, dt, vp.data(), vs.data(), den.data(), matf[0].data(), matf[1].data(),
+                               matf[2].data(), matf[3].data(), matf[4].data(), &stab);
+    } else {
+        mr = fd23_sh_material(nx, nz, h, dt, vs.data(), den.data(), matf[0].data(), matf[1].data(), matf[2].data(),
+                              &stab);
+    }
+    if (mr == FD23_ESTAB) {
+        fprintf(stderr, "unstable: stability number %g > 1\n", stab);
+        return -1;
+    }
+    if (mr) die("material", mr);
+    if (!psv && f64) { // SH material is computed in float; widen it
+        matd.assign(nmat, std::vector<double>(N));
+        for (int m = 0; m < nmat; m++)
+            for (size_t q = 0; q < N; q++) matd[m][q] = matf[m][q];
+    }
+    std::vector<const void *> matp(nmat);
+    for (int m = 0; m < nmat; m++) matp[m] = f64 ? (const void *)matd[m].data() : (const void *)matf[m].data();
+    const int wave = psv ? FD23_WAVE_PSV : FD23_WAVE_SH;
+    std::vector<double> cpml(fd23_cpml_pack_len(wave, nx, nz));
+    CHECK(fd23_cpml_pack(wave, nx, nz, npml, pml_dt, pml_r, pml_v, pml_fc, cpml.data()));
+
+    // ---- source ----------------------------------------------------------
+    std::vector<double> stf(100000);
+    int lsrc = 0;
+    if (srctime[0] == 'T' || srctime[0] == 't')
+        CHECK(fd23_stf_trapezoid(dt, trap1, trap2, trap3, stf.data(), (int)stf.size(), &lsrc));
+    else if (srctime[0] == 'G' || srctime[0] == 'g')
+        CHECK(fd23_stf_gaussian((int)alpha, stf.data(), (int)stf.size(), &lsrc));
+    else {
+        fprintf(stderr, "srctime must be Gaussian or Trapezoid\n");
+        return -1;
+    }
+    stf.resize(lsrc);
+    int kind;
+    if (psv) {
+        if (srctype == '1')
+            kind = FD23_SRC_LINE;
+        else if (srctype == 'p' || srctype == 'P')
+            kind = FD23_SRC_P;
+        else {
+            fprintf(stderr, "Wrong srctype! Specify 1 or p.");
+            return -1;
+        }
+    } else {
+        kind = stype == 13 ? FD23_SRC_P : FD23_SRC_LINE; // other stypes fall back to 3 (SH/main.cpp:104-112)
+    }
+    if (kind == FD23_SRC_P) { // running sum in float (main.cpp:94-99)
+        float sum = 0;
+        for (auto &x : stf) {
+            sum += (float)x;
+            x = sum;
+        }
+    }
+    double M[3];
+    if (psv)
+        CHECK(fd23_psv_moment(kind, explosion ? 1 : 0, strike, dip, rake, azimuth, h, dt, M));
+    else
+        CHECK(fd23_sh_moment(kind, strike, dip, rake, azimuth, h, dt, M));
+
+    // ---- context ----------------------------------------------------------
+    const int first_frame = (nt0 + itrecord - 1) / itrecord;
+    const int end_frame = (nt0 + nt + itrecord - 1) / itrecord;
+    int max_rec = 0;
+    for (auto &r : rx) max_rec = std::max<int>(max_rec, (int)r.size());
+    fd23_desc d{};
+    d.wave = wave;
+    d.storage = storage;
+    d.nx = nx;
+    d.nz = nz;
+    d.npml = npml;
+    d.topabsorb = top ? 1 : 0;
+    d.nshots = 1;
+    d.max_frames = end_frame;
+    d.max_rec = max_rec;
+    fd23_ctx *ctx = nullptr;
+    CHECK(fd23_ctx_create(&d, &ctx));
+    CHECK(fd23_set_material(ctx, matp.data()));
+    CHECK(fd23_set_cpml(ctx, cpml.data()));
+    CHECK(fd23_set_source(ctx, 0, kind, xs, zs, stf.data(), lsrc, M));
+    for (size_t c = 0; c < rx.size(); c++)
+        CHECK(fd23_set_receivers(ctx, 0, (int)c, rx[c].data(), rz[c].data(), (int)rx[c].size()));
+
+    const size_t sbytes = fd23_state_bytes(ctx);
+    if (restart) {
+        char name[64];
+        snprintf(name, sizeof name, "state%06d", nt0 - 1);
+        std::ifstream f(name, std::ios::binary);
+        std::vector<char> buf(sbytes);
+        f.read(buf.data(), (std::streamsize)sbytes);
+        if (!f) {
+            fprintf(stderr, "cannot read %s\n", name);
+            return -1;
+        }
+        CHECK(fd23_set_state(ctx, 0, buf.data(), sbytes));
+    }
+
+    // ---- time loop -------------------------------------------------------
+    const int csize = f64 ? 8 : 4;
+    std::vector<char> snap(N * csize);
+    const char *snames[2] = {psv ? "U" : "V", "W"};
+    const int sfields[2] = {psv ? FD23_F_U : FD23_F_V, FD23_F_W};
+    const auto t0 = std::chrono::steady_clock::now();
+    int it = nt0;
+    while (it < nt0 + nt) {
+        // run through the next step that snapshots or prints
+        int e = it;
+        while (e < nt0 + nt - 1 && !((e % ntsnap == 0 || extra_snaps.count(e)) && e != 0) && e % itprint != 0) e++;
+        CHECK(fd23_run(ctx, it, e - it + 1, itrecord));
+        if ((e % ntsnap == 0 || extra_snaps.count(e)) && e != 0) {
+            CHECK(fd23_sync(ctx));
+            std::cout << e << std::endl;
+            for (int k = 0; k < (psv ? 2 : 1); k++) {
+                CHECK(fd23_get_field(ctx, 0, sfields[k], snap.data()));
+                char name[32];
+                snprintf(name, sizeof name, "%s%05d", snames[k], e);
+                write_file(name, snap.data(), N * csize);
+            }
+        }
+        if (e % itprint == 0) {
+            CHECK(fd23_sync(ctx));
+            const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            std::cout << std::setw(10) << e << " TOTAL " << std::setw(10) << el << " sec" << std::setw(10)
+                      << el / (e - nt0 + 1) * 1000.0 << " msec per step\n";
+            std::cout.flush();
+        }
+        it = e + 1;
+    }
+    CHECK(fd23_sync(ctx));
+
+    if (savestate) {
+        std::vector<char> buf(sbytes);
+        CHECK(fd23_get_state(ctx, 0, buf.data(), sbytes));
+        char name[64];
+        snprintf(name, sizeof name, "state%06d", nt + nt0 - 1);
+        write_file(name, buf.data(), sbytes);
+    }
+
+    // ---- seismograms: raw time-major files, then ISIS ---------------------
+    const int nframes = end_frame - first_frame;
+    const char *comp[2] = {psv ? "_U" : "_V", "_W"};
+    const double shx[2] = {psv ? 0.5 : 0.0, 0.0}, shz[2] = {0.0, psv ? -0.5 : 0.0};
+    for (size_t c = 0; c < rx.size(); c++) {
+        const int n = (int)rx[c].size();
+        std::vector<char> all((size_t)end_frame * n * csize);
+        if (n) CHECK(fd23_get_seismograms(ctx, 0, (int)c, (float *)all.data(), end_frame));
+        const std::string raw = output + comp[c];
+        write_file(raw, all.data() + (size_t)first_frame * n * csize, (size_t)nframes * n * csize);
+        // toisis reads floor(nt/itrecord) frames of the raw file (isis.cpp:20)
+        const int ntrec = std::min(nt / itrecord, nframes);
+        std::vector<float> tr((size_t)ntrec * n);
+        for (size_t q = 0; q < tr.size(); q++)
+            tr[q] = f64 ? (float)((const double *)all.data())[(size_t)first_frame * n + q]
+                        : ((const float *)all.data())[(size_t)first_frame * n + q];
+        CHECK(fd23_isis_write((raw + ".isis").c_str(), tr.data(), n, ntrec, rx[c].data(), rz[c].data(), shx[c],
+                              shz[c], h, dt, itrecord, xs, zs));
+    }
+    fd23_ctx_destroy(ctx);
+    std::cout << "finish\n";
+    return 0;
+}

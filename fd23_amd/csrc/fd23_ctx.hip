@@ -87,7 +87,7 @@ struct fd23_ctx {
     int *d_compfield = nullptr;
     float *scl = nullptr;
     unsigned *amax = nullptr;
-    float *resc = nullptr;
+    int staged_steps = 0; // steps taken through the staged entry points
     int *clock = nullptr;
     bool scales_init = false;
     double bmax = 0;   // largest buoyancy-type material value (scale estimate)
@@ -190,7 +190,7 @@ static void free_ctx(fd23_ctx *c) {
     for (auto p : c->field_alloc) if (p) hipFree(p);
     for (auto p : c->psi) if (p) hipFree(p);
     for (auto p : c->mat) if (p) hipFree(p);
-    void *others[] = {c->prof, c->d_fieldtab, c->d_comptab, c->d_compfield, c->scl, c->amax, c->resc, c->clock,
+    void *others[] = {c->prof, c->d_fieldtab, c->d_comptab, c->d_compfield, c->scl, c->amax, c->clock,
                       c->d_taps, c->d_src, c->d_stf, c->d_roff, c->d_nrec, c->seis, c->d_tap_start,
                       c->d_rec_start, c->d_tapsT, c->d_recs};
     for (auto p : others) if (p) hipFree(p);
@@ -273,9 +273,8 @@ extern "C" int fd23_ctx_create(const fd23_desc *desc, fd23_ctx **out) {
     ALLOC(c->d_fieldtab, sizeof(void *) * 5);
     ALLOC(c->d_comptab, sizeof(void *) * 2);
     ALLOC(c->d_compfield, sizeof(int) * 2);
-    ALLOC(c->scl, sizeof(float) * 16);
-    ALLOC(c->amax, sizeof(unsigned) * 16);
-    ALLOC(c->resc, sizeof(float) * 16);
+    ALLOC(c->scl, sizeof(float) * 2 * kNextScale);
+    ALLOC(c->amax, sizeof(unsigned) * 16 * kAmaxSpread);
     ALLOC(c->clock, sizeof(int) * 2);
     ALLOC(c->d_src, sizeof(SrcInfo) * S);
     ALLOC(c->d_nrec, sizeof(int) * S * c->ncomp);
@@ -289,9 +288,8 @@ extern "C" int fd23_ctx_create(const fd23_desc *desc, fd23_ctx **out) {
     hipMemcpy(c->d_comptab, comps, sizeof(void *) * 2, hipMemcpyHostToDevice);
     int cf[2] = {0, 1};
     hipMemcpy(c->d_compfield, cf, sizeof cf, hipMemcpyHostToDevice);
-    std::vector<float> ones(16, 1.0f);
-    hipMemcpy(c->scl, ones.data(), sizeof(float) * 16, hipMemcpyHostToDevice);
-    hipMemcpy(c->resc, ones.data(), sizeof(float) * 16, hipMemcpyHostToDevice);
+    std::vector<float> ones(2 * kNextScale, 1.0f);
+    hipMemcpy(c->scl, ones.data(), sizeof(float) * ones.size(), hipMemcpyHostToDevice);
     hipMemcpy(c->d_roff, c->h_roff.data(), sizeof(long) * c->h_roff.size(), hipMemcpyHostToDevice);
     c->tile = (d.nshots >= 4) ? 4 : 0;
     hipError_t e = hipDeviceSynchronize();
@@ -352,7 +350,11 @@ extern "C" int fd23_set_cpml(fd23_ctx *c, const double *packed) {
 // per-field power-of-two scales for FP16/BF16 storage
 extern "C" int fd23_set_scales(fd23_ctx *c, const float *scales, int n) {
     if (!c || !scales || n < 1 || n > 16) return fail(FD23_EINVAL, "bad scales");
+    for (int i = 0; i < n; i++)
+        if (!(scales[i] > 0) || !std::isfinite(1.0f / scales[i])) return fail(FD23_EINVAL, "scale %d is not usable", i);
+    // current and next scales alike
     HIPCHK(hipMemcpyAsync(c->scl, scales, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->scl + kNextScale, scales, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->scales_init = true;
     return FD23_OK;
@@ -361,7 +363,15 @@ extern "C" int fd23_get_scales(fd23_ctx *c, float *scales, unsigned *amax_bits, 
     if (!c || n < 1 || n > 16) return fail(FD23_EINVAL, "bad scales");
     HIPCHK(hipStreamSynchronize(c->stream));
     if (scales) HIPCHK(hipMemcpy(scales, c->scl, sizeof(float) * n, hipMemcpyDeviceToHost));
-    if (amax_bits) HIPCHK(hipMemcpy(amax_bits, c->amax, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
+    if (amax_bits) {
+        std::vector<unsigned> all((size_t)16 * kAmaxSpread);
+        HIPCHK(hipMemcpy(all.data(), c->amax, sizeof(unsigned) * all.size(), hipMemcpyDeviceToHost));
+        for (int f = 0; f < n; f++) {
+            unsigned m = 0;
+            for (int i = 0; i < kAmaxSpread; i++) m = std::max(m, all[(size_t)f * kAmaxSpread + i]);
+            amax_bits[f] = m;
+        }
+    }
     return FD23_OK;
 }
 
@@ -800,30 +810,35 @@ static int init_scales(fd23_ctx *c) {
         sc[SC_PV_X] = sc[SC_PV_Z] = pow2_scale(std::max(v_est, t_est));
     }
     HIPCHK(hipMemcpyAsync(c->scl, sc, sizeof sc, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->scl + kNextScale, sc, sizeof sc, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->scales_init = true;
     return FD23_OK;
 }
 
+// FP16/BF16 storage: every kScaleEvery steps the control kernel picks the
+// next scale of each array from its running max; each pass reads arrays in
+// the current scales and writes what it updates in the next ones, and a
+// one-thread commit after the pass makes those the current scales.  The
+// arrays are never rescaled in a pass of their own.
 static void enqueue_scale_control(fd23_ctx *c) {
-    hipLaunchKernelGGL(k_scale_control, dim3(1), dim3(64), 0, c->stream, c->scl, c->amax, c->resc, c->nscale,
-                       kScaleLo, kScaleHi, kScaleTarget);
-    RescaleTable tab{};
-    const size_t S = c->d.nshots;
-    for (int f = 0; f < c->nfield; f++) {
-        tab.ptr[f] = c->field_alloc[f];
-        tab.n[f] = (long)(S * c->sstride + 2 * kGuard);
-    }
-    const int half = c->npsi / 2;
-    for (int q = 0; q < c->npsi; q++) {
-        tab.ptr[c->nfield + q] = c->psi[q];
-        tab.n[c->nfield + q] = (long)(S * (q < half ? c->xpsi : c->zpsi));
-    }
-    const dim3 grid(1024, c->nscale);
-    if (c->d.storage == FD23_STORE_F16)
-        hipLaunchKernelGGL((k_rescale<__half>), grid, dim3(256), 0, c->stream, tab, (const float *)c->resc);
-    else
-        hipLaunchKernelGGL((k_rescale<__hip_bfloat16>), grid, dim3(256), 0, c->stream, tab, (const float *)c->resc);
+    hipLaunchKernelGGL(k_scale_control, dim3(1), dim3(kAmaxSpread), 0, c->stream, c->scl, c->amax, c->nscale, kScaleLo,
+                       kScaleHi, kScaleTarget);
+}
+
+// scale slots a pass rewrites
+static unsigned pass_mask(const fd23_ctx *c, bool stress) {
+    if (is_psv(c))
+        return stress ? (1u << SC_TXX) | (1u << SC_TZZ) | (1u << SC_TXZ) | (1u << SC_PU_X) | (1u << SC_PW_X) |
+                            (1u << SC_PU_Z) | (1u << SC_PW_Z)
+                      : (1u << SC_U) | (1u << SC_W) | (1u << SC_PTXX_X) | (1u << SC_PTXZ_X) | (1u << SC_PTZZ_Z) |
+                            (1u << SC_PTXZ_Z);
+    return stress ? (1u << SC_TX) | (1u << SC_TZ) | (1u << SC_PV_X) | (1u << SC_PV_Z)
+                  : (1u << SC_V) | (1u << SC_PTX_X) | (1u << SC_PTZ_Z);
+}
+
+static void enqueue_commit(fd23_ctx *c, bool stress) {
+    hipLaunchKernelGGL(k_commit_scales, dim3(1), dim3(64), 0, c->stream, c->scl, pass_mask(c, stress));
 }
 
 extern "C" int fd23_step_stress(fd23_ctx *c) {
@@ -831,6 +846,7 @@ extern "C" int fd23_step_stress(fd23_ctx *c) {
     if (r) return r;
     if (c->box && (r = fd23_box_launch_pre_stress(c, -1))) return r;
     dispatch_pass(c, true, kNoEvents);
+    if (scaled_storage(c)) enqueue_commit(c, true);
     if (c->box && (r = fd23_box_launch_post_stress(c, -1))) return r;
     return check_launch();
 }
@@ -844,6 +860,10 @@ extern "C" int fd23_step_velocity(fd23_ctx *c) {
     int r = ready(c);
     if (r) return r;
     dispatch_pass(c, false, kNoEvents);
+    if (scaled_storage(c)) {
+        enqueue_commit(c, false);
+        if (++c->staged_steps % kScaleEvery == 0) enqueue_scale_control(c);
+    }
     if (c->box && (r = fd23_box_launch_post_velocity(c, -1))) return r;
     return check_launch();
 }
@@ -858,10 +878,15 @@ extern "C" int fd23_record(fd23_ctx *c, int it, int itrecord) {
 static int enqueue_step(fd23_ctx *c, int k, int itrecord) {
     int r;
     const Events ev = make_events(c, k, itrecord);
+    // a scale check ends every kScaleEvery-th step of a chunk, so the passes
+    // of steps k = 0, kScaleEvery, ... commit the scales they wrote
+    const bool commit = scaled_storage(c) && k % kScaleEvery == 0;
     if (c->box && (r = fd23_box_launch_pre_stress(c, k))) return r;
     dispatch_pass(c, true, ev);
+    if (commit) enqueue_commit(c, true);
     if (c->box && (r = fd23_box_launch_post_stress(c, k))) return r;
     dispatch_pass(c, false, ev);
+    if (commit) enqueue_commit(c, false);
     if (c->box && (r = fd23_box_launch_post_velocity(c, k))) return r;
     if (scaled_storage(c) && (k + 1) % kScaleEvery == 0) enqueue_scale_control(c);
     return FD23_OK;
@@ -1090,32 +1115,35 @@ extern "C" int fd23_time_passes(fd23_ctx *c, int nsteps, double *stress_ms, doub
     int r = ready(c);
     if (r) return r;
     if (nsteps < 1 || nsteps > 4096) return fail(FD23_EINVAL, "nsteps must be in [1, 4096]");
-    // all passes are queued back to back with an event between each pair,
-    // and the host waits once at the end: the stream never drains between
-    // kernels, so each interval is one kernel plus its dispatch, as in the
-    // graph-replayed loop
-    std::vector<hipEvent_t> ev(2 * nsteps + 1);
-    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
-    hipEventRecord(ev[0], c->stream);
-    for (int i = 0; i < nsteps; i++) {
-        dispatch_pass(c, true, kNoEvents);
-        hipEventRecord(ev[2 * i + 1], c->stream);
-        dispatch_pass(c, false, kNoEvents);
-        hipEventRecord(ev[2 * i + 2], c->stream);
+    // nsteps stress passes back to back, then nsteps velocity passes: the
+    // queue never drains and no event sits between two passes, so each
+    // interval over nsteps is one kernel's duration plus its dispatch
+    hipEvent_t e0, e1, e2;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventCreate(&e2));
+    const bool sc = scaled_storage(c);
+    dispatch_pass(c, true, kNoEvents); // first pass outside the timing, then
+    if (sc) enqueue_commit(c, true);   // the scales of its arrays are current
+    dispatch_pass(c, false, kNoEvents);
+    if (sc) enqueue_commit(c, false);
+    hipEventRecord(e0, c->stream);
+    for (int i = 0; i < nsteps; i++) dispatch_pass(c, true, kNoEvents);
+    hipEventRecord(e1, c->stream);
+    for (int i = 0; i < nsteps; i++) dispatch_pass(c, false, kNoEvents);
+    hipEventRecord(e2, c->stream);
+    const hipError_t se = hipEventSynchronize(e2);
+    float a = 0, b = 0;
+    if (se == hipSuccess) {
+        hipEventElapsedTime(&a, e0, e1);
+        hipEventElapsedTime(&b, e1, e2);
     }
-    const hipError_t se = hipEventSynchronize(ev.back());
-    double ts = 0, tv = 0;
-    for (int i = 0; i < nsteps && se == hipSuccess; i++) {
-        float a = 0, b = 0;
-        hipEventElapsedTime(&a, ev[2 * i], ev[2 * i + 1]);
-        hipEventElapsedTime(&b, ev[2 * i + 1], ev[2 * i + 2]);
-        ts += a;
-        tv += b;
-    }
-    for (auto &e : ev) hipEventDestroy(e);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipEventDestroy(e2);
     if (se != hipSuccess) return fail(FD23_EHIP, "%s", hipGetErrorString(se));
-    if (stress_ms) *stress_ms = ts / nsteps;
-    if (velocity_ms) *velocity_ms = tv / nsteps;
+    if (stress_ms) *stress_ms = a / nsteps;
+    if (velocity_ms) *velocity_ms = b / nsteps;
     return check_launch();
 }
 
@@ -1148,6 +1176,11 @@ extern "C" int fd23_autotune(fd23_ctx *c, int steps, double *best_ms) {
         for (auto p : saved) hipFree(p);
         return r;
     }
+    // scales and running maxima too (FP16/BF16 storage)
+    float scl_saved[2 * kNextScale];
+    std::vector<unsigned> amax_saved((size_t)16 * kAmaxSpread);
+    HIPCHK(hipMemcpy(scl_saved, c->scl, sizeof scl_saved, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(amax_saved.data(), c->amax, sizeof(unsigned) * amax_saved.size(), hipMemcpyDeviceToHost));
     const int orig = c->tile;
     int best = orig;
     double bt = 1e30;
@@ -1168,6 +1201,8 @@ extern "C" int fd23_autotune(fd23_ctx *c, int steps, double *best_ms) {
         hipMemcpyAsync(c->field_alloc[f], saved[k], sizes[k], hipMemcpyDeviceToDevice, c->stream);
     for (int q = 0; q < c->npsi; q++, k++)
         hipMemcpyAsync(c->psi[q], saved[k], sizes[k], hipMemcpyDeviceToDevice, c->stream);
+    hipMemcpyAsync(c->scl, scl_saved, sizeof scl_saved, hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(c->amax, amax_saved.data(), sizeof(unsigned) * amax_saved.size(), hipMemcpyHostToDevice, c->stream);
     hipStreamSynchronize(c->stream);
     for (auto p : saved) hipFree(p);
     c->tile = orig;

@@ -108,11 +108,16 @@ __device__ __forceinline__ C dp(const C *f, int s, C c0, C c1, C c2, C c3) {
 // Running max of |v| per array (scaled storage): reduced across the 64
 // lanes of the wavefront, then one atomicMax on the float bits (non-negative
 // floats order like unsigned integers).
-__device__ __forceinline__ void amax_update(unsigned *slot, float v) {
+// Each array's maximum is spread over kAmaxSpread words picked by the
+// workgroup index, so that the atomics of a pass do not all queue on one
+// address; the scale check reduces the words.
+constexpr int kAmaxSpread = 256;
+__device__ __forceinline__ void amax_update(unsigned *amax, int slot, float v) {
     v = fabsf(v);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    if ((threadIdx.x & 63) == 0 && v > 0.f) atomicMax(slot, __float_as_uint(v));
+    if ((threadIdx.x & 63) == 0 && v > 0.f)
+        atomicMax(&amax[slot * kAmaxSpread + (blockIdx.x & (kAmaxSpread - 1))], __float_as_uint(v));
 }
 
 // ---------------------------------------------------------------------------

@@ -204,6 +204,9 @@ template <class C> struct PsvStressProf {
 // Tzz_z Txz_z U_z W_z = 5..12
 enum { SC_U = 0, SC_W, SC_TXX, SC_TZZ, SC_TXZ, SC_PTXX_X, SC_PTXZ_X, SC_PU_X, SC_PW_X,
        SC_PTZZ_Z, SC_PTXZ_Z, SC_PU_Z, SC_PW_Z, SC_PSV_N };
+// scale buffer: current scales [0, kNextScale), next scales after them
+constexpr int kNextScale = 16;
+constexpr int kScaleExpMax = 100;
 
 template <class S, int RZ, int SPB>
 __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStressMat<typename Store<S>::C> m,
@@ -228,10 +231,16 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
     const bool interior = tile_interior(g, x0, z0, TZ);
     const int it = ev.clock ? ev.clock[0] + ev.k : 0;
 
+    // stored arrays are read in the current scales scl[0..15] and the
+    // updated ones written in the next scales scl[16..31] (equal except
+    // right after a scale change)
     float sc_u = 1.f, sc_w = 1.f, sc_xx = 1.f, sc_zz = 1.f, sc_xz = 1.f;
+    float wi_xx = 1.f, wi_zz = 1.f, wi_xz = 1.f;
     if (ST::scaled) {
         sc_u = scl[SC_U]; sc_w = scl[SC_W];
         sc_xx = scl[SC_TXX]; sc_zz = scl[SC_TZZ]; sc_xz = scl[SC_TXZ];
+        wi_xx = 1.f / scl[kNextScale + SC_TXX]; wi_zz = 1.f / scl[kNextScale + SC_TZZ];
+        wi_xz = 1.f / scl[kNextScale + SC_TXZ];
     }
     float mx_xx = 0.f, mx_zz = 0.f, mx_xz = 0.f;
     float mp[4] = {0.f, 0.f, 0.f, 0.f}; // psi U_x W_x W_z U_z
@@ -286,8 +295,11 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
             }
         } else {
             float sp_ux = 1.f, sp_wx = 1.f, sp_uz = 1.f, sp_wz = 1.f;
+            float wp_ux = 1.f, wp_wx = 1.f, wp_uz = 1.f, wp_wz = 1.f;
             if (ST::scaled) {
                 sp_ux = scl[SC_PU_X]; sp_wx = scl[SC_PW_X]; sp_uz = scl[SC_PU_Z]; sp_wz = scl[SC_PW_Z];
+                wp_ux = 1.f / scl[kNextScale + SC_PU_X]; wp_wx = 1.f / scl[kNextScale + SC_PW_X];
+                wp_uz = 1.f / scl[kNextScale + SC_PU_Z]; wp_wz = 1.f / scl[kNextScale + SC_PW_Z];
             }
 #pragma unroll
             for (int r = 0; r < RZ; r++) {
@@ -308,12 +320,12 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
                 if (ix < npml || ix >= nx - npml) {
                     const long pi = (long)shot * g.xpsi_stride + xpsi_idx(ix, iz, nx, npml);
                     C p1 = mad(pr.Ux.b[ix], ST::ld(ps.Ux, pi, sp_ux), pr.Ux.c[ix] * ux);
-                    ST::st(ps.Ux, pi, p1, 1.f / sp_ux);
+                    ST::st(ps.Ux, pi, p1, wp_ux);
                     const C e1 = mad(ux, pr.Ux.k[ix], p1);
                     a = mad(gam, e1, a);
                     b = mad(lm, e1, b);
                     C p2 = mad(pr.Wx.b[ix], ST::ld(ps.Wx, pi, sp_wx), pr.Wx.c[ix] * wx);
-                    ST::st(ps.Wx, pi, p2, 1.f / sp_wx);
+                    ST::st(ps.Wx, pi, p2, wp_wx);
                     cc = mad(ma, mad(wx, pr.Wx.k[ix], p2), cc);
                     if (ST::scaled) {
                         mp[0] = fmaxf(mp[0], fabsf((float)p1));
@@ -326,12 +338,12 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
                 if (topcell || iz >= nz - npml) {
                     const long pi = (long)shot * g.zpsi_stride + zpsi_idx(ix, iz, nx, nz, npml, topcell);
                     C p1 = mad(pr.Wz.b[iz], ST::ld(ps.Wz, pi, sp_wz), pr.Wz.c[iz] * wz);
-                    ST::st(ps.Wz, pi, p1, 1.f / sp_wz);
+                    ST::st(ps.Wz, pi, p1, wp_wz);
                     const C e1 = mad(wz, pr.Wz.k[iz], p1);
                     a = mad(lm, e1, a);
                     b = mad(gam, e1, b);
                     C p2 = mad(pr.Uz.b[iz], ST::ld(ps.Uz, pi, sp_uz), pr.Uz.c[iz] * uz);
-                    ST::st(ps.Uz, pi, p2, 1.f / sp_uz);
+                    ST::st(ps.Uz, pi, p2, wp_uz);
                     cc = mad(ma, mad(uz, pr.Uz.k[iz], p2), cc);
                     if (ST::scaled) {
                         mp[2] = fmaxf(mp[2], fabsf((float)p1));
@@ -373,9 +385,9 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
             const int gz = z0 + lz0 + r;
             if (gx >= nx || gz >= nz) continue;
             const long q = base + (long)gz * px + gx;
-            ST::st(f.Txx, q, txx[r], 1.f / sc_xx);
-            ST::st(f.Tzz, q, tzz[r], 1.f / sc_zz);
-            ST::st(f.Txz, q, txz[r], 1.f / sc_xz);
+            ST::st(f.Txx, q, txx[r], wi_xx);
+            ST::st(f.Tzz, q, tzz[r], wi_zz);
+            ST::st(f.Txz, q, txz[r], wi_xz);
             if (ST::scaled) {
                 mx_xx = fmaxf(mx_xx, fabsf((float)txx[r]));
                 mx_zz = fmaxf(mx_zz, fabsf((float)tzz[r]));
@@ -384,13 +396,13 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
         }
     }
     if (ST::scaled) {
-        amax_update(&amax[SC_TXX], mx_xx);
-        amax_update(&amax[SC_TZZ], mx_zz);
-        amax_update(&amax[SC_TXZ], mx_xz);
-        amax_update(&amax[SC_PU_X], mp[0]);
-        amax_update(&amax[SC_PW_X], mp[1]);
-        amax_update(&amax[SC_PW_Z], mp[2]);
-        amax_update(&amax[SC_PU_Z], mp[3]);
+        amax_update(amax, SC_TXX, mx_xx);
+        amax_update(amax, SC_TZZ, mx_zz);
+        amax_update(amax, SC_TXZ, mx_xz);
+        amax_update(amax, SC_PU_X, mp[0]);
+        amax_update(amax, SC_PW_X, mp[1]);
+        amax_update(amax, SC_PW_Z, mp[2]);
+        amax_update(amax, SC_PU_Z, mp[3]);
     }
 }
 
@@ -456,9 +468,11 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
     const int it = ev.clock ? ev.clock[0] + ev.k : 0;
 
     float sc_u = 1.f, sc_w = 1.f, sc_xx = 1.f, sc_zz = 1.f, sc_xz = 1.f;
+    float wi_u = 1.f, wi_w = 1.f;
     if (ST::scaled) {
         sc_u = scl[SC_U]; sc_w = scl[SC_W];
         sc_xx = scl[SC_TXX]; sc_zz = scl[SC_TZZ]; sc_xz = scl[SC_TXZ];
+        wi_u = 1.f / scl[kNextScale + SC_U]; wi_w = 1.f / scl[kNextScale + SC_W];
     }
     float mx_u = 0.f, mx_w = 0.f;
     float mp[4] = {0.f, 0.f, 0.f, 0.f}; // psi Txx_x Txz_x Txz_z Tzz_z
@@ -514,8 +528,11 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
             }
         } else {
             float sp_xx = 1.f, sp_xzx = 1.f, sp_xzz = 1.f, sp_zz = 1.f;
+            float wp_xx = 1.f, wp_xzx = 1.f, wp_xzz = 1.f, wp_zz = 1.f;
             if (ST::scaled) {
                 sp_xx = scl[SC_PTXX_X]; sp_xzx = scl[SC_PTXZ_X]; sp_xzz = scl[SC_PTXZ_Z]; sp_zz = scl[SC_PTZZ_Z];
+                wp_xx = 1.f / scl[kNextScale + SC_PTXX_X]; wp_xzx = 1.f / scl[kNextScale + SC_PTXZ_X];
+                wp_xzz = 1.f / scl[kNextScale + SC_PTXZ_Z]; wp_zz = 1.f / scl[kNextScale + SC_PTZZ_Z];
             }
 #pragma unroll
             for (int r = 0; r < RZ; r++) {
@@ -534,10 +551,10 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
                 if (ix < npml || ix >= nx - npml) {
                     const long pi = (long)shot * g.xpsi_stride + xpsi_idx(ix, iz, nx, npml);
                     C p1 = mad(pr.Txx_x.b[ix], ST::ld(ps.Txx_x, pi, sp_xx), pr.Txx_x.c[ix] * dxTxx);
-                    ST::st(ps.Txx_x, pi, p1, 1.f / sp_xx);
+                    ST::st(ps.Txx_x, pi, p1, wp_xx);
                     u = mad(b_u, mad(dxTxx, pr.Txx_x.k[ix], p1), u);
                     C p2 = mad(pr.Txz_x.b[ix], ST::ld(ps.Txz_x, pi, sp_xzx), pr.Txz_x.c[ix] * dxTxz);
-                    ST::st(ps.Txz_x, pi, p2, 1.f / sp_xzx);
+                    ST::st(ps.Txz_x, pi, p2, wp_xzx);
                     ww = mad(b_w, mad(dxTxz, pr.Txz_x.k[ix], p2), ww);
                     if (ST::scaled) {
                         mp[0] = fmaxf(mp[0], fabsf((float)p1));
@@ -552,8 +569,8 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
                     const long pi = (long)shot * g.zpsi_stride + zpsi_idx(ix, iz, nx, nz, npml, topcell);
                     C p1 = mad(pr.Txz_z.b[iz], ST::ld(ps.Txz_z, pi, sp_xzz), pr.Txz_z.c[iz] * dzTxz);
                     C p2 = mad(pr.Tzz_z.b[iz], ST::ld(ps.Tzz_z, pi, sp_zz), pr.Tzz_z.c[iz] * dzTzz);
-                    ST::st(ps.Txz_z, pi, p1, 1.f / sp_xzz);
-                    ST::st(ps.Tzz_z, pi, p2, 1.f / sp_zz);
+                    ST::st(ps.Txz_z, pi, p1, wp_xzz);
+                    ST::st(ps.Tzz_z, pi, p2, wp_zz);
                     u = mad(b_u, mad(dzTxz, pr.Txz_z.k[iz], p1), u);
                     ww = mad(b_w, mad(dzTzz, pr.Tzz_z.k[iz], p2), ww);
                     if (ST::scaled) {
@@ -575,8 +592,8 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
             const int gz = z0 + lz0 + r;
             if (gx >= nx || gz >= nz) continue;
             const long q = base + (long)gz * px + gx;
-            ST::st(f.U, q, nv[0][r], 1.f / sc_u);
-            ST::st(f.W, q, nv[1][r], 1.f / sc_w);
+            ST::st(f.U, q, nv[0][r], wi_u);
+            ST::st(f.W, q, nv[1][r], wi_w);
             if (ST::scaled) {
                 mx_u = fmaxf(mx_u, fabsf((float)nv[0][r]));
                 mx_w = fmaxf(mx_w, fabsf((float)nv[1][r]));
@@ -585,12 +602,12 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
         if (ev.clock) fused_record<C, RZ, 2>(ev, it, t.tile * g.nshots + shot, sXX, sZZ, nv, lz0);
     }
     if (ST::scaled) {
-        amax_update(&amax[SC_U], mx_u);
-        amax_update(&amax[SC_W], mx_w);
-        amax_update(&amax[SC_PTXX_X], mp[0]);
-        amax_update(&amax[SC_PTXZ_X], mp[1]);
-        amax_update(&amax[SC_PTXZ_Z], mp[2]);
-        amax_update(&amax[SC_PTZZ_Z], mp[3]);
+        amax_update(amax, SC_U, mx_u);
+        amax_update(amax, SC_W, mx_w);
+        amax_update(amax, SC_PTXX_X, mp[0]);
+        amax_update(amax, SC_PTXZ_X, mp[1]);
+        amax_update(amax, SC_PTXZ_Z, mp[2]);
+        amax_update(amax, SC_PTZZ_Z, mp[3]);
     }
 }
 
@@ -631,8 +648,11 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
     const bool interior = tile_interior(g, x0, z0, TZ);
     const int it = ev.clock ? ev.clock[0] + ev.k : 0;
 
-    float sc_v = 1.f, sc_x = 1.f, sc_z = 1.f;
-    if (ST::scaled) { sc_v = scl[SC_V]; sc_x = scl[SC_TX]; sc_z = scl[SC_TZ]; }
+    float sc_v = 1.f, sc_x = 1.f, sc_z = 1.f, wi_x = 1.f, wi_z = 1.f;
+    if (ST::scaled) {
+        sc_v = scl[SC_V]; sc_x = scl[SC_TX]; sc_z = scl[SC_TZ];
+        wi_x = 1.f / scl[kNextScale + SC_TX]; wi_z = 1.f / scl[kNextScale + SC_TZ];
+    }
     float mx_x = 0.f, mx_z = 0.f;
     float mp[2] = {0.f, 0.f}; // psi V_x V_z
 
@@ -675,8 +695,11 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
                 vz[r] = mad(muz[r], dzV, vz[r]);
             }
         } else {
-            float sp_x = 1.f, sp_z = 1.f;
-            if (ST::scaled) { sp_x = scl[SC_PV_X]; sp_z = scl[SC_PV_Z]; }
+            float sp_x = 1.f, sp_z = 1.f, wp_x = 1.f, wp_z = 1.f;
+            if (ST::scaled) {
+                sp_x = scl[SC_PV_X]; sp_z = scl[SC_PV_Z];
+                wp_x = 1.f / scl[kNextScale + SC_PV_X]; wp_z = 1.f / scl[kNextScale + SC_PV_Z];
+            }
 #pragma unroll
             for (int r = 0; r < RZ; r++) {
                 const int lz = lz0 + r;
@@ -692,7 +715,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
                 if (ix < npml || ix >= nx - npml) {
                     const long pi = (long)shot * g.xpsi_stride + xpsi_idx(ix, iz, nx, npml);
                     C p = mad(pr.x.b[ix], ST::ld(ps.x, pi, sp_x), pr.x.c[ix] * dxV);
-                    ST::st(ps.x, pi, p, 1.f / sp_x);
+                    ST::st(ps.x, pi, p, wp_x);
                     tx_ = mad(mux[r], mad(dxV, pr.x.k[ix], p), tx_);
                     if (ST::scaled) mp[0] = fmaxf(mp[0], fabsf((float)p));
                 } else {
@@ -702,7 +725,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
                 if (topcell || iz >= nz - npml) {
                     const long pi = (long)shot * g.zpsi_stride + zpsi_idx(ix, iz, nx, nz, npml, topcell);
                     C p = mad(pr.z.b[iz], ST::ld(ps.z, pi, sp_z), pr.z.c[iz] * dzV);
-                    ST::st(ps.z, pi, p, 1.f / sp_z);
+                    ST::st(ps.z, pi, p, wp_z);
                     tz_ = mad(muz[r], mad(dzV, pr.z.k[iz], p), tz_);
                     if (ST::scaled) mp[1] = fmaxf(mp[1], fabsf((float)p));
                 } else {
@@ -736,8 +759,8 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
             const int gz = z0 + lz0 + r;
             if (gx >= nx || gz >= nz) continue;
             const long q = base + (long)gz * px + gx;
-            ST::st(f.Tx, q, vx[r], 1.f / sc_x);
-            ST::st(f.Tz, q, vz[r], 1.f / sc_z);
+            ST::st(f.Tx, q, vx[r], wi_x);
+            ST::st(f.Tz, q, vz[r], wi_z);
             if (ST::scaled) {
                 mx_x = fmaxf(mx_x, fabsf((float)vx[r]));
                 mx_z = fmaxf(mx_z, fabsf((float)vz[r]));
@@ -745,10 +768,10 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
         }
     }
     if (ST::scaled) {
-        amax_update(&amax[SC_TX], mx_x);
-        amax_update(&amax[SC_TZ], mx_z);
-        amax_update(&amax[SC_PV_X], mp[0]);
-        amax_update(&amax[SC_PV_Z], mp[1]);
+        amax_update(amax, SC_TX, mx_x);
+        amax_update(amax, SC_TZ, mx_z);
+        amax_update(amax, SC_PV_X, mp[0]);
+        amax_update(amax, SC_PV_Z, mp[1]);
     }
 }
 
@@ -778,8 +801,11 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
     const bool interior = tile_interior(g, x0, z0, TZ);
     const int it = ev.clock ? ev.clock[0] + ev.k : 0;
 
-    float sc_v = 1.f, sc_x = 1.f, sc_z = 1.f;
-    if (ST::scaled) { sc_v = scl[SC_V]; sc_x = scl[SC_TX]; sc_z = scl[SC_TZ]; }
+    float sc_v = 1.f, sc_x = 1.f, sc_z = 1.f, wi_v = 1.f;
+    if (ST::scaled) {
+        sc_v = scl[SC_V]; sc_x = scl[SC_TX]; sc_z = scl[SC_TZ];
+        wi_v = 1.f / scl[kNextScale + SC_V];
+    }
     float mx_v = 0.f;
     float mp[2] = {0.f, 0.f}; // psi Tx_x Tz_z
 
@@ -850,8 +876,11 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
                 nv[0][r] = v;
             }
         } else {
-            float sp_x = 1.f, sp_z = 1.f;
-            if (ST::scaled) { sp_x = scl[SC_PTX_X]; sp_z = scl[SC_PTZ_Z]; }
+            float sp_x = 1.f, sp_z = 1.f, wp_x = 1.f, wp_z = 1.f;
+            if (ST::scaled) {
+                sp_x = scl[SC_PTX_X]; sp_z = scl[SC_PTZ_Z];
+                wp_x = 1.f / scl[kNextScale + SC_PTX_X]; wp_z = 1.f / scl[kNextScale + SC_PTZ_Z];
+            }
 #pragma unroll
             for (int r = 0; r < RZ; r++) {
                 const int lz = lz0 + r;
@@ -866,7 +895,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
                 if (ix < npml || ix >= nx - npml) {
                     const long pi = (long)shot * g.xpsi_stride + xpsi_idx(ix, iz, nx, npml);
                     C p = mad(pr.x.b[ix], ST::ld(ps.x, pi, sp_x), pr.x.c[ix] * Tx_x);
-                    ST::st(ps.x, pi, p, 1.f / sp_x);
+                    ST::st(ps.x, pi, p, wp_x);
                     v = mad(bv[r], mad(Tx_x, pr.x.k[ix], p), v);
                     if (ST::scaled) mp[0] = fmaxf(mp[0], fabsf((float)p));
                 } else {
@@ -876,7 +905,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
                 if (topcell || iz >= nz - npml) {
                     const long pi = (long)shot * g.zpsi_stride + zpsi_idx(ix, iz, nx, nz, npml, topcell);
                     C p = mad(pr.z.b[iz], ST::ld(ps.z, pi, sp_z), pr.z.c[iz] * Tz_z);
-                    ST::st(ps.z, pi, p, 1.f / sp_z);
+                    ST::st(ps.z, pi, p, wp_z);
                     v = mad(bv[r], mad(Tz_z, pr.z.k[iz], p), v);
                     if (ST::scaled) mp[1] = fmaxf(mp[1], fabsf((float)p));
                 } else {
@@ -889,15 +918,15 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
         for (int r = 0; r < RZ; r++) {
             const int gz = z0 + lz0 + r;
             if (gx >= nx || gz >= nz) continue;
-            ST::st(f.V, base + (long)gz * px + gx, nv[0][r], 1.f / sc_v);
+            ST::st(f.V, base + (long)gz * px + gx, nv[0][r], wi_v);
             if (ST::scaled) mx_v = fmaxf(mx_v, fabsf((float)nv[0][r]));
         }
         if (ev.clock) fused_record<C, RZ, 1>(ev, it, t.tile * g.nshots + shot, sX, sX, nv, lz0);
     }
     if (ST::scaled) {
-        amax_update(&amax[SC_V], mx_v);
-        amax_update(&amax[SC_PTX_X], mp[0]);
-        amax_update(&amax[SC_PTZ_Z], mp[1]);
+        amax_update(amax, SC_V, mx_v);
+        amax_update(amax, SC_PTX_X, mp[0]);
+        amax_update(amax, SC_PTZ_Z, mp[1]);
     }
 }
 
@@ -973,39 +1002,40 @@ __global__ void k_record(S *const *comps, int ncomp, long sstride, const long *_
 // rescale kernel multiplies the stored data by the (exact) power-of-two
 // ratio of old to new scale.
 // ===========================================================================
-__global__ void k_scale_control(float *scl, unsigned *amax, float *resc, int n, int lo, int hi, int target) {
-    const int f = threadIdx.x;
-    if (f >= n) return;
-    const float m = __uint_as_float(amax[f]);
-    amax[f] = 0u;
-    float r = 1.f;
-    if (m > 0.f && isfinite(m)) {
-        const float cur = scl[f];
-        const float stored = m / cur;
-        if (stored > ldexpf(1.f, hi) || stored < ldexpf(1.f, lo)) {
-            int e;
-            frexpf(m, &e); // m in [2^(e-1), 2^e)
-            const float nw = ldexpf(1.f, e - target);
-            r = cur / nw;
-            scl[f] = nw;
+__global__ void k_scale_control(float *scl, unsigned *amax, int n, int lo, int hi, int target) {
+    __shared__ unsigned red[kAmaxSpread];
+    const int t = threadIdx.x; // blockDim.x == kAmaxSpread
+    for (int f = 0; f < n; f++) {
+        red[t] = amax[f * kAmaxSpread + t];
+        amax[f * kAmaxSpread + t] = 0u;
+        __syncthreads();
+        for (int o = kAmaxSpread / 2; o > 0; o >>= 1) {
+            if (t < o) red[t] = max(red[t], red[t + o]);
+            __syncthreads();
         }
+        if (t == 0) {
+            const float m = __uint_as_float(red[0]);
+            const float cur = scl[f];
+            float nw = cur;
+            if (m > 0.f && isfinite(m)) {
+                const float stored = m / cur;
+                if (stored > ldexpf(1.f, hi) || stored < ldexpf(1.f, lo)) {
+                    int e;
+                    frexpf(m, &e); // m in [2^(e-1), 2^e)
+                    // kept within [2^-100, 2^100] so that 1/scale stays a normal float
+                    nw = ldexpf(1.f, max(-kScaleExpMax, min(kScaleExpMax, e - target)));
+                }
+            }
+            scl[kNextScale + f] = nw;
+        }
+        __syncthreads();
     }
-    resc[f] = r;
 }
 
-struct RescaleTable {
-    void *ptr[16];
-    long n[16];
-};
-template <class S>
-__global__ void k_rescale(RescaleTable tab, const float *__restrict__ resc) {
-    const int a = blockIdx.y;
-    const float r = resc[a];
-    if (r == 1.f) return;
-    S *p = (S *)tab.ptr[a];
-    const long n = tab.n[a];
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        Store<S>::st(p, i, Store<S>::ld(p, i, 1.f) * r, 1.f);
+// the arrays a pass has rewritten now hold values in the next scales
+__global__ void k_commit_scales(float *scl, unsigned mask) {
+    const int f = threadIdx.x;
+    if (f < kNextScale && ((mask >> f) & 1u)) scl[f] = scl[kNextScale + f];
 }
 
 // advance the step clock after a captured chunk of steps

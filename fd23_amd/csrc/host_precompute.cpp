@@ -525,3 +525,42 @@ extern "C" int fd23_isis_write(const char *path, const float *raw, int nrec, int
     }
     return out ? FD23_OK : herr(FD23_EIO, std::string("write failed: ") + path);
 }
+
+// ---------------------------------------------------------------------------
+// receivers: every station gives the pair (x, z) and (x-1, z)
+// (RECORD::init_line and init_list, PSV/clean_nobox/record/cu_record.cu:12-76)
+// ---------------------------------------------------------------------------
+extern "C" int fd23_line_receivers(int nsta, int ix0, int iz0, int idx, int idz, int *rx, int *rz) {
+    if (nsta < 0 || (nsta > 0 && (!rx || !rz))) return herr(FD23_EINVAL, "line_receivers: bad arguments");
+    for (int i = 0; i < nsta; i++) {
+        rx[2 * i] = ix0 + idx * i;
+        rz[2 * i] = iz0 + idz * i;
+        rx[2 * i + 1] = rx[2 * i] - 1;
+        rz[2 * i + 1] = rz[2 * i];
+    }
+    return FD23_OK;
+}
+
+// station file of "z x" integer pairs, at most 100000 stations
+extern "C" int fd23_station_read(const char *path, int *rx, int *rz, int cap, int *nrec) {
+    if (!path || !nrec) return herr(FD23_EINVAL, "station_read: bad arguments");
+    std::ifstream f(path);
+    if (!f.is_open()) return herr(FD23_EIO, std::string(path) + " does not exist");
+    std::vector<int> v;
+    int a;
+    while (f >> a) v.push_back(a);
+    if (!f.eof()) return herr(FD23_EIO, std::string("non-integer entry in ") + path);
+    if (v.empty() || v.size() % 2) return herr(FD23_EIO, std::string("station file needs 'z x' pairs: ") + path);
+    const int nsta = (int)(v.size() / 2);
+    if (nsta > 100000) return herr(FD23_EINVAL, "too many stations (limit 100000)");
+    *nrec = 2 * nsta;
+    if (!rx || !rz) return FD23_OK; // size query
+    if (cap < 2 * nsta) return herr(FD23_EINVAL, "station_read: capacity too small");
+    for (int i = 0; i < nsta; i++) {
+        rz[2 * i] = v[2 * i];
+        rx[2 * i] = v[2 * i + 1];
+        rx[2 * i + 1] = rx[2 * i] - 1;
+        rz[2 * i + 1] = rz[2 * i];
+    }
+    return FD23_OK;
+}

@@ -102,6 +102,23 @@ def isis_write(path, raw, rx, rz, shiftx, shiftz, h, dt, itrecord, xs, zs):
                                       h, dt, itrecord, xs, zs))
 
 
+def line_receivers(nsta, ix0, iz0, idx, idz):
+    rx = np.zeros(2 * nsta, np.int32)
+    rz = np.zeros(2 * nsta, np.int32)
+    check(_lib.load().fd23_line_receivers(nsta, ix0, iz0, idx, idz, _ptr(rx), _ptr(rz)))
+    return rx, rz
+
+
+def station_read(path):
+    lib = _lib.load()
+    n = ctypes.c_int()
+    check(lib.fd23_station_read(path.encode(), None, None, 0, ctypes.byref(n)))
+    rx = np.zeros(n.value, np.int32)
+    rz = np.zeros(n.value, np.int32)
+    check(lib.fd23_station_read(path.encode(), _ptr(rx), _ptr(rz), n.value, ctypes.byref(n)))
+    return rx, rz
+
+
 class Par:
     """getpar-compatible parameters: ``Par(["prog", "par=file", "nx=100"])``."""
 

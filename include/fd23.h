@@ -172,10 +172,11 @@ int fd23_set_scales(fd23_ctx *ctx, const float *scales, int n);
 int fd23_get_scales(fd23_ctx *ctx, float *scales, unsigned *amax_bits, int n);
 /* algorithmic HBM bytes of one stress+velocity step for this context   */
 double fd23_bytes_per_step(const fd23_ctx *ctx);
-/* Time nsteps (1..4096) stress+velocity steps with HIP events on the
- * context stream, queued back to back; fills per-pass average
- * milliseconds.  The passes advance the fields (no injection, no
- * recording).                                                          */
+/* Time the passes with HIP events on the context stream: one untimed
+ * stress+velocity step, then nsteps (1..4096) stress passes back to back,
+ * then nsteps velocity passes; fills the average milliseconds of each
+ * pass.  The passes change the fields (no injection, no recording): use
+ * it on a context whose state is no longer needed, or save the state.   */
 int fd23_time_passes(fd23_ctx *ctx, int nsteps, double *stress_ms,
                      double *velocity_ms);
 
@@ -222,6 +223,14 @@ int fd23_isis_write(const char *path, const float *raw, int nrec,
                     int ntrecord, const int *rx, const int *rz, double shiftx,
                     double shiftz, double h, double dt, int itrecord, int xs,
                     int zs);
+
+/* receiver pairs (x, z), (x-1, z) per station: a line of nsta stations
+ * (RECORD::init_line, PSV/clean_nobox/record/cu_record.cu:12-28) or a
+ * station file of "z x" pairs (RECORD::init_list, :30-76); rx/rz hold
+ * 2*nsta entries; fd23_station_read with rx == NULL only counts         */
+int fd23_line_receivers(int nsta, int ix0, int iz0, int idx, int idz, int *rx,
+                        int *rz);
+int fd23_station_read(const char *path, int *rx, int *rz, int cap, int *nrec);
 
 /* ---- par files (Caltech getpar semantics, getpar/getpar.c) ----------- */
 typedef struct fd23_par fd23_par;

@@ -1,0 +1,59 @@
+"""The par-file driver (fd23_amd/bin, the reference's nbpsv2d / sh2d
+interface) on the CPU: it reads the reference's own par files, model and
+station files, and stops with an error before any device work when a key
+is missing or a feature is unsupported.  With no GPU the run ends at
+context creation, after every host step (par, model, material, C-PML,
+wavelet, moment tensor, receivers) has succeeded."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "fd23_amd", "bin")
+TT = os.path.join(ROOT, "tests", "golden", "ref", "psv_tt")
+
+
+def _run(prog, args, cwd):
+    exe = os.path.join(BIN, prog)
+    if not os.path.exists(exe):
+        pytest.skip("driver not built")
+    return subprocess.run([exe] + args, cwd=cwd, capture_output=True, text=True, timeout=60)
+
+
+@pytest.fixture()
+def ttdir(tmp_path):
+    shutil.copy(os.path.join(TT, "t1"), tmp_path / "t1")
+    shutil.copy(os.path.join(TT, "homo_6.8.mdl"), tmp_path / "homo.mdl")
+    shutil.copy(os.path.join(TT, "station200"), tmp_path / "station200")
+    return tmp_path
+
+
+def test_reference_par_file_reaches_the_device(ttdir):
+    r = _run("nbpsv2d_topabsorb", ["par=t1"], ttdir)
+    try:
+        import torch
+
+        gpu = torch.cuda.is_available()
+    except Exception:
+        gpu = False
+    if gpu:
+        pytest.skip("a GPU is present: the GPU test runs this case to the end")
+    assert r.returncode != 0
+    assert "fd23_ctx_create" in r.stderr, r.stderr
+
+
+def test_missing_key_is_reported(ttdir):
+    r = _run("sh2d", ["par=t1"], ttdir)
+    assert r.returncode != 0 and "must specify value for 'nrec'" in r.stderr
+
+
+def test_punch_is_rejected(ttdir):
+    r = _run("nbpsv2d", ["par=t1", "usepunch=1"], ttdir)
+    assert r.returncode != 0 and "usepunch" in r.stderr
+
+
+def test_missing_model_is_an_error(ttdir):
+    r = _run("nbpsv2d", ["par=t1", "model=none.mdl"], ttdir)
+    assert r.returncode != 0 and "fd23_model_read" in r.stderr

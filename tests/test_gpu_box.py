@@ -4,8 +4,9 @@ The oracle follows PSV/psv_clean_double/main.cpp:153-200 (ring add/sub of
 box/box.cpp:198-210, residual passes of fd/cu_fd.cu:93-207) with the
 plane-wave drive; both sides get the same heterogeneous model, so the
 global grid carries a scattered field.  Tolerances: 1e-5 relative L2 for
-FP32, 1e-10 for FP64 (global fields measured against the total-field
-scale: inside the box they hold total minus incident, a small difference).
+FP32, 1e-10 for FP64, with field errors measured against the total-field
+scale (inside the box the global grid holds total minus incident, a small
+difference of large numbers).
 """
 import numpy as np
 import pytest
@@ -80,13 +81,17 @@ def test_box_matches_oracle(f64, incident, angle, staged):
     errs = {}
     with _gpu(mat, packed, wave, f64, nt, rx, rz, staged) as ctx:
         res = ctx.box_residual()
+        # Errors are measured against the scale of the total field of the
+        # same kind (velocities U, W; stresses Txx, Tzz, Txz): inside the box
+        # the global grid holds total minus incident, a small difference,
+        # and a component the incident wave lacks (U and Txz at vertical
+        # incidence) is small in the residual grid too.
+        kinds = {"U": ("U", "W"), "W": ("U", "W"), "Txx": ("Txx", "Tzz", "Txz"),
+                 "Tzz": ("Txx", "Tzz", "Txz"), "Txz": ("Txx", "Tzz", "Txz")}
         for name in ("U", "W", "Txx", "Tzz", "Txz"):
-            errs["resid " + name] = rel(res[name], box.r[name])
-            # inside the box the global grid holds total minus incident, a
-            # small difference of the residual grid's total field: measure
-            # it against the total-field scale, not against itself
+            scale = max(np.linalg.norm(box.r[k]) for k in kinds[name])
+            errs["resid " + name] = float(np.linalg.norm(res[name].astype(np.float64) - box.r[name]) / scale)
             g, o = ctx.field(name), getattr(sim, name)
-            scale = max(np.linalg.norm(o), np.linalg.norm(box.r[name]))
             errs["field " + name] = float(np.linalg.norm(g.astype(np.float64) - o) / scale)
         errs["seis U"] = rel(ctx.seismograms(0, 0, nt), u)
         errs["seis W"] = rel(ctx.seismograms(0, 1, nt), w)

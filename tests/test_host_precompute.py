@@ -188,3 +188,25 @@ def test_isis_layout_matches_reference_file():
         assert np.isclose(f[3], (rx[i] + 0.5) * np.float32(0.3)) and iv[7] == 7
         assert iv[9] == 512 and iv[10] == rx[i] and iv[-2] == 123456789 and iv[-3] == 123456789
         assert np.array_equal(rec[84:].view(np.float32), raw[:, i])
+
+
+def test_station_file_and_line_receivers():
+    """The library's receiver lists equal the oracle's (whose station parsing
+    reproduces the reference's FP64 seismogram fixture), pairs at x, x-1."""
+    import oracle as O
+
+    path = os.path.join(os.path.dirname(D), "ref", "psv_fp64", "station200")
+    rx, rz = host.station_read(path)
+    ox, oz = O.station_receivers(path)
+    assert np.array_equal(rx, ox) and np.array_equal(rz, oz)
+    assert rx.size == 36 and (rz[0], rx[0], rx[1]) == (711, 529, 528)
+    lx, lz = host.line_receivers(5, 513, 4, 2, 1)
+    ox, oz = O.line_receivers(5, 513, 4, 2, 1)
+    assert np.array_equal(lx, ox) and np.array_equal(lz, oz)
+    with tempfile.TemporaryDirectory() as d:
+        bad = os.path.join(d, "odd")
+        open(bad, "w").write("1 2 3\n")
+        with pytest.raises(_lib.Fd23Error):
+            host.station_read(bad)
+    with pytest.raises(_lib.Fd23Error):
+        host.station_read("/nonexistent/stations")
