@@ -81,13 +81,17 @@ def run_gpu(wl, steps, warmup, autotune=True):
     return dict(elapsed=el, stress_ms=s_ms, vel_ms=v_ms, tile=tile, tune_ms=tune_ms, finite=finite)
 
 
-def cpu_baseline(wl, budget_s=12.0):
-    """Oracle passes (C, OpenMP) on this host over a bounded sample."""
+def cpu_baseline(wl, budget_s=12.0, budget_1t=4.0):
+    """Oracle passes (C, OpenMP) on this host over a bounded sample: all
+    cores, then one thread."""
+    import ctypes
+
     import oracle as O
     from fd23_amd import _lib
 
     lib = O.lib()
-    lib.or_num_threads.restype = __import__("ctypes").c_int
+    lib.or_num_threads.restype = ctypes.c_int
+    lib.or_set_threads.restype = None
     cores = int(lib.or_num_threads())
     names = ("BU", "BW", "MU", "MUA", "LAM")
     mat = dict(zip(names, wl.mat))
@@ -97,20 +101,41 @@ def cpu_baseline(wl, budget_s=12.0):
         off += 3 * n
     assert wl.wave == _lib.WAVE_PSV
     sim = O.PSV(wl.nx, wl.nz, wl.npml, mat, cp, wl.topabsorb)
-    sim.stress()
-    sim.velocity()
-    t0 = time.perf_counter()
-    k = 0
-    while True:
+    stf, M = wl.stf.astype(np.float32), wl.M.astype(np.float32)
+
+    def timed(budget, cap):
         sim.stress()
-        sim.inject(0, k, wl.stf.astype(np.float32), wl.M.astype(np.float32), *wl.sources[0])
         sim.velocity()
-        k += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or k >= 2000:
-            break
+        t0 = time.perf_counter()
+        k = 0
+        while True:
+            sim.stress()
+            sim.inject(0, k, stf, M, *wl.sources[0])
+            sim.velocity()
+            k += 1
+            el = time.perf_counter() - t0
+            if el > budget or k >= cap:
+                return k, el
+
+    k, el = timed(budget_s, 2000)
+    lib.or_set_threads(1)
+    k1, el1 = timed(budget_1t, 200)
+    lib.or_set_threads(cores)
     return dict(value=wl.nx * wl.nz * k / el / 1e9, unit="Gcell/s", cores=cores, kind="port",
-                sample="%d steps of the %dx%d P-SV FP32 loop (oracle C restatement, OpenMP)" % (k, wl.nx, wl.nz))
+                sample="%d steps of the %dx%d P-SV FP32 loop (oracle C restatement, OpenMP, %d threads)"
+                       % (k, wl.nx, wl.nz, cores),
+                value_1thread=wl.nx * wl.nz * k1 / el1 / 1e9, sample_1thread="%d steps, 1 thread" % k1,
+                cpu=_cpu_model())
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 STORAGE = {"f32": 0, "f16": 1, "bf16": 2, "f64": 3}

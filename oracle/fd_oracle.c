@@ -40,6 +40,8 @@ void or_ricker(int n, double dt, double f0, double t0, double *out)
 #ifdef _OPENMP
 #include <omp.h>
 int or_num_threads(void) { return omp_get_max_threads(); }
+void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 #else
 int or_num_threads(void) { return 1; }
+void or_set_threads(int n) { (void)n; }
 #endif

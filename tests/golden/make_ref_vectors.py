@@ -1,8 +1,8 @@
 """Write tests/golden/ref_host/*.npz from the reference's own host code.
 
-Runs oracle/_ref/ref_psv and oracle/_ref/ref_sh (the reference's material,
-C-PML, source-time-function, model and getpar sources compiled by
-oracle/ref/Makefile) on small synthetic inputs and stores inputs and
+Runs oracle/_ref/ref_psv, ref_sh and ref_sh_fd (the reference's material,
+C-PML, source-time-function, model and getpar sources and its SH CPU loop,
+compiled by oracle/ref/Makefile) on small synthetic inputs and stores inputs and
 outputs as fixtures, so the tests pin the host precompute without the
 reference tree.  Needs /root/reference; run from the repo root:
     make -C oracle/ref && python tests/golden/make_ref_vectors.py
@@ -140,6 +140,40 @@ def main():
         out["src"] = read(d, "src.f32", np.float32)
         np.savez_compressed(os.path.join(OUT, "sh_layered.npz"), **out)
         print("sh_layered ok")
+    sh_fd()
+
+
+def sh_fd():
+    """One stress pass and one velocity pass of the reference's SH CPU loop
+    (SH/fd/fd.cpp) from random fields, psi and material, with its C-PML
+    profiles (npml 8, bottom and side strips)."""
+    exe = os.path.join(REF, "ref_sh_fd")
+    nx, nz, npml = 48, 40, 8
+    rng = np.random.default_rng(77)
+    f32 = lambda *s: rng.standard_normal(s).astype(np.float32)
+    ins = {"V": f32(nz, nx), "Tx": f32(nz, nx), "Tz": f32(nz, nx),
+           "BV": rng.uniform(0.1, 0.3, (nz, nx)).astype(np.float32),
+           "MUX": rng.uniform(0.5, 2.0, (nz, nx)).astype(np.float32),
+           "MUZ": rng.uniform(0.5, 2.0, (nz, nx)).astype(np.float32),
+           "psi_V_x": f32(2 * npml * nz), "psi_V_z": f32(2 * npml * nx),
+           "psi_Tx_x": f32(2 * npml * nz), "psi_Tz_z": f32(2 * npml * nx)}
+    args = ["nx=%d" % nx, "nz=%d" % nz, "npml=%d" % npml, "pml_dt=0.01", "pml_r=1e-6", "pml_v=30", "pml_fc=2"]
+    out = {"in_" + k: v for k, v in ins.items()}
+    for mode, names in (("stress", ("Tx", "Tz", "psi_V_x", "psi_V_z")),
+                        ("velocity", ("V", "Tz", "psi_Tx_x", "psi_Tz_z"))):
+        with tempfile.TemporaryDirectory() as d:
+            for k, v in ins.items():
+                v.tofile(os.path.join(d, k + ".in"))
+            run(exe, args + ["mode=" + mode], d)
+            for k in names:
+                out["%s_%s" % (mode, k)] = read(d, k + ".out", np.float32)
+            for c in SH_CPML:
+                n = nx if c.endswith("_x") else nz
+                for a in "bck":
+                    out["%s_%s" % (a, c)] = read(d, "%s_%s.f32" % (a, c), np.float32, n)
+    out["dims"] = np.array([nx, nz, npml], np.int32)
+    np.savez_compressed(os.path.join(OUT, "sh_fd.npz"), **out)
+    print("sh_fd ok")
 
 
 if __name__ == "__main__":

@@ -105,3 +105,58 @@ def test_fp64_golden_seismograms():
     for i in range(36):
         if np.abs(ru[:, i]).max() > 0:
             assert rel(su[:, i], ru[:, i]) < 1e-10
+
+
+def test_sh_passes_match_reference_cpu_loop():
+    """One SH stress pass and one velocity pass of the oracle against the
+    reference's own SH CPU loop (SH/fd/fd.cpp:45-193, compiled from the
+    reference sources; vectors in tests/golden/ref_host/sh_fd.npz), from
+    random fields, psi and material with the reference's C-PML profiles.
+    That loop differs from the GPU kernels the oracle follows only within 4
+    cells of the domain edge (order reduction without iz, no early return,
+    row-major wrap), so interior cells -- including the side and bottom
+    C-PML strips -- are compared, and the top imaging row in full."""
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_host", "sh_fd.npz"))
+    nx, nz, npml = (int(v) for v in d["dims"])
+    cp = {c: tuple(d["%s_%s" % (a, c)] for a in "bck") for c in ("Tx_x", "V_x", "Tz_z", "V_z")}
+
+    def fresh():
+        mat = {k: d["in_" + k] for k in ("BV", "MUX", "MUZ")}
+        sim = O.SH(nx, nz, npml, mat, cp, dtype=np.float32)
+        for k in ("V", "Tx", "Tz"):
+            getattr(sim, k)[:] = d["in_" + k]
+        for k in ("V_x", "V_z", "Tx_x", "Tz_z"):
+            sim.psi[k][:] = d["in_psi_" + k]
+        return sim
+
+    ix, iz = np.meshgrid(np.arange(nx), np.arange(nz))
+    inner = (ix >= 4) & (ix <= nx - 5) & (iz >= 4) & (iz <= nz - 5)
+    # psi cells: x-psi index ix + iz*2npml (left), npml+nx-1-ix + iz*2npml (right)
+    xl = inner & (ix < npml)
+    xr = inner & (ix >= nx - npml)
+    zb = inner & (iz >= nz - npml)
+
+    def xpsi(a):
+        a2 = a.reshape(nz, 2 * npml)
+        return np.concatenate([a2[:, :npml][xl[:, :npml]], a2[:, npml:][xr[:, nx - npml:][:, ::-1]]])
+
+    def zpsi(a):
+        rows = a.reshape(2 * npml, nx)
+        return rows[(npml + nz - 1 - iz)[zb], ix[zb]]
+
+    sim = fresh()
+    sim.stress()
+    for k in ("Tx", "Tz"):
+        ref = d["stress_" + k].reshape(nz, nx)
+        np.testing.assert_allclose(getattr(sim, k)[inner], ref[inner], rtol=2e-6, atol=2e-6)
+    np.testing.assert_allclose(xpsi(sim.psi["V_x"]), xpsi(d["stress_psi_V_x"]), rtol=2e-6, atol=2e-6)
+    np.testing.assert_allclose(zpsi(sim.psi["V_z"]), zpsi(d["stress_psi_V_z"]), rtol=2e-6, atol=2e-6)
+    assert xl.any() and xr.any() and zb.any()
+
+    sim = fresh()
+    sim.velocity()
+    ref = d["velocity_V"].reshape(nz, nx)
+    np.testing.assert_allclose(sim.V[inner], ref[inner], rtol=2e-6, atol=2e-6)
+    np.testing.assert_array_equal(sim.Tz[0], d["velocity_Tz"].reshape(nz, nx)[0])
+    np.testing.assert_allclose(xpsi(sim.psi["Tx_x"]), xpsi(d["velocity_psi_Tx_x"]), rtol=2e-6, atol=2e-6)
+    np.testing.assert_allclose(zpsi(sim.psi["Tz_z"]), zpsi(d["velocity_psi_Tz_z"]), rtol=2e-6, atol=2e-6)

@@ -128,3 +128,50 @@ def test_linearity_and_zero_source(dtype):
             x, y = getattr(b, n).astype(np.float64), 4 * getattr(a, n).astype(np.float64)
             assert np.linalg.norm(x - y) <= 1e-6 * np.linalg.norm(y)
         assert not np.any(getattr(z, n))
+
+
+def _unit_sh(nx, nz, dtype=np.float64):
+    """SH material MUX = MUZ = BV = 1, no C-PML."""
+    z = lambda v: np.full((nz, nx), v, dtype)
+    mat = {"BV": z(1.0), "MUX": z(1.0), "MUZ": z(1.0)}
+    cp = {k: (np.zeros(n, dtype), np.zeros(n, dtype), np.zeros(n, dtype))
+          for k, n in (("Tx_x", nx), ("V_x", nx), ("Tz_z", nz), ("V_z", nz))}
+    return O.SH(nx, nz, 0, mat, cp, dtype=dtype)
+
+
+@pytest.mark.parametrize("deg", [1, 3, 5, 8])
+def test_sh_stencils_exact_on_polynomials(deg):
+    """SH stress: Tx += MUX Dx+ V (at x+1/2), Tz += MUZ Dz- V (at z-1/2);
+    SH velocity: V += BV (Dx- Tx + Dz+ Tz) -- exact up to degree 8."""
+    n, x0 = 40, 17.3
+    a = np.arange(n, dtype=np.float64)
+    p = lambda t: ((t - x0) / 10.0) ** deg
+    dp_ = lambda t: deg * ((t - x0) / 10.0) ** (deg - 1) / 10.0
+    sim = _unit_sh(n, n)
+    sim.V[:] = p(a)[None, :] + p(a)[:, None]
+    sim.stress()
+    sl = slice(5, 35)
+    np.testing.assert_allclose(sim.Tx[20, sl], dp_(a + 0.5)[sl], rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(sim.Tz[sl, 20], dp_(a - 0.5)[sl], rtol=1e-10, atol=1e-12)
+    sim = _unit_sh(n, n)
+    sim.Tx[:] = p(a)[None, :]
+    sim.Tz[:] = p(a)[:, None]
+    sim.velocity()
+    # Dx- Tx at x - 1/2 plus Dz+ Tz at z + 1/2
+    np.testing.assert_allclose(sim.V[20, sl], dp_(a - 0.5)[sl] + dp_(20 + 0.5), rtol=1e-10, atol=1e-12)
+
+
+def test_sh_stress_skips_edge_cells_and_images_the_top_row():
+    """SH/fd/cu_fd.cu:140 (no stress update at ix == 0, ix >= nx-1,
+    iz >= nz-1) and :226 (Tz row 0 = -Tz row 1 before the velocity pass)."""
+    n = 30
+    sim = _unit_sh(n, n)
+    rng = np.random.default_rng(3)
+    sim.V[:] = rng.standard_normal((n, n))
+    sim.stress()
+    assert not sim.Tx[:, 0].any() and not sim.Tx[:, -1].any() and not sim.Tx[-1, :].any()
+    assert not sim.Tz[:, 0].any() and not sim.Tz[-1, :].any()
+    assert sim.Tx[1:-1, 1:-1].all()
+    tz1 = sim.Tz[1].copy()
+    sim.velocity()
+    assert np.array_equal(sim.Tz[0], -tz1)
