@@ -174,6 +174,13 @@ class Context:
     def tile(self, t):
         check(self.lib.fd23_set_tile(self.h, t))
 
+    def tile_shape(self, tile=None):
+        """(rz, spb, fused_rows) of a tile index (default: the current one)."""
+        t = self.tile if tile is None else tile
+        a, b, f = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(self.lib.fd23_tile_shape(t, ctypes.byref(a), ctypes.byref(b), ctypes.byref(f)))
+        return a.value, b.value, f.value
+
     def autotune(self, steps=4):
         best = ctypes.c_double()
         check(self.lib.fd23_autotune(self.h, steps, ctypes.byref(best)))

@@ -46,6 +46,7 @@ long fd23_ctx_sstride(fd23_ctx *c);
 void *&fd23_ctx_box(fd23_ctx *c);
 const fd23_desc &fd23_ctx_desc(fd23_ctx *c);
 int *fd23_ctx_clock(fd23_ctx *c);
+void fd23_ctx_invalidate_graphs(fd23_ctx *c);
 extern "C" void fd23_set_error_(const char *msg);
 void fd23_box_free(fd23_ctx *c);
 
@@ -412,6 +413,7 @@ extern "C" int fd23_box_configure(fd23_ctx *c, int shot, int x0, int z0, int bx,
         return FD23_EHIP;
     }
     fd23_ctx_box(c) = b;
+    fd23_ctx_invalidate_graphs(c);
     return FD23_OK;
 }
 

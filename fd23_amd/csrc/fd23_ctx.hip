@@ -2,6 +2,7 @@
 // C ABI declared in include/fd23.h.
 #include "fd23.h"
 #include "fd_kernels.hpp"
+#include "fd_fused.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -11,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -43,15 +45,24 @@ extern "C" const char *fd23_last_error(void) { return g_err.c_str(); }
 extern "C" void fd23_set_error_(const char *msg) { g_err = msg; }
 
 // ---------------------------------------------------------------------------
-// tile shapes: rows per thread (tile height 4*RZ) x shots per workgroup
+// tile shapes: two-pass tiles (rows per thread RZ, tile height 4*RZ, shots
+// per workgroup SPB) and fused P-SV steps (segment height FH rows)
 // ---------------------------------------------------------------------------
 struct TileCfg {
-    int rz, spb;
+    int rz, spb, fh;
 };
-static const TileCfg kTiles[] = {{4, 1}, {2, 1}, {8, 1}, {4, 2}, {4, 4}, {4, 8}, {8, 2}, {8, 4}, {2, 4}, {2, 8}};
+static const TileCfg kTiles[] = {{4, 1, 0}, {2, 1, 0}, {8, 1, 0}, {4, 2, 0}, {4, 4, 0}, {4, 8, 0}, {8, 2, 0},
+                                 {8, 4, 0}, {2, 4, 0}, {2, 8, 0}, {0, 1, 32}, {0, 1, 64}, {0, 1, 128}};
 static const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 extern "C" int fd23_num_tiles(void) { return kNumTiles; }
+extern "C" int fd23_tile_shape(int tile, int *rz, int *spb, int *fused_rows) {
+    if (tile < 0 || tile >= kNumTiles) return fail(FD23_EINVAL, "bad tile %d", tile);
+    if (rz) *rz = kTiles[tile].rz;
+    if (spb) *spb = kTiles[tile].spb;
+    if (fused_rows) *fused_rows = kTiles[tile].fh;
+    return FD23_OK;
+}
 
 constexpr int kMaxTZ = 32;     // largest tile height (BY * max RZ)
 constexpr int kGuard = 256;    // zero elements before shot 0 / after the last
@@ -78,6 +89,13 @@ struct fd23_ctx {
     void *field_alloc[5] = {};
     void *fields[5] = {};   // shot 0, row 0, col 0
     void *psi[8] = {};
+    // the other buffer set of the fused step (allocated on first use); the
+    // fused step reads the current set and writes this one, then they swap
+    void *alt_alloc[5] = {};
+    void *alt_fields[5] = {};
+    void *alt_psi[8] = {};
+    int cur = 0;            // which set is current (keys the captured graphs)
+    bool tables_dirty = false;
     void *mat[5] = {};
     void *prof = nullptr;   // [npsi][3][n] compute type
     long prof_off[8] = {};  // element offset of profile p (b, then c, then k)
@@ -102,6 +120,14 @@ struct fd23_ctx {
     std::vector<Tap> h_taps;
     std::vector<SrcInfo> h_srcinfo;
 
+    // per-strip lists of the fused step
+    int *d_ftap_start = nullptr, *d_frec_start = nullptr;
+    TapF *d_ftaps = nullptr;
+    RecF *d_frecs = nullptr;
+    size_t ftap_start_cap = 0, frec_start_cap = 0, ftaps_cap = 0, frecs_cap = 0;
+    bool fevents_dirty = true;
+    int fevents_h = -1;
+
     // per-tile lists of the fused graph loop
     int *d_tap_start = nullptr, *d_rec_start = nullptr;
     TapT *d_tapsT = nullptr;
@@ -121,7 +147,7 @@ struct fd23_ctx {
     bool have_mat = false, have_cpml = false;
 
     // graphs keyed by (itrecord, steps)
-    std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs; // (itrecord, steps, current set)
 
     // box coupling state (see fd23_box.hip)
     void *box = nullptr;
@@ -143,6 +169,11 @@ long fd23_ctx_sstride(fd23_ctx *c) { return c->sstride; }
 void *&fd23_ctx_box(fd23_ctx *c) { return c->box; }
 const fd23_desc &fd23_ctx_desc(fd23_ctx *c) { return c->d; }
 int *fd23_ctx_clock(fd23_ctx *c) { return c->clock; }
+// captured graphs bake in the per-step launches; the box changes them
+void fd23_ctx_invalidate_graphs(fd23_ctx *c) {
+    for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+}
 const float *fd23_ctx_scl(fd23_ctx *c) { return c->scl; }
 
 static bool is_psv(const fd23_ctx *c) { return c->d.wave == FD23_WAVE_PSV; }
@@ -170,6 +201,11 @@ static size_t layout(const fd23_desc *d, int *px, int *rows, long *sstride, long
     bytes += (size_t)2 * np * 3 * (d->nx + d->nz) * (*csize);
     bytes += (size_t)d->max_frames * S * nc * d->max_rec * (*csize);
     bytes += (size_t)S * nc * d->max_rec * sizeof(long);
+    // P-SV: the second field/psi set of the fused step
+    if (psv) {
+        bytes += (size_t)nf * ((size_t)S * (*sstride) + 2 * kGuard) * (*esize);
+        bytes += (size_t)np * S * ((*xpsi) + (*zpsi)) * (*esize);
+    }
     return bytes;
 }
 
@@ -189,10 +225,13 @@ static void free_ctx(fd23_ctx *c) {
     fd23_box_free(c);
     for (auto p : c->field_alloc) if (p) hipFree(p);
     for (auto p : c->psi) if (p) hipFree(p);
+    for (auto p : c->alt_alloc) if (p) hipFree(p);
+    for (auto p : c->alt_psi) if (p) hipFree(p);
     for (auto p : c->mat) if (p) hipFree(p);
     void *others[] = {c->prof, c->d_fieldtab, c->d_comptab, c->d_compfield, c->scl, c->amax, c->clock,
                       c->d_taps, c->d_src, c->d_stf, c->d_roff, c->d_nrec, c->seis, c->d_tap_start,
-                      c->d_rec_start, c->d_tapsT, c->d_recs};
+                      c->d_rec_start, c->d_tapsT, c->d_recs, c->d_ftap_start, c->d_frec_start, c->d_ftaps,
+                      c->d_frecs};
     for (auto p : others) if (p) hipFree(p);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -484,6 +523,7 @@ static int upload_sources(fd23_ctx *c) {
     HIPCHK(hipStreamSynchronize(c->stream));
     c->src_dirty = false;
     c->events_dirty = true;
+    c->fevents_dirty = true;
     return FD23_OK;
 }
 
@@ -514,13 +554,19 @@ extern "C" int fd23_set_receivers(fd23_ctx *c, int shot, int comp, const int *rx
                           c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->events_dirty = true;
+    c->fevents_dirty = true;
     return FD23_OK;
 }
 
 // ---------------------------------------------------------------------------
 // per-tile event lists for the fused injection / gather of the graph loop
 // ---------------------------------------------------------------------------
-static int tile_rows(const fd23_ctx *c) { return BY * kTiles[c->tile].rz; }
+static bool tile_fused(int t) { return kTiles[t].fh > 0; }
+// the fused step runs in fd23_run for P-SV contexts without a box; the
+// staged entry points, the box and SH always use two-pass tiles
+static bool use_fused(const fd23_ctx *c) { return tile_fused(c->tile) && is_psv(c) && !c->box; }
+static int pass_tile(const fd23_ctx *c) { return tile_fused(c->tile) ? 0 : c->tile; }
+static int tile_rows(const fd23_ctx *c) { return BY * kTiles[pass_tile(c)].rz; }
 
 template <class T> static int upload_vec(fd23_ctx *c, T *&dev, size_t &cap, const std::vector<T> &v) {
     if (v.size() > cap || !dev) {
@@ -533,7 +579,7 @@ template <class T> static int upload_vec(fd23_ctx *c, T *&dev, size_t &cap, cons
 }
 
 static int build_events(fd23_ctx *c) {
-    if (!c->events_dirty && c->events_tile == c->tile) return FD23_OK;
+    if (!c->events_dirty && c->events_tile == pass_tile(c)) return FD23_OK;
     const int TZ = tile_rows(c);
     const int ntx = (c->d.nx + TX - 1) / TX, ntz = (c->d.nz + TZ - 1) / TZ;
     const int S = c->d.nshots;
@@ -584,11 +630,116 @@ static int build_events(fd23_ctx *c) {
     if ((r = upload_vec(c, c->d_recs, c->recs_cap, recs))) return r;
     HIPCHK(hipStreamSynchronize(c->stream));
     c->events_dirty = false;
-    c->events_tile = c->tile;
+    c->events_tile = pass_tile(c);
     // graphs hold the old list pointers
     for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
     c->graphs.clear();
     return FD23_OK;
+}
+
+// Lists of the fused step: source taps per (strip, shot) -- every strip
+// whose 64 stress columns (its 56 own columns and 4 on each side) hold
+// the tap -- and receivers per (segment, strip, shot) sorted by row.
+static int build_fused_events(fd23_ctx *c) {
+    const int H = kTiles[c->tile].fh;
+    if (!c->fevents_dirty && c->fevents_h == H) return FD23_OK;
+    const int nstrip = (c->d.nx + FO - 1) / FO, nseg = (c->d.nz + H - 1) / H;
+    const int S = c->d.nshots;
+    std::vector<std::vector<TapF>> tb((size_t)nstrip * S);
+    std::vector<std::vector<RecF>> rb((size_t)nseg * nstrip * S);
+    for (int s = 0; s < S; s++) {
+        const SrcInfo &si = c->h_srcinfo[s];
+        for (int i = 0; i < si.ntap; i++) {
+            const Tap &tp = c->h_taps[si.tap0 + i];
+            const int z = (int)(tp.off / c->px), x = (int)(tp.off % c->px);
+            for (int st = 0; st < nstrip; st++) {
+                const int lane = x - (st * FO - HALO);
+                if (lane >= 0 && lane < FW) tb[(size_t)st * S + s].push_back(TapF{z, lane, tp.field, tp.mode, tp.m, tp.q});
+            }
+        }
+        for (int comp = 0; comp < c->ncomp; comp++) {
+            const int n = c->h_nrec[(size_t)s * c->ncomp + comp];
+            const long base = ((long)s * c->ncomp + comp) * c->d.max_rec;
+            for (int r = 0; r < n; r++) {
+                const long off = c->h_roff[base + r];
+                if (off < 0) continue; // outside the grid: the frame keeps 0
+                const int z = (int)(off / c->px), x = (int)(off % c->px);
+                const int st = x / FO, sg = z / H;
+                rb[((size_t)sg * nstrip + st) * S + s].push_back(RecF{z, x - st * FO + HALO, comp, (int)(base + r)});
+            }
+        }
+    }
+    std::vector<int> ts(tb.size() + 1), rs(rb.size() + 1);
+    std::vector<TapF> taps;
+    std::vector<RecF> recs;
+    for (size_t k = 0; k < tb.size(); k++) {
+        ts[k] = (int)taps.size();
+        taps.insert(taps.end(), tb[k].begin(), tb[k].end());
+    }
+    ts[tb.size()] = (int)taps.size();
+    for (size_t k = 0; k < rb.size(); k++) {
+        std::stable_sort(rb[k].begin(), rb[k].end(), [](const RecF &a, const RecF &b) { return a.row < b.row; });
+        rs[k] = (int)recs.size();
+        recs.insert(recs.end(), rb[k].begin(), rb[k].end());
+    }
+    rs[rb.size()] = (int)recs.size();
+    if (taps.empty()) taps.push_back(TapF{-1, -1, 0, 0, 0.0, 0.0});
+    if (recs.empty()) recs.push_back(RecF{-1, -1, 0, 0});
+    int r;
+    if ((r = upload_vec(c, c->d_ftap_start, c->ftap_start_cap, ts))) return r;
+    if ((r = upload_vec(c, c->d_frec_start, c->frec_start_cap, rs))) return r;
+    if ((r = upload_vec(c, c->d_ftaps, c->ftaps_cap, taps))) return r;
+    if ((r = upload_vec(c, c->d_frecs, c->frecs_cap, recs))) return r;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->fevents_dirty = false;
+    c->fevents_h = H;
+    for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+    return FD23_OK;
+}
+
+// second field/psi set of the fused step, zeroed (pads stay zero)
+static int ensure_alt(fd23_ctx *c) {
+    if (c->alt_alloc[0]) return FD23_OK;
+    const size_t S = c->d.nshots;
+    for (int f = 0; f < c->nfield; f++) {
+        const size_t n = ((size_t)S * c->sstride + 2 * kGuard) * c->esize;
+        if (hipMalloc(&c->alt_alloc[f], n) != hipSuccess) return fail(FD23_ENOMEM, "fused buffer set");
+        HIPCHK(hipMemsetAsync(c->alt_alloc[f], 0, n, c->stream));
+        c->alt_fields[f] = (char *)c->alt_alloc[f] + (size_t)(kGuard + (long)HALO * c->px) * c->esize;
+    }
+    for (int p = 0; p < c->npsi; p++) {
+        const bool xdir = p < 4;
+        const size_t n = std::max<size_t>(1, S * (xdir ? c->xpsi : c->zpsi)) * c->esize;
+        if (hipMalloc(&c->alt_psi[p], n) != hipSuccess) return fail(FD23_ENOMEM, "fused buffer set");
+        HIPCHK(hipMemsetAsync(c->alt_psi[p], 0, n, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FD23_OK;
+}
+
+static void swap_sets(fd23_ctx *c) {
+    for (int f = 0; f < 5; f++) {
+        std::swap(c->field_alloc[f], c->alt_alloc[f]);
+        std::swap(c->fields[f], c->alt_fields[f]);
+    }
+    for (int p = 0; p < 8; p++) std::swap(c->psi[p], c->alt_psi[p]);
+    c->cur ^= 1;
+    c->tables_dirty = true;
+}
+
+// the device pointer tables of the staged inject/record kernels, rewritten
+// in stream order after the buffer sets have swapped
+static void refresh_tables(fd23_ctx *c) {
+    if (!c->tables_dirty) return;
+    PtrTable5 t{};
+    for (int f = 0; f < 5; f++) t.p[f] = c->fields[f];
+    hipLaunchKernelGGL(k_set_ptrs, dim3(1), dim3(64), 0, c->stream, (void **)c->d_fieldtab, t, 5);
+    PtrTable5 cp{};
+    cp.p[0] = c->fields[0];
+    cp.p[1] = is_psv(c) ? c->fields[1] : nullptr;
+    hipLaunchKernelGGL(k_set_ptrs, dim3(1), dim3(64), 0, c->stream, (void **)c->d_comptab, cp, 2);
+    c->tables_dirty = false;
 }
 
 // ---------------------------------------------------------------------------
@@ -691,7 +842,7 @@ template <class S, int RZ, int SPB> static void launch_velocity_t(fd23_ctx *c, c
 }
 
 template <class S> static void launch_pass(fd23_ctx *c, bool stress, const Events &ev) {
-    const TileCfg t = kTiles[c->tile];
+    const TileCfg t = kTiles[pass_tile(c)];
 #define TILE_CASE(R, P)                                                                        \
     if (t.rz == R && t.spb == P) {                                                             \
         if (stress) launch_stress_t<S, R, P>(c, ev); else launch_velocity_t<S, R, P>(c, ev); \
@@ -711,6 +862,62 @@ static void dispatch_pass(fd23_ctx *c, bool stress, const Events &ev) {
     }
 }
 
+template <class S, int H> static void launch_fused_t(fd23_ctx *c, int k, int itrecord, bool record) {
+    using C = typename Store<S>::C;
+    const Grid g = make_grid(c);
+    FusedWork w;
+    w.nstrip = (c->d.nx + FO - 1) / FO;
+    w.nseg = (c->d.nz + H - 1) / H;
+    w.H = H;
+    w.nwork = w.nstrip * w.nseg * c->d.nshots;
+    const dim3 grid(round_up(w.nwork, 8));
+    FusedEvents ev{};
+    ev.clock = c->clock;
+    ev.k = k;
+    ev.itrecord = itrecord;
+    ev.tap_start = c->d_ftap_start;
+    ev.taps = c->d_ftaps;
+    ev.src = c->d_src;
+    ev.stf = c->d_stf;
+    ev.rec_start = c->d_frec_start;
+    ev.recs = c->d_frecs;
+    ev.seis = c->seis;
+    ev.max_frames = (c->seis && record) ? c->d.max_frames : 0;
+    ev.frame_size = (long)c->d.nshots * c->ncomp * c->d.max_rec;
+    auto fields = [](void *const *f) {
+        return PsvFields<S>{(S *)f[0], (S *)f[1], (S *)f[2], (S *)f[3], (S *)f[4]};
+    };
+    PsvMat5<C> m{(const C *)c->mat[0], (const C *)c->mat[1], (const C *)c->mat[2], (const C *)c->mat[3],
+                 (const C *)c->mat[4]};
+    PsvStressPsi<S> spa{(S *)c->psi[2], (S *)c->psi[3], (S *)c->psi[6], (S *)c->psi[7]};
+    PsvStressPsi<S> spb{(S *)c->alt_psi[2], (S *)c->alt_psi[3], (S *)c->alt_psi[6], (S *)c->alt_psi[7]};
+    PsvVelPsi<S> vpa{(S *)c->psi[0], (S *)c->psi[1], (S *)c->psi[5], (S *)c->psi[4]};
+    PsvVelPsi<S> vpb{(S *)c->alt_psi[0], (S *)c->alt_psi[1], (S *)c->alt_psi[5], (S *)c->alt_psi[4]};
+    PsvStressProf<C> spr{prof_of<C>(c, 2), prof_of<C>(c, 3), prof_of<C>(c, 6), prof_of<C>(c, 7)};
+    PsvVelProf<C> vpr{prof_of<C>(c, 0), prof_of<C>(c, 1), prof_of<C>(c, 5), prof_of<C>(c, 4)};
+    hipLaunchKernelGGL((k_psv_fused<S, H>), grid, dim3(FW), 0, c->stream, fields(c->fields), fields(c->alt_fields),
+                       m, spa, spb, vpa, vpb, spr, vpr, g, w, ev, c->scl, c->amax);
+}
+
+template <class S> static void launch_fused_h(fd23_ctx *c, int k, int itrecord, bool record) {
+    switch (kTiles[c->tile].fh) {
+    case 32: launch_fused_t<S, 32>(c, k, itrecord, record); break;
+    case 64: launch_fused_t<S, 64>(c, k, itrecord, record); break;
+    default: launch_fused_t<S, 128>(c, k, itrecord, record); break;
+    }
+}
+
+// one fused step: reads the current set, writes the other, then swaps
+static void dispatch_fused(fd23_ctx *c, int k, int itrecord, bool record = true) {
+    switch (c->d.storage) {
+    case FD23_STORE_F16: launch_fused_h<__half>(c, k, itrecord, record); break;
+    case FD23_STORE_BF16: launch_fused_h<__hip_bfloat16>(c, k, itrecord, record); break;
+    case FD23_STORE_F64: launch_fused_h<double>(c, k, itrecord, record); break;
+    default: launch_fused_h<float>(c, k, itrecord, record); break;
+    }
+    swap_sets(c);
+}
+
 template <class S> static void launch_inject_t(fd23_ctx *c, int it) {
     const int nthr = std::min(256, round_up(c->d.nshots, 64));
     hipLaunchKernelGGL((k_inject<S>), dim3(1), dim3(nthr), 0, c->stream, (S *const *)c->d_fieldtab, c->sstride,
@@ -718,6 +925,7 @@ template <class S> static void launch_inject_t(fd23_ctx *c, int it) {
 }
 
 static void dispatch_inject(fd23_ctx *c, int it) {
+    refresh_tables(c);
     switch (c->d.storage) {
     case FD23_STORE_F16: launch_inject_t<__half>(c, it); break;
     case FD23_STORE_BF16: launch_inject_t<__hip_bfloat16>(c, it); break;
@@ -737,6 +945,7 @@ template <class S> static void launch_record_t(fd23_ctx *c, int it, int itrecord
 }
 
 static void dispatch_record(fd23_ctx *c, int it, int itrecord) {
+    refresh_tables(c);
     switch (c->d.storage) {
     case FD23_STORE_F16: launch_record_t<__half>(c, it, itrecord); break;
     case FD23_STORE_BF16: launch_record_t<__hip_bfloat16>(c, it, itrecord); break;
@@ -881,6 +1090,15 @@ static int enqueue_step(fd23_ctx *c, int k, int itrecord) {
     // a scale check ends every kScaleEvery-th step of a chunk, so the passes
     // of steps k = 0, kScaleEvery, ... commit the scales they wrote
     const bool commit = scaled_storage(c) && k % kScaleEvery == 0;
+    if (use_fused(c)) {
+        dispatch_fused(c, k, itrecord);
+        if (commit) {
+            enqueue_commit(c, true);
+            enqueue_commit(c, false);
+        }
+        if (scaled_storage(c) && (k + 1) % kScaleEvery == 0) enqueue_scale_control(c);
+        return FD23_OK;
+    }
     if (c->box && (r = fd23_box_launch_pre_stress(c, k))) return r;
     dispatch_pass(c, true, ev);
     if (commit) enqueue_commit(c, true);
@@ -895,7 +1113,7 @@ static int enqueue_step(fd23_ctx *c, int k, int itrecord) {
 static const int kGraphSteps = 32;
 
 static int get_graph(fd23_ctx *c, int itrecord, int steps, hipGraphExec_t *out) {
-    auto key = std::make_pair(itrecord, steps);
+    const auto key = std::make_tuple(itrecord, steps, c->cur);
     auto it = c->graphs.find(key);
     if (it != c->graphs.end()) {
         *out = it->second;
@@ -923,6 +1141,7 @@ extern "C" int fd23_run(fd23_ctx *c, int it0, int nt, int itrecord) {
     if (r) return r;
     if (nt < 0 || it0 < 0 || itrecord < 1) return fail(FD23_EINVAL, "bad run range");
     if ((r = build_events(c))) return r;
+    if (use_fused(c) && ((r = ensure_alt(c)) || (r = build_fused_events(c)))) return r;
     if ((r = init_scales(c))) return r;
     hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(64), 0, c->stream, c->clock, it0);
     const int full = nt / kGraphSteps, rest = nt % kGraphSteps;
@@ -1092,6 +1311,7 @@ extern "C" int fd23_set_state(fd23_ctx *c, int shot, const void *buf, size_t byt
 // ---------------------------------------------------------------------------
 extern "C" int fd23_set_tile(fd23_ctx *c, int tile) {
     if (!c || tile < 0 || tile >= kNumTiles) return fail(FD23_EINVAL, "bad tile %d", tile);
+    if (tile_fused(tile) && !is_psv(c)) return fail(FD23_EINVAL, "the fused step is P-SV only");
     if (c->tile != tile) {
         c->tile = tile;
         for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
@@ -1107,6 +1327,8 @@ extern "C" double fd23_bytes_per_step(const fd23_ctx *c) {
     const double wf = c->esize, wm = c->csize;
     const double nxp = 2.0 * c->d.npml * c->d.nz;
     const double nzp = (c->d.topabsorb ? 2.0 : 1.0) * c->d.npml * c->d.nx;
+    // the fused step reads the five fields once and writes them once
+    if (is_psv(c) && use_fused(c)) return S * (10 * wf * N + 8 * wf * (nxp + nzp)) + 5 * wm * N;
     if (is_psv(c)) return S * (15 * wf * N + 8 * wf * (nxp + nzp)) + 5 * wm * N;
     return S * (9 * wf * N + 4 * wf * (nxp + nzp)) + 3 * wm * N;
 }
@@ -1115,23 +1337,39 @@ extern "C" int fd23_time_passes(fd23_ctx *c, int nsteps, double *stress_ms, doub
     int r = ready(c);
     if (r) return r;
     if (nsteps < 1 || nsteps > 4096) return fail(FD23_EINVAL, "nsteps must be in [1, 4096]");
-    // nsteps stress passes back to back, then nsteps velocity passes: the
-    // queue never drains and no event sits between two passes, so each
-    // interval over nsteps is one kernel's duration plus its dispatch
     hipEvent_t e0, e1, e2;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventCreate(&e2));
     const bool sc = scaled_storage(c);
-    dispatch_pass(c, true, kNoEvents); // first pass outside the timing, then
-    if (sc) enqueue_commit(c, true);   // the scales of its arrays are current
-    dispatch_pass(c, false, kNoEvents);
-    if (sc) enqueue_commit(c, false);
-    hipEventRecord(e0, c->stream);
-    for (int i = 0; i < nsteps; i++) dispatch_pass(c, true, kNoEvents);
-    hipEventRecord(e1, c->stream);
-    for (int i = 0; i < nsteps; i++) dispatch_pass(c, false, kNoEvents);
-    hipEventRecord(e2, c->stream);
+    const bool fused = use_fused(c);
+    if (fused && ((r = ensure_alt(c)) || (r = build_fused_events(c)))) return r;
+    if (fused) {
+        // nsteps fused steps back to back (no recording): the first value
+        // is the step time, the second 0
+        dispatch_fused(c, 0, 1, false);
+        if (sc) {
+            enqueue_commit(c, true);
+            enqueue_commit(c, false);
+        }
+        hipEventRecord(e0, c->stream);
+        for (int i = 0; i < nsteps; i++) dispatch_fused(c, 0, 1, false);
+        hipEventRecord(e1, c->stream);
+        hipEventRecord(e2, c->stream);
+    } else {
+        // nsteps stress passes back to back, then nsteps velocity passes:
+        // the queue never drains and no event sits between two passes, so
+        // each interval over nsteps is one kernel's duration plus its dispatch
+        dispatch_pass(c, true, kNoEvents); // first pass outside the timing, then
+        if (sc) enqueue_commit(c, true);   // the scales of its arrays are current
+        dispatch_pass(c, false, kNoEvents);
+        if (sc) enqueue_commit(c, false);
+        hipEventRecord(e0, c->stream);
+        for (int i = 0; i < nsteps; i++) dispatch_pass(c, true, kNoEvents);
+        hipEventRecord(e1, c->stream);
+        for (int i = 0; i < nsteps; i++) dispatch_pass(c, false, kNoEvents);
+        hipEventRecord(e2, c->stream);
+    }
     const hipError_t se = hipEventSynchronize(e2);
     float a = 0, b = 0;
     if (se == hipSuccess) {
@@ -1181,11 +1419,12 @@ extern "C" int fd23_autotune(fd23_ctx *c, int steps, double *best_ms) {
     std::vector<unsigned> amax_saved((size_t)16 * kAmaxSpread);
     HIPCHK(hipMemcpy(scl_saved, c->scl, sizeof scl_saved, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(amax_saved.data(), c->amax, sizeof(unsigned) * amax_saved.size(), hipMemcpyDeviceToHost));
-    const int orig = c->tile;
+    const int orig = c->tile, orig_set = c->cur;
     int best = orig;
     double bt = 1e30;
     for (int t = 0; t < kNumTiles; t++) {
         if (kTiles[t].spb > 1 && c->d.nshots < kTiles[t].spb) continue;
+        if (tile_fused(t) && (!is_psv(c) || c->box)) continue;
         c->tile = t;
         double a, b;
         fd23_time_passes(c, 1, &a, &b); // warm-up
@@ -1195,7 +1434,8 @@ extern "C" int fd23_autotune(fd23_ctx *c, int steps, double *best_ms) {
             best = t;
         }
     }
-    // restore
+    // restore (the fused candidates may have left the other set current)
+    if (c->cur != orig_set) swap_sets(c);
     size_t k = 0;
     for (int f = 0; f < c->nfield; f++, k++)
         hipMemcpyAsync(c->field_alloc[f], saved[k], sizes[k], hipMemcpyDeviceToDevice, c->stream);

@@ -124,7 +124,7 @@ __device__ __forceinline__ bool src_sample(const SrcInfo &si, const double *stf,
     return true;
 }
 
-template <class C> __device__ __forceinline__ C apply_tap(C v, const TapT &tp, double sv) {
+template <class C, class T> __device__ __forceinline__ C apply_tap(C v, const T &tp, double sv) {
     if (tp.mode == 0) {
         const C prod = C(sv) * C(tp.m);
         return v + prod * C(tp.q);
@@ -208,6 +208,72 @@ enum { SC_U = 0, SC_W, SC_TXX, SC_TZZ, SC_TXZ, SC_PTXX_X, SC_PTXZ_X, SC_PU_X, SC
 constexpr int kNextScale = 16;
 constexpr int kScaleExpMax = 100;
 
+// ---------------------------------------------------------------------------
+// per-cell P-SV updates, shared by the two-pass kernels and the fused step so
+// that both run the same arithmetic (cu_fd.cu:183-243 and :334-386)
+// ---------------------------------------------------------------------------
+struct PsiScl4 {
+    float r[4], w[4]; // read scales and inverse write scales
+};
+
+template <class C>
+__device__ __forceinline__ void psv_stress_interior(C ux, C uz, C wx, C wz, C lm, C mm, C ma, C &a, C &b, C &cc) {
+    const C gam = lm + C(2.0) * mm;
+    a = mad(gam, ux, a); b = mad(lm, ux, b); cc = mad(ma, wx, cc);
+    a = mad(lm, wz, a); b = mad(gam, wz, b); cc = mad(ma, uz, cc);
+}
+
+// exact path of one stress cell; psi read from rd and (when own) written to
+// wr; sc: U_x W_x U_z W_z; mp: running max of U_x W_x W_z U_z
+template <class S, class C>
+__device__ __forceinline__ void psv_stress_exact(const Grid &g, int ix, int iz, int shot, C ux, C uz, C wx, C wz,
+                                                 C lm, C mm, C ma, C &a, C &b, C &cc, const PsvStressPsi<S> &rd,
+                                                 const PsvStressPsi<S> &wr, bool own,
+                                                 const PsvStressProf<C> &pr, const PsiScl4 &sc, float (&mp)[4]) {
+    using ST = Store<S>;
+    const int nx = g.nx, nz = g.nz, npml = g.npml;
+    const C gam = lm + C(2.0) * mm;
+    if (ix < npml || ix >= nx - npml) {
+        const long pi = (long)shot * g.xpsi_stride + xpsi_idx(ix, iz, nx, npml);
+        C p1 = mad(pr.Ux.b[ix], ST::ld(rd.Ux, pi, sc.r[0]), pr.Ux.c[ix] * ux);
+        if (own) ST::st(wr.Ux, pi, p1, sc.w[0]);
+        const C e1 = mad(ux, pr.Ux.k[ix], p1);
+        a = mad(gam, e1, a);
+        b = mad(lm, e1, b);
+        C p2 = mad(pr.Wx.b[ix], ST::ld(rd.Wx, pi, sc.r[1]), pr.Wx.c[ix] * wx);
+        if (own) ST::st(wr.Wx, pi, p2, sc.w[1]);
+        cc = mad(ma, mad(wx, pr.Wx.k[ix], p2), cc);
+        if (ST::scaled && own) {
+            mp[0] = fmaxf(mp[0], fabsf((float)p1));
+            mp[1] = fmaxf(mp[1], fabsf((float)p2));
+        }
+    } else if (iz != 0) {
+        a = mad(gam, ux, a); b = mad(lm, ux, b); cc = mad(ma, wx, cc);
+    }
+    const bool topcell = iz < npml && g.top;
+    if (topcell || iz >= nz - npml) {
+        const long pi = (long)shot * g.zpsi_stride + zpsi_idx(ix, iz, nx, nz, npml, topcell);
+        C p1 = mad(pr.Wz.b[iz], ST::ld(rd.Wz, pi, sc.r[3]), pr.Wz.c[iz] * wz);
+        if (own) ST::st(wr.Wz, pi, p1, sc.w[3]);
+        const C e1 = mad(wz, pr.Wz.k[iz], p1);
+        a = mad(lm, e1, a);
+        b = mad(gam, e1, b);
+        C p2 = mad(pr.Uz.b[iz], ST::ld(rd.Uz, pi, sc.r[2]), pr.Uz.c[iz] * uz);
+        if (own) ST::st(wr.Uz, pi, p2, sc.w[2]);
+        cc = mad(ma, mad(uz, pr.Uz.k[iz], p2), cc);
+        if (ST::scaled && own) {
+            mp[2] = fmaxf(mp[2], fabsf((float)p1));
+            mp[3] = fmaxf(mp[3], fabsf((float)p2));
+        }
+    } else if (iz == 0) {
+        a = mad(C(2.0) * mm, ux, a);
+        b = C(0);
+        cc = C(0);
+    } else {
+        a = mad(lm, wz, a); b = mad(gam, wz, b); cc = mad(ma, uz, cc);
+    }
+}
+
 template <class S, int RZ, int SPB>
 __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStressMat<typename Store<S>::C> m,
                                                        PsvStressPsi<S> ps,
@@ -225,7 +291,7 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
     if (!map_tile<TZ>(w, t)) return;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int x0 = t.x0, z0 = t.z0;
-    const int nx = g.nx, nz = g.nz, px = g.px, npml = g.npml;
+    const int nx = g.nx, nz = g.nz, px = g.px;
     const int gx = x0 + tx;
     const int lz0 = ty * RZ;
     const bool interior = tile_interior(g, x0, z0, TZ);
@@ -289,17 +355,14 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
                 const C uz = dm(pu, LW, c0, c1, c2, c3);
                 const C wx = dp(pw, 1, c0, c1, c2, c3);
                 const C wz = dp(pw, LW, c0, c1, c2, c3);
-                const C gam = lam[r] + C(2.0) * mu[r];
-                txx[r] = mad(gam, ux, txx[r]); tzz[r] = mad(lam[r], ux, tzz[r]); txz[r] = mad(mua[r], wx, txz[r]);
-                txx[r] = mad(lam[r], wz, txx[r]); tzz[r] = mad(gam, wz, tzz[r]); txz[r] = mad(mua[r], uz, txz[r]);
+                psv_stress_interior(ux, uz, wx, wz, lam[r], mu[r], mua[r], txx[r], tzz[r], txz[r]);
             }
         } else {
-            float sp_ux = 1.f, sp_wx = 1.f, sp_uz = 1.f, sp_wz = 1.f;
-            float wp_ux = 1.f, wp_wx = 1.f, wp_uz = 1.f, wp_wz = 1.f;
+            PsiScl4 sc{{1.f, 1.f, 1.f, 1.f}, {1.f, 1.f, 1.f, 1.f}};
             if (ST::scaled) {
-                sp_ux = scl[SC_PU_X]; sp_wx = scl[SC_PW_X]; sp_uz = scl[SC_PU_Z]; sp_wz = scl[SC_PW_Z];
-                wp_ux = 1.f / scl[kNextScale + SC_PU_X]; wp_wx = 1.f / scl[kNextScale + SC_PW_X];
-                wp_uz = 1.f / scl[kNextScale + SC_PU_Z]; wp_wz = 1.f / scl[kNextScale + SC_PW_Z];
+                sc.r[0] = scl[SC_PU_X]; sc.r[1] = scl[SC_PW_X]; sc.r[2] = scl[SC_PU_Z]; sc.r[3] = scl[SC_PW_Z];
+                sc.w[0] = 1.f / scl[kNextScale + SC_PU_X]; sc.w[1] = 1.f / scl[kNextScale + SC_PW_X];
+                sc.w[2] = 1.f / scl[kNextScale + SC_PU_Z]; sc.w[3] = 1.f / scl[kNextScale + SC_PW_Z];
             }
 #pragma unroll
             for (int r = 0; r < RZ; r++) {
@@ -314,49 +377,8 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
                 const C uz = dm(pu, LW, c0, c1, c2, c3);
                 const C wx = dp(pw, 1, c0, c1, c2, c3);
                 const C wz = dp(pw, LW, c0, c1, c2, c3);
-                const C lm = lam[r], mm = mu[r], ma = mua[r];
-                const C gam = lm + C(2.0) * mm;
-                C a = txx[r], b = tzz[r], cc = txz[r];
-                if (ix < npml || ix >= nx - npml) {
-                    const long pi = (long)shot * g.xpsi_stride + xpsi_idx(ix, iz, nx, npml);
-                    C p1 = mad(pr.Ux.b[ix], ST::ld(ps.Ux, pi, sp_ux), pr.Ux.c[ix] * ux);
-                    ST::st(ps.Ux, pi, p1, wp_ux);
-                    const C e1 = mad(ux, pr.Ux.k[ix], p1);
-                    a = mad(gam, e1, a);
-                    b = mad(lm, e1, b);
-                    C p2 = mad(pr.Wx.b[ix], ST::ld(ps.Wx, pi, sp_wx), pr.Wx.c[ix] * wx);
-                    ST::st(ps.Wx, pi, p2, wp_wx);
-                    cc = mad(ma, mad(wx, pr.Wx.k[ix], p2), cc);
-                    if (ST::scaled) {
-                        mp[0] = fmaxf(mp[0], fabsf((float)p1));
-                        mp[1] = fmaxf(mp[1], fabsf((float)p2));
-                    }
-                } else if (iz != 0) {
-                    a = mad(gam, ux, a); b = mad(lm, ux, b); cc = mad(ma, wx, cc);
-                }
-                const bool topcell = iz < npml && g.top;
-                if (topcell || iz >= nz - npml) {
-                    const long pi = (long)shot * g.zpsi_stride + zpsi_idx(ix, iz, nx, nz, npml, topcell);
-                    C p1 = mad(pr.Wz.b[iz], ST::ld(ps.Wz, pi, sp_wz), pr.Wz.c[iz] * wz);
-                    ST::st(ps.Wz, pi, p1, wp_wz);
-                    const C e1 = mad(wz, pr.Wz.k[iz], p1);
-                    a = mad(lm, e1, a);
-                    b = mad(gam, e1, b);
-                    C p2 = mad(pr.Uz.b[iz], ST::ld(ps.Uz, pi, sp_uz), pr.Uz.c[iz] * uz);
-                    ST::st(ps.Uz, pi, p2, wp_uz);
-                    cc = mad(ma, mad(uz, pr.Uz.k[iz], p2), cc);
-                    if (ST::scaled) {
-                        mp[2] = fmaxf(mp[2], fabsf((float)p1));
-                        mp[3] = fmaxf(mp[3], fabsf((float)p2));
-                    }
-                } else if (iz == 0) {
-                    a = mad(C(2.0) * mm, ux, a);
-                    b = C(0);
-                    cc = C(0);
-                } else {
-                    a = mad(lm, wz, a); b = mad(gam, wz, b); cc = mad(ma, uz, cc);
-                }
-                txx[r] = a; tzz[r] = b; txz[r] = cc;
+                psv_stress_exact<S, C>(g, ix, iz, shot, ux, uz, wx, wz, lam[r], mu[r], mua[r], txx[r], tzz[r],
+                                       txz[r], ps, ps, true, pr, sc, mp);
             }
         }
         // fused source injection: the owner of a tap cell adds it after the
@@ -443,6 +465,60 @@ template <class C> struct PsvVelProf {
     Prof<C> Txx_x, Txz_x, Txz_z, Tzz_z;
 };
 
+template <class C>
+__device__ __forceinline__ void psv_velocity_interior(C dxTxx, C dzTzz, C dxTxz, C dzTxz, C bu, C bw, C &u, C &w) {
+    u = mad(bu, dxTxx, u); w = mad(bw, dxTxz, w);
+    u = mad(bu, dzTxz, u); w = mad(bw, dzTzz, w);
+}
+
+// exact path of one velocity cell; sc: Txx_x Txz_x Txz_z Tzz_z; mp: the same
+template <class S, class C>
+__device__ __forceinline__ void psv_velocity_exact(const Grid &g, int ix, int iz, int shot, C dxTxx, C dzTzz,
+                                                   C dxTxz, C dzTxz, C b_u, C b_w, C &u, C &ww,
+                                                   const PsvVelPsi<S> &rd, const PsvVelPsi<S> &wr, bool own,
+                                                   const PsvVelProf<C> &pr, const PsiScl4 &sc, float (&mp)[4]) {
+    using ST = Store<S>;
+    const int nx = g.nx, nz = g.nz, npml = g.npml;
+    if (ix < npml || ix >= nx - npml) {
+        const long pi = (long)shot * g.xpsi_stride + xpsi_idx(ix, iz, nx, npml);
+        C p1 = mad(pr.Txx_x.b[ix], ST::ld(rd.Txx_x, pi, sc.r[0]), pr.Txx_x.c[ix] * dxTxx);
+        if (own) ST::st(wr.Txx_x, pi, p1, sc.w[0]);
+        u = mad(b_u, mad(dxTxx, pr.Txx_x.k[ix], p1), u);
+        C p2 = mad(pr.Txz_x.b[ix], ST::ld(rd.Txz_x, pi, sc.r[1]), pr.Txz_x.c[ix] * dxTxz);
+        if (own) ST::st(wr.Txz_x, pi, p2, sc.w[1]);
+        ww = mad(b_w, mad(dxTxz, pr.Txz_x.k[ix], p2), ww);
+        if (ST::scaled && own) {
+            mp[0] = fmaxf(mp[0], fabsf((float)p1));
+            mp[1] = fmaxf(mp[1], fabsf((float)p2));
+        }
+    } else if (iz != 0) {
+        u = mad(b_u, dxTxx, u);
+        ww = mad(b_w, dxTxz, ww);
+    }
+    const bool topcell = iz < npml && g.top;
+    if (topcell || iz >= nz - npml) {
+        const long pi = (long)shot * g.zpsi_stride + zpsi_idx(ix, iz, nx, nz, npml, topcell);
+        C p1 = mad(pr.Txz_z.b[iz], ST::ld(rd.Txz_z, pi, sc.r[2]), pr.Txz_z.c[iz] * dzTxz);
+        C p2 = mad(pr.Tzz_z.b[iz], ST::ld(rd.Tzz_z, pi, sc.r[3]), pr.Tzz_z.c[iz] * dzTzz);
+        if (own) {
+            ST::st(wr.Txz_z, pi, p1, sc.w[2]);
+            ST::st(wr.Tzz_z, pi, p2, sc.w[3]);
+        }
+        u = mad(b_u, mad(dzTxz, pr.Txz_z.k[iz], p1), u);
+        ww = mad(b_w, mad(dzTzz, pr.Tzz_z.k[iz], p2), ww);
+        if (ST::scaled && own) {
+            mp[2] = fmaxf(mp[2], fabsf((float)p1));
+            mp[3] = fmaxf(mp[3], fabsf((float)p2));
+        }
+    } else if (iz == 0) {
+        u = mad(b_u, dxTxx + dzTxz, u);
+        ww = C(0);
+    } else {
+        u = mad(b_u, dzTxz, u);
+        ww = mad(b_w, dzTzz, ww);
+    }
+}
+
 template <class S, int RZ, int SPB>
 __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelMat<typename Store<S>::C> m,
                                                          PsvVelPsi<S> ps,
@@ -461,7 +537,7 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
     if (!map_tile<TZ>(w, t)) return;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int x0 = t.x0, z0 = t.z0;
-    const int nx = g.nx, nz = g.nz, px = g.px, npml = g.npml;
+    const int nx = g.nx, nz = g.nz, px = g.px;
     const int gx = x0 + tx;
     const int lz0 = ty * RZ;
     const bool interior = tile_interior(g, x0, z0, TZ);
@@ -521,18 +597,14 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
                 const C dzTzz = dm(&sZZ[o], LW, c0, c1, c2, c3);
                 const C dxTxz = dm(&sXZ[o], 1, c0, c1, c2, c3);
                 const C dzTxz = dp(&sXZ[o], LW, c0, c1, c2, c3);
-                C u = nv[0][r], ww = nv[1][r];
-                u = mad(bu[r], dxTxx, u); ww = mad(bw[r], dxTxz, ww);
-                u = mad(bu[r], dzTxz, u); ww = mad(bw[r], dzTzz, ww);
-                nv[0][r] = u; nv[1][r] = ww;
+                psv_velocity_interior(dxTxx, dzTzz, dxTxz, dzTxz, bu[r], bw[r], nv[0][r], nv[1][r]);
             }
         } else {
-            float sp_xx = 1.f, sp_xzx = 1.f, sp_xzz = 1.f, sp_zz = 1.f;
-            float wp_xx = 1.f, wp_xzx = 1.f, wp_xzz = 1.f, wp_zz = 1.f;
+            PsiScl4 sc{{1.f, 1.f, 1.f, 1.f}, {1.f, 1.f, 1.f, 1.f}};
             if (ST::scaled) {
-                sp_xx = scl[SC_PTXX_X]; sp_xzx = scl[SC_PTXZ_X]; sp_xzz = scl[SC_PTXZ_Z]; sp_zz = scl[SC_PTZZ_Z];
-                wp_xx = 1.f / scl[kNextScale + SC_PTXX_X]; wp_xzx = 1.f / scl[kNextScale + SC_PTXZ_X];
-                wp_xzz = 1.f / scl[kNextScale + SC_PTXZ_Z]; wp_zz = 1.f / scl[kNextScale + SC_PTZZ_Z];
+                sc.r[0] = scl[SC_PTXX_X]; sc.r[1] = scl[SC_PTXZ_X]; sc.r[2] = scl[SC_PTXZ_Z]; sc.r[3] = scl[SC_PTZZ_Z];
+                sc.w[0] = 1.f / scl[kNextScale + SC_PTXX_X]; sc.w[1] = 1.f / scl[kNextScale + SC_PTXZ_X];
+                sc.w[2] = 1.f / scl[kNextScale + SC_PTXZ_Z]; sc.w[3] = 1.f / scl[kNextScale + SC_PTZZ_Z];
             }
 #pragma unroll
             for (int r = 0; r < RZ; r++) {
@@ -546,45 +618,8 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
                 const C dzTzz = dm(&sZZ[o], LW, c0, c1, c2, c3);
                 const C dxTxz = dm(&sXZ[o], 1, c0, c1, c2, c3);
                 const C dzTxz = dp(&sXZ[o], LW, c0, c1, c2, c3);
-                const C b_u = bu[r], b_w = bw[r];
-                C u = nv[0][r], ww = nv[1][r];
-                if (ix < npml || ix >= nx - npml) {
-                    const long pi = (long)shot * g.xpsi_stride + xpsi_idx(ix, iz, nx, npml);
-                    C p1 = mad(pr.Txx_x.b[ix], ST::ld(ps.Txx_x, pi, sp_xx), pr.Txx_x.c[ix] * dxTxx);
-                    ST::st(ps.Txx_x, pi, p1, wp_xx);
-                    u = mad(b_u, mad(dxTxx, pr.Txx_x.k[ix], p1), u);
-                    C p2 = mad(pr.Txz_x.b[ix], ST::ld(ps.Txz_x, pi, sp_xzx), pr.Txz_x.c[ix] * dxTxz);
-                    ST::st(ps.Txz_x, pi, p2, wp_xzx);
-                    ww = mad(b_w, mad(dxTxz, pr.Txz_x.k[ix], p2), ww);
-                    if (ST::scaled) {
-                        mp[0] = fmaxf(mp[0], fabsf((float)p1));
-                        mp[1] = fmaxf(mp[1], fabsf((float)p2));
-                    }
-                } else if (iz != 0) {
-                    u = mad(b_u, dxTxx, u);
-                    ww = mad(b_w, dxTxz, ww);
-                }
-                const bool topcell = iz < npml && g.top;
-                if (topcell || iz >= nz - npml) {
-                    const long pi = (long)shot * g.zpsi_stride + zpsi_idx(ix, iz, nx, nz, npml, topcell);
-                    C p1 = mad(pr.Txz_z.b[iz], ST::ld(ps.Txz_z, pi, sp_xzz), pr.Txz_z.c[iz] * dzTxz);
-                    C p2 = mad(pr.Tzz_z.b[iz], ST::ld(ps.Tzz_z, pi, sp_zz), pr.Tzz_z.c[iz] * dzTzz);
-                    ST::st(ps.Txz_z, pi, p1, wp_xzz);
-                    ST::st(ps.Tzz_z, pi, p2, wp_zz);
-                    u = mad(b_u, mad(dzTxz, pr.Txz_z.k[iz], p1), u);
-                    ww = mad(b_w, mad(dzTzz, pr.Tzz_z.k[iz], p2), ww);
-                    if (ST::scaled) {
-                        mp[2] = fmaxf(mp[2], fabsf((float)p1));
-                        mp[3] = fmaxf(mp[3], fabsf((float)p2));
-                    }
-                } else if (iz == 0) {
-                    u = mad(b_u, dxTxx + dzTxz, u);
-                    ww = C(0);
-                } else {
-                    u = mad(b_u, dzTxz, u);
-                    ww = mad(b_w, dzTzz, ww);
-                }
-                nv[0][r] = u; nv[1][r] = ww;
+                psv_velocity_exact<S, C>(g, ix, iz, shot, dxTxx, dzTzz, dxTxz, dzTxz, bu[r], bw[r], nv[0][r],
+                                         nv[1][r], ps, ps, true, pr, sc, mp);
             }
         }
 #pragma unroll
@@ -1041,6 +1076,14 @@ __global__ void k_commit_scales(float *scl, unsigned mask) {
 // advance the step clock after a captured chunk of steps
 __global__ void k_tick(int *clock, int steps) {
     if (threadIdx.x == 0) clock[0] += steps;
+}
+
+// rewrite a small device pointer table in stream order
+struct PtrTable5 {
+    void *p[5];
+};
+__global__ void k_set_ptrs(void **tab, PtrTable5 v, int n) {
+    if (threadIdx.x < n) tab[threadIdx.x] = v.p[threadIdx.x];
 }
 
 // set the step clock in stream order (no host round trip)

@@ -162,6 +162,12 @@ int fd23_box_get_residual(fd23_ctx *ctx, int shot, void *out);
 int fd23_set_tile(fd23_ctx *ctx, int tile);
 int fd23_get_tile(const fd23_ctx *ctx);
 int fd23_num_tiles(void);
+/* shape of a tile index: two-pass tiles have rows-per-thread rz (tile
+ * height 4*rz) and spb shots per workgroup; fused_rows > 0 marks the
+ * fused P-SV step (stress and velocity in one pass over two buffer sets,
+ * segments of fused_rows rows); it applies to fd23_run of P-SV contexts
+ * without a box, the staged entry points keep the two-pass kernels      */
+int fd23_tile_shape(int tile, int *rz, int *spb, int *fused_rows);
 int fd23_autotune(fd23_ctx *ctx, int steps, double *best_ms);
 /* FP16/BF16 storage: per-array power-of-two scales (stored = value/scale);
  * P-SV order U W Txx Tzz Txz then psi Txx_x Txz_x U_x W_x Tzz_z Txz_z U_z

@@ -180,10 +180,74 @@ def test_all_tiles_bitwise_equal():
             ctx.tile = t
             ctx.run(0, p.nt, 1)
             st = ctx.get_state(0) + ctx.get_state(1)
+            st += ctx.seismograms(1, 0, p.nt).tobytes() + ctx.seismograms(0, 1, p.nt).tobytes()
         if ref is None:
             ref = st
         _print("tile", t, st == ref)
         assert st == ref
+
+
+FUSED = (10, 11, 12)  # fused P-SV step, segment heights 32, 64, 128
+
+
+@pytest.mark.parametrize("nx,nz,npml,top,kind,storage", [
+    (173, 150, 12, False, _lib.SRC_LINE, _lib.STORE_F32),
+    (130, 97, 10, True, _lib.SRC_P, _lib.STORE_F32),
+    (121, 140, 9, False, _lib.SRC_LINE, _lib.STORE_F64),
+])
+def test_fused_step_bitwise_equals_two_pass(nx, nz, npml, top, kind, storage):
+    """The fused step runs the two-pass kernels' per-cell code: states and
+    seismograms must be identical, across buffer swaps (odd step counts),
+    split runs, and staged steps taken after fused ones."""
+    _print("case: fused", nx, nz, npml, top, kind, storage)
+    srcs = [(nx // 2, 7), (5, 3), (nx - 6, nz // 2)]
+    p = Problem(_lib.WAVE_PSV, nx, nz, npml, 10.0, 7e-4, 75, seed=21, topabsorb=top, kind=kind,
+                sources=srcs, rec_z=npml + 3 if top else 2, f64=storage == _lib.STORE_F64)
+    p.rx = np.concatenate([p.rx, np.array([0, nx - 1, 55, 56, 111, 112], np.int32)])
+    p.rz = np.concatenate([p.rz, np.array([nz - 1, 0, nz // 2, nz // 2, 60, 61], np.int32)])
+
+    def run(tile, staged_tail):
+        with p.context(storage=storage) as ctx:
+            ctx.tile = tile
+            ctx.run(0, 37, 1)
+            ctx.run(37, 28, 1)
+            for it in range(65, 65 + staged_tail):
+                ctx.step_stress()
+                ctx.inject(it)
+                ctx.step_velocity()
+                ctx.record(it)
+            if staged_tail < 10:
+                ctx.run(65 + staged_tail, 10 - staged_tail, 1)
+            ctx.sync()
+            st = b"".join(ctx.get_state(s) for s in range(len(srcs)))
+            seis = [ctx.seismograms(s, c, p.nt) for s in range(len(srcs)) for c in range(2)]
+        return st, seis
+
+    ref_st, ref_seis = run(0, 3)
+    assert any(np.abs(x).max() > 0 for x in ref_seis)
+    for t in FUSED:
+        st, seis = run(t, 3)
+        _print("fused tile", t, st == ref_st)
+        assert st == ref_st
+        for a, b in zip(seis, ref_seis):
+            assert np.array_equal(a, b)
+
+
+def test_fused_step_f16_close_to_two_pass():
+    """FP16 storage: the fused step uses the unrounded new stresses in the
+    velocity update, so it differs from the two-pass path at the level of
+    one FP16 rounding; both stay within the FP16 tolerance of each other."""
+    _print("case: fused f16")
+    p = Problem(_lib.WAVE_PSV, 200, 160, 16, 10.0, 7e-4, 300, seed=23, sources=[(100, 8), (60, 12)], rec_z=5)
+    out = {}
+    for t in (0, 11):
+        with p.context(storage=_lib.STORE_F16) as ctx:
+            ctx.tile = t
+            ctx.run(0, p.nt, 1)
+            out[t] = np.stack([ctx.seismograms(s, c, p.nt) for s in range(2) for c in range(2)])
+    e = rel(out[11], out[0])
+    _print("fused vs two-pass f16", e)
+    assert np.isfinite(out[11]).all() and e < 1e-3
 
 
 def test_batch_equals_single_shots():
