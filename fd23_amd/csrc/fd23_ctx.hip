@@ -49,10 +49,11 @@ extern "C" void fd23_set_error_(const char *msg) { g_err = msg; }
 // per workgroup SPB) and fused P-SV steps (segment height FH rows)
 // ---------------------------------------------------------------------------
 struct TileCfg {
-    int rz, spb, fh;
+    int rz, spb, fh, fr; // fused: segment height fh, rows per block fr
 };
-static const TileCfg kTiles[] = {{4, 1, 0}, {2, 1, 0}, {8, 1, 0}, {4, 2, 0}, {4, 4, 0}, {4, 8, 0}, {8, 2, 0},
-                                 {8, 4, 0}, {2, 4, 0}, {2, 8, 0}, {0, 1, 32}, {0, 1, 64}, {0, 1, 128}};
+static const TileCfg kTiles[] = {{4, 1, 0, 0},  {2, 1, 0, 0},  {8, 1, 0, 0},  {4, 2, 0, 0},   {4, 4, 0, 0},
+                                 {4, 8, 0, 0},  {8, 2, 0, 0},  {8, 4, 0, 0},  {2, 4, 0, 0},   {2, 8, 0, 0},
+                                 {0, 1, 32, 4}, {0, 1, 64, 4}, {0, 1, 128, 4}, {0, 1, 64, 2}};
 static const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 extern "C" int fd23_num_tiles(void) { return kNumTiles; }
@@ -862,7 +863,7 @@ static void dispatch_pass(fd23_ctx *c, bool stress, const Events &ev) {
     }
 }
 
-template <class S, int H> static void launch_fused_t(fd23_ctx *c, int k, int itrecord, bool record) {
+template <class S, int H, int R> static void launch_fused_t(fd23_ctx *c, int k, int itrecord, bool record) {
     using C = typename Store<S>::C;
     const Grid g = make_grid(c);
     FusedWork w;
@@ -895,16 +896,16 @@ template <class S, int H> static void launch_fused_t(fd23_ctx *c, int k, int itr
     PsvVelPsi<S> vpb{(S *)c->alt_psi[0], (S *)c->alt_psi[1], (S *)c->alt_psi[5], (S *)c->alt_psi[4]};
     PsvStressProf<C> spr{prof_of<C>(c, 2), prof_of<C>(c, 3), prof_of<C>(c, 6), prof_of<C>(c, 7)};
     PsvVelProf<C> vpr{prof_of<C>(c, 0), prof_of<C>(c, 1), prof_of<C>(c, 5), prof_of<C>(c, 4)};
-    hipLaunchKernelGGL((k_psv_fused<S, H>), grid, dim3(FW), 0, c->stream, fields(c->fields), fields(c->alt_fields),
+    hipLaunchKernelGGL((k_psv_fused<S, H, R>), grid, dim3(FW), 0, c->stream, fields(c->fields), fields(c->alt_fields),
                        m, spa, spb, vpa, vpb, spr, vpr, g, w, ev, c->scl, c->amax);
 }
 
 template <class S> static void launch_fused_h(fd23_ctx *c, int k, int itrecord, bool record) {
-    switch (kTiles[c->tile].fh) {
-    case 32: launch_fused_t<S, 32>(c, k, itrecord, record); break;
-    case 64: launch_fused_t<S, 64>(c, k, itrecord, record); break;
-    default: launch_fused_t<S, 128>(c, k, itrecord, record); break;
-    }
+    const TileCfg t = kTiles[c->tile];
+    if (t.fr == 2) launch_fused_t<S, 64, 2>(c, k, itrecord, record);
+    else if (t.fh == 32) launch_fused_t<S, 32, 4>(c, k, itrecord, record);
+    else if (t.fh == 64) launch_fused_t<S, 64, 4>(c, k, itrecord, record);
+    else launch_fused_t<S, 128, 4>(c, k, itrecord, record);
 }
 
 // one fused step: reads the current set, writes the other, then swaps
@@ -978,7 +979,11 @@ static bool scaled_storage(const fd23_ctx *c) {
     return c->d.storage == FD23_STORE_F16 || c->d.storage == FD23_STORE_BF16;
 }
 static const int kScaleEvery = 8;            // steps between scale checks
-static const int kScaleLo = 7, kScaleHi = 13, kScaleTarget = 10; // stored max band (log2)
+// stored max band (log2): grow the scale when the stored maximum passes
+// 2^11 (FP16 overflows at 2^16, so a field may grow 32x between checks);
+// shrink it only when the maximum has decayed below 2^-4 (FP16 keeps its
+// full relative precision down to 2^-14); a change puts the maximum at 2^8
+static const int kScaleLo = -4, kScaleHi = 11, kScaleTarget = 8;
 
 static float pow2_scale(double est) {
     if (!(est > 0) || !std::isfinite(est)) return 1.0f;

@@ -7,13 +7,13 @@
 //  - lane l holds column xc = x0 - 4 + l; lanes 4..59 own columns
 //    x0 .. x0+55 (56 per strip), lanes 0-3 and 60-63 recompute the
 //    neighbours' stresses that the owned velocities need;
-//  - the work item covers velocity rows [zv0, zv0+H) of one shot; at row
-//    step j it computes the stresses of row rs = zv0-4+j and the velocities
-//    of row rv = rs-4, which need new stresses down to rs;
-//  - z-derivatives come from per-lane register queues (old U rows rs-4..rs+3,
-//    old W rows rs-4..rs+4, new Tzz rows rs-8..rs, new Txz rows rs-7..rs,
-//    new Txx rows rs-4..rs), x-derivatives from one LDS row per field;
-//  - the next row's loads are issued before the current row is computed.
+//  - the work item covers velocity rows [zv0, zv0+H) of one shot; in blocks
+//    of R rows it computes the stresses of rows rs0..rs0+R-1 and then the
+//    velocities of the rows 4 above, which need new stresses down to rs0+R-1;
+//  - z-derivatives come from per-lane register queues of rows, x-derivatives
+//    from LDS rows;
+//  - all loads of the next block are issued before the current block is
+//    computed, so each wavefront keeps R rows of every array in flight.
 // Every cell runs the same per-cell code as the two-pass kernels
 // (psv_stress_* / psv_velocity_*), so results are bitwise equal to them.
 #pragma once
@@ -67,7 +67,7 @@ __device__ __forceinline__ C dp8(C fm3, C fm2, C fm1, C f0, C f1, C f2, C f3, C 
     return mad(-c3, f4 - fm3, r);
 }
 
-template <class S, int H>
+template <class S, int H, int R>
 __global__ __launch_bounds__(FW) void k_psv_fused(PsvFields<S> fa, PsvFields<S> fb, PsvMat5<typename Store<S>::C> m,
                                                   PsvStressPsi<S> spa, PsvStressPsi<S> spb, PsvVelPsi<S> vpa,
                                                   PsvVelPsi<S> vpb, PsvStressProf<typename Store<S>::C> spr,
@@ -76,8 +76,9 @@ __global__ __launch_bounds__(FW) void k_psv_fused(PsvFields<S> fa, PsvFields<S> 
                                                   unsigned *__restrict__ amax) {
     using C = typename Store<S>::C;
     using ST = Store<S>;
+    static_assert(H % R == 0, "segment height must be a multiple of the row block");
     constexpr int LWF = FW + 2 * HALO;
-    __shared__ C lU[LWF], lW[LWF], lXX[FW], lXZ[FW], pU[FW], pW[FW];
+    __shared__ C lU[R][LWF], lW[R][LWF], lXX[R][FW], lXZ[R][FW], pU[FW], pW[FW];
 
     const int qd = gridDim.x >> 3; // grid padded to a multiple of 8
     const int L = (int)(blockIdx.x & 7) * qd + (int)(blockIdx.x >> 3);
@@ -96,6 +97,7 @@ __global__ __launch_bounds__(FW) void k_psv_fused(PsvFields<S> fa, PsvFields<S> 
     const int zv0 = seg * H, zv1 = min(zv0 + H, nz);
     const long base = (long)shot * g.sstride;
     const int lo = max(g.npml, 4);
+    const int ztop = g.top ? lo : 4;
     // strip columns x0-4 .. x0+59 all at full order with no C-PML
     const bool xint = x0 - HALO >= lo && x0 + FO + HALO <= nx - lo;
     const int it = ev.clock[0] + ev.k;
@@ -136,131 +138,167 @@ __global__ __launch_bounds__(FW) void k_psv_fused(PsvFields<S> fa, PsvFields<S> 
     // the 8 extra LDS columns: lanes 0-3 -> x0-8..x0-5, lanes 4-7 -> x0+60..x0+63
     const int xcol = lane < 4 ? x0 - 2 * HALO + lane : x0 + FO + HALO + (lane - 4);
 
-    C qU[8], qW[9], qZZ[9], qXZ[8], qXX[5];
+    // Row block b: stresses of rows rs0 .. rs0+R-1 (rs0 = zv0-4+R*b) and
+    // velocities of rows rs0-4 .. rs0+R-5.  Queues, lowest row first:
+    //   qU  old U    rows rs0-4 .. rs0+R+2     qW  old W    rows rs0-4 .. rs0+R+3
+    //   qZZ new Tzz  rows rs0-8 .. rs0+R-1     qXZ new Txz  rows rs0-8 .. rs0+R-1
+    //   qXX new Txx  rows rs0-4 .. rs0+R-1
+    C qU[R + 7], qW[R + 8], qZZ[R + 8], qXZ[R + 8], qXX[R + 4];
+    int rs0 = zv0 - HALO;
 #pragma unroll
-    for (int i = 0; i < 8; i++) qU[i] = ldrow(fa.U, zv0 - 8 + i, xc, sc_u);
+    for (int i = 0; i < R + 7; i++) qU[i] = ldrow(fa.U, rs0 - 4 + i, xc, sc_u);
 #pragma unroll
-    for (int i = 0; i < 9; i++) qW[i] = ldrow(fa.W, zv0 - 8 + i, xc, sc_w);
+    for (int i = 0; i < R + 8; i++) qW[i] = ldrow(fa.W, rs0 - 4 + i, xc, sc_w);
 #pragma unroll
-    for (int i = 0; i < 9; i++) qZZ[i] = C(0);
+    for (int i = 0; i < R + 8; i++) qZZ[i] = qXZ[i] = C(0);
 #pragma unroll
-    for (int i = 0; i < 8; i++) qXZ[i] = C(0);
-#pragma unroll
-    for (int i = 0; i < 5; i++) qXX[i] = C(0);
+    for (int i = 0; i < R + 4; i++) qXX[i] = C(0);
 
-    // operands of row step j (prefetched one step ahead)
-    int rs = zv0 - HALO;
-    C hU = lane < 8 ? ldrow(fa.U, rs, xcol, sc_u) : C(0);
-    C hW = lane < 8 ? ldrow(fa.W, rs, xcol, sc_w) : C(0);
-    C oxx = ldrow(fa.Txx, rs, xc, sc_xx), ozz = ldrow(fa.Tzz, rs, xc, sc_zz), oxz = ldrow(fa.Txz, rs, xc, sc_xz);
-    C lam = ldmat(m.LAM, rs), mu = ldmat(m.MU, rs), mua = ldmat(m.MUA, rs);
-    C bu = ldmat(m.BU, rs - HALO), bw = ldmat(m.BW, rs - HALO);
-    C nU = ldrow(fa.U, rs + HALO, xc, sc_u), nW = ldrow(fa.W, rs + HALO + 1, xc, sc_w);
+    // operands of the current block
+    C hU[R], hW[R], oxx[R], ozz[R], oxz[R], lam[R], mu[R], mua[R], bu[R], bw[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const int rs = rs0 + k, rv = rs - HALO;
+        hU[k] = lane < 8 ? ldrow(fa.U, rs, xcol, sc_u) : C(0);
+        hW[k] = lane < 8 ? ldrow(fa.W, rs, xcol, sc_w) : C(0);
+        oxx[k] = ldrow(fa.Txx, rs, xc, sc_xx);
+        ozz[k] = ldrow(fa.Tzz, rs, xc, sc_zz);
+        oxz[k] = ldrow(fa.Txz, rs, xc, sc_xz);
+        lam[k] = ldmat(m.LAM, rs);
+        mu[k] = ldmat(m.MU, rs);
+        mua[k] = ldmat(m.MUA, rs);
+        bu[k] = ldmat(m.BU, rv);
+        bw[k] = ldmat(m.BW, rv);
+    }
 
-    for (int j = 0; j < H + 2 * HALO; j++, rs++) {
-        const int rv = rs - HALO;
-        // ---- prefetch the next row step's operands
-        const int rn = rs + 1;
-        const C hU_n = lane < 8 ? ldrow(fa.U, rn, xcol, sc_u) : C(0);
-        const C hW_n = lane < 8 ? ldrow(fa.W, rn, xcol, sc_w) : C(0);
-        const C oxx_n = ldrow(fa.Txx, rn, xc, sc_xx), ozz_n = ldrow(fa.Tzz, rn, xc, sc_zz);
-        const C oxz_n = ldrow(fa.Txz, rn, xc, sc_xz);
-        const C lam_n = ldmat(m.LAM, rn), mu_n = ldmat(m.MU, rn), mua_n = ldmat(m.MUA, rn);
-        const C bu_n = ldmat(m.BU, rn - HALO), bw_n = ldmat(m.BW, rn - HALO);
-        const C nU_n = ldrow(fa.U, rn + HALO, xc, sc_u), nW_n = ldrow(fa.W, rn + HALO + 1, xc, sc_w);
+    constexpr int NB = (H + 2 * HALO) / R;
+#pragma unroll 1
+    for (int b = 0; b < NB; b++, rs0 += R) {
+        const int rv0 = rs0 - HALO;
+        // ---- prefetch the next block's operands and queue rows
+        C nU[R], nW[R], hU_n[R], hW_n[R], oxx_n[R], ozz_n[R], oxz_n[R], lam_n[R], mu_n[R], mua_n[R], bu_n[R], bw_n[R];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int rs = rs0 + R + k, rv = rs - HALO;
+            nU[k] = ldrow(fa.U, rs + 3, xc, sc_u);
+            nW[k] = ldrow(fa.W, rs + 4, xc, sc_w);
+            hU_n[k] = lane < 8 ? ldrow(fa.U, rs, xcol, sc_u) : C(0);
+            hW_n[k] = lane < 8 ? ldrow(fa.W, rs, xcol, sc_w) : C(0);
+            oxx_n[k] = ldrow(fa.Txx, rs, xc, sc_xx);
+            ozz_n[k] = ldrow(fa.Tzz, rs, xc, sc_zz);
+            oxz_n[k] = ldrow(fa.Txz, rs, xc, sc_xz);
+            lam_n[k] = ldmat(m.LAM, rs);
+            mu_n[k] = ldmat(m.MU, rs);
+            mua_n[k] = ldmat(m.MUA, rs);
+            bu_n[k] = ldmat(m.BU, rv);
+            bw_n[k] = ldmat(m.BW, rv);
+        }
 
-        // ---- stage the U and W rows rs for the x-derivatives
-        lU[lane + HALO] = qU[4];
-        lW[lane + HALO] = qW[4];
-        if (lane < 8) {
-            const int o = lane < 4 ? lane : FW + lane;
-            lU[o] = hU;
-            lW[o] = hW;
+        // ---- stage the U and W rows rs0..rs0+R-1 for the x-derivatives
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            lU[k][lane + HALO] = qU[k + 4];
+            lW[k][lane + HALO] = qW[k + 4];
+            if (lane < 8) {
+                const int o = lane < 4 ? lane : FW + lane;
+                lU[k][o] = hU[k];
+                lW[k][o] = hW[k];
+            }
         }
         __syncthreads();
 
-        // ---- stresses of row rs
-        C a = oxx, b = ozz, cc = oxz;
-        if (rs >= 0 && rs < nz && xin) {
-            const bool rint = rs >= (g.top ? lo : 4) && rs < nz - lo;
-            if (xint && rint) {
-                const C c0 = C(1225. / 1024.), c1 = C(245. / 3072.), c2 = C(49. / 5120.), c3 = C(5. / 7168.);
-                const C ux = dm(&lU[lane + HALO], 1, c0, c1, c2, c3);
-                const C wx = dp(&lW[lane + HALO], 1, c0, c1, c2, c3);
-                const C uz = dm8(qU[0], qU[1], qU[2], qU[3], qU[4], qU[5], qU[6], qU[7], c0, c1, c2, c3);
-                const C wz = dp8(qW[1], qW[2], qW[3], qW[4], qW[5], qW[6], qW[7], qW[8], c0, c1, c2, c3);
-                psv_stress_interior(ux, uz, wx, wz, lam, mu, mua, a, b, cc);
-            } else {
-                C c0, c1, c2, c3;
-                Coef<C>::get(order_at(xc, rs, nx, nz), c0, c1, c2, c3);
-                const C ux = dm(&lU[lane + HALO], 1, c0, c1, c2, c3);
-                const C wx = dp(&lW[lane + HALO], 1, c0, c1, c2, c3);
-                const C uz = dm8(qU[0], qU[1], qU[2], qU[3], qU[4], qU[5], qU[6], qU[7], c0, c1, c2, c3);
-                const C wz = dp8(qW[1], qW[2], qW[3], qW[4], qW[5], qW[6], qW[7], qW[8], c0, c1, c2, c3);
-                const bool own = own_x && rs >= zv0 && rs < zv1;
-                psv_stress_exact<S, C>(g, xc, rs, shot, ux, uz, wx, wz, lam, mu, mua, a, b, cc, spa, spb, own,
-                                       spr, ssc, smp);
-            }
-            // source taps after the cell's update, in the reference's order
-            if (src_on) {
-                for (int e = t0; e < t1; e++) {
-                    const TapF tp = ev.taps[e];
-                    if (tp.row != rs || tp.lane != lane) continue;
-                    if (tp.field == 2) a = apply_tap(a, tp, sv);
-                    else if (tp.field == 3) b = apply_tap(b, tp, sv);
-                    else cc = apply_tap(cc, tp, sv);
-                }
-            }
-            if (own_x && rs >= zv0 && rs < zv1) {
-                const long q = base + (long)rs * px + xc;
-                ST::st(fb.Txx, q, a, wi_xx);
-                ST::st(fb.Tzz, q, b, wi_zz);
-                ST::st(fb.Txz, q, cc, wi_xz);
-                if (ST::scaled) {
-                    mx[2] = fmaxf(mx[2], fabsf((float)a));
-                    mx[3] = fmaxf(mx[3], fabsf((float)b));
-                    mx[4] = fmaxf(mx[4], fabsf((float)cc));
-                }
-            }
-        } else {
-            a = b = cc = C(0); // outside the domain the new stresses are zero
-        }
+        // ---- stresses of rows rs0 .. rs0+R-1
 #pragma unroll
-        for (int i = 0; i < 8; i++) qZZ[i] = qZZ[i + 1];
-        qZZ[8] = b;
-#pragma unroll
-        for (int i = 0; i < 7; i++) qXZ[i] = qXZ[i + 1];
-        qXZ[7] = cc;
-#pragma unroll
-        for (int i = 0; i < 4; i++) qXX[i] = qXX[i + 1];
-        qXX[4] = a;
-
-        // ---- stage the new Txx and Txz rows rv for the x-derivatives
-        lXX[lane] = qXX[0];
-        lXZ[lane] = qXZ[3];
-        __syncthreads();
-
-        // ---- velocities of row rv
-        if (rv >= zv0 && rv < zv1) {
-            C u = qU[0], ww = qW[0];
-            if (own_x) {
-                const bool rint = rv >= (g.top ? lo : 4) && rv < nz - lo;
-                if (xint && rint) {
+        for (int k = 0; k < R; k++) {
+            const int rs = rs0 + k;
+            C a = oxx[k], bb = ozz[k], cc = oxz[k];
+            if (rs >= 0 && rs < nz && xin) {
+                if (xint && rs >= ztop && rs < nz - lo) {
                     const C c0 = C(1225. / 1024.), c1 = C(245. / 3072.), c2 = C(49. / 5120.), c3 = C(5. / 7168.);
-                    const C dxTxx = dp(&lXX[lane], 1, c0, c1, c2, c3);
-                    const C dxTxz = dm(&lXZ[lane], 1, c0, c1, c2, c3);
-                    const C dzTzz = dm8(qZZ[0], qZZ[1], qZZ[2], qZZ[3], qZZ[4], qZZ[5], qZZ[6], qZZ[7], c0, c1, c2, c3);
-                    const C dzTxz = dp8(qXZ[0], qXZ[1], qXZ[2], qXZ[3], qXZ[4], qXZ[5], qXZ[6], qXZ[7], c0, c1, c2, c3);
-                    psv_velocity_interior(dxTxx, dzTzz, dxTxz, dzTxz, bu, bw, u, ww);
+                    const C ux = dm(&lU[k][lane + HALO], 1, c0, c1, c2, c3);
+                    const C wx = dp(&lW[k][lane + HALO], 1, c0, c1, c2, c3);
+                    const C uz = dm8(qU[k], qU[k + 1], qU[k + 2], qU[k + 3], qU[k + 4], qU[k + 5], qU[k + 6], qU[k + 7],
+                                     c0, c1, c2, c3);
+                    const C wz = dp8(qW[k + 1], qW[k + 2], qW[k + 3], qW[k + 4], qW[k + 5], qW[k + 6], qW[k + 7],
+                                     qW[k + 8], c0, c1, c2, c3);
+                    psv_stress_interior(ux, uz, wx, wz, lam[k], mu[k], mua[k], a, bb, cc);
+                } else {
+                    C c0, c1, c2, c3;
+                    Coef<C>::get(order_at(xc, rs, nx, nz), c0, c1, c2, c3);
+                    const C ux = dm(&lU[k][lane + HALO], 1, c0, c1, c2, c3);
+                    const C wx = dp(&lW[k][lane + HALO], 1, c0, c1, c2, c3);
+                    const C uz = dm8(qU[k], qU[k + 1], qU[k + 2], qU[k + 3], qU[k + 4], qU[k + 5], qU[k + 6], qU[k + 7],
+                                     c0, c1, c2, c3);
+                    const C wz = dp8(qW[k + 1], qW[k + 2], qW[k + 3], qW[k + 4], qW[k + 5], qW[k + 6], qW[k + 7],
+                                     qW[k + 8], c0, c1, c2, c3);
+                    const bool own = own_x && rs >= zv0 && rs < zv1;
+                    psv_stress_exact<S, C>(g, xc, rs, shot, ux, uz, wx, wz, lam[k], mu[k], mua[k], a, bb, cc, spa, spb,
+                                           own, spr, ssc, smp);
+                }
+                // source taps after the cell's update, in the reference's order
+                if (src_on) {
+                    for (int e = t0; e < t1; e++) {
+                        const TapF tp = ev.taps[e];
+                        if (tp.row != rs || tp.lane != lane) continue;
+                        if (tp.field == 2) a = apply_tap(a, tp, sv);
+                        else if (tp.field == 3) bb = apply_tap(bb, tp, sv);
+                        else cc = apply_tap(cc, tp, sv);
+                    }
+                }
+                if (own_x && rs >= zv0 && rs < zv1) {
+                    const long q = base + (long)rs * px + xc;
+                    ST::st(fb.Txx, q, a, wi_xx);
+                    ST::st(fb.Tzz, q, bb, wi_zz);
+                    ST::st(fb.Txz, q, cc, wi_xz);
+                    if (ST::scaled) {
+                        mx[2] = fmaxf(mx[2], fabsf((float)a));
+                        mx[3] = fmaxf(mx[3], fabsf((float)bb));
+                        mx[4] = fmaxf(mx[4], fabsf((float)cc));
+                    }
+                }
+            } else {
+                a = bb = cc = C(0); // outside the domain the new stresses are zero
+            }
+            qXX[4 + k] = a;
+            qZZ[8 + k] = bb;
+            qXZ[8 + k] = cc;
+        }
+
+        // ---- stage the new Txx and Txz rows rv0..rv0+R-1 for the x-derivatives
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            lXX[k][lane] = qXX[k];
+            lXZ[k][lane] = qXZ[k + 4];
+        }
+        __syncthreads();
+
+        // ---- velocities of rows rv0 .. rv0+R-1
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int rv = rv0 + k;
+            if (rv < zv0 || rv >= zv1) continue;
+            C u = qU[k], ww = qW[k];
+            if (own_x) {
+                if (xint && rv >= ztop && rv < nz - lo) {
+                    const C c0 = C(1225. / 1024.), c1 = C(245. / 3072.), c2 = C(49. / 5120.), c3 = C(5. / 7168.);
+                    const C dxTxx = dp(&lXX[k][lane], 1, c0, c1, c2, c3);
+                    const C dxTxz = dm(&lXZ[k][lane], 1, c0, c1, c2, c3);
+                    const C dzTzz = dm8(qZZ[k], qZZ[k + 1], qZZ[k + 2], qZZ[k + 3], qZZ[k + 4], qZZ[k + 5], qZZ[k + 6],
+                                        qZZ[k + 7], c0, c1, c2, c3);
+                    const C dzTxz = dp8(qXZ[k + 1], qXZ[k + 2], qXZ[k + 3], qXZ[k + 4], qXZ[k + 5], qXZ[k + 6],
+                                        qXZ[k + 7], qXZ[k + 8], c0, c1, c2, c3);
+                    psv_velocity_interior(dxTxx, dzTzz, dxTxz, dzTxz, bu[k], bw[k], u, ww);
                 } else {
                     C c0, c1, c2, c3;
                     Coef<C>::get(order_at(xc, rv, nx, nz), c0, c1, c2, c3);
-                    const C dxTxx = dp(&lXX[lane], 1, c0, c1, c2, c3);
-                    const C dxTxz = dm(&lXZ[lane], 1, c0, c1, c2, c3);
-                    const C dzTzz = dm8(qZZ[0], qZZ[1], qZZ[2], qZZ[3], qZZ[4], qZZ[5], qZZ[6], qZZ[7], c0, c1, c2, c3);
-                    const C dzTxz = dp8(qXZ[0], qXZ[1], qXZ[2], qXZ[3], qXZ[4], qXZ[5], qXZ[6], qXZ[7], c0, c1, c2, c3);
-                    psv_velocity_exact<S, C>(g, xc, rv, shot, dxTxx, dzTzz, dxTxz, dzTxz, bu, bw, u, ww, vpa, vpb,
-                                             true, vpr, vsc, vmp);
+                    const C dxTxx = dp(&lXX[k][lane], 1, c0, c1, c2, c3);
+                    const C dxTxz = dm(&lXZ[k][lane], 1, c0, c1, c2, c3);
+                    const C dzTzz = dm8(qZZ[k], qZZ[k + 1], qZZ[k + 2], qZZ[k + 3], qZZ[k + 4], qZZ[k + 5], qZZ[k + 6],
+                                        qZZ[k + 7], c0, c1, c2, c3);
+                    const C dzTxz = dp8(qXZ[k + 1], qXZ[k + 2], qXZ[k + 3], qXZ[k + 4], qXZ[k + 5], qXZ[k + 6],
+                                        qXZ[k + 7], qXZ[k + 8], c0, c1, c2, c3);
+                    psv_velocity_exact<S, C>(g, xc, rv, shot, dxTxx, dzTzz, dxTxz, dzTxz, bu[k], bw[k], u, ww, vpa,
+                                             vpb, true, vpr, vsc, vmp);
                 }
                 const long q = base + (long)rv * px + xc;
                 ST::st(fb.U, q, u, wi_u);
@@ -283,21 +321,31 @@ __global__ __launch_bounds__(FW) void k_psv_fused(PsvFields<S> fa, PsvFields<S> 
                     seis[rc.out] = rc.comp == 0 ? pU[rc.lane] : pW[rc.lane];
                 }
                 r0 = r;
+                __syncthreads();
             }
         }
 
-        // ---- advance the queues and the prefetched operands
+        // ---- advance the queues by R rows and take the prefetched operands
 #pragma unroll
-        for (int i = 0; i < 7; i++) qU[i] = qU[i + 1];
-        qU[7] = nU;
+        for (int i = 0; i < 7; i++) qU[i] = qU[i + R];
 #pragma unroll
-        for (int i = 0; i < 8; i++) qW[i] = qW[i + 1];
-        qW[8] = nW;
-        hU = hU_n; hW = hW_n;
-        oxx = oxx_n; ozz = ozz_n; oxz = oxz_n;
-        lam = lam_n; mu = mu_n; mua = mua_n;
-        bu = bu_n; bw = bw_n;
-        nU = nU_n; nW = nW_n;
+        for (int i = 0; i < 8; i++) qW[i] = qW[i + R];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            qZZ[i] = qZZ[i + R];
+            qXZ[i] = qXZ[i + R];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) qXX[i] = qXX[i + R];
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            qU[7 + k] = nU[k];
+            qW[8 + k] = nW[k];
+            hU[k] = hU_n[k]; hW[k] = hW_n[k];
+            oxx[k] = oxx_n[k]; ozz[k] = ozz_n[k]; oxz[k] = oxz_n[k];
+            lam[k] = lam_n[k]; mu[k] = mu_n[k]; mua[k] = mua_n[k];
+            bu[k] = bu_n[k]; bw[k] = bw_n[k];
+        }
     }
     if (ST::scaled) {
         amax_update(amax, SC_U, mx[0]);

@@ -66,9 +66,10 @@ template <int TZ> __device__ __forceinline__ bool map_tile(const Work &w, TileId
     const int q = gridDim.x >> 3; // grid padded to a multiple of 8
     const int L = (int)(blockIdx.x & 7) * q + (int)(blockIdx.x >> 3);
     if (L >= w.nwork) return false;
-    const int ntiles = w.ntx * w.ntz;
-    t.group = L / ntiles;
-    t.tile = L - t.group * ntiles;
+    // shot groups of one tile are consecutive: they run together on one
+    // XCD, so the tile's material is read from HBM once and from L2 after
+    t.tile = L / w.ngroups;
+    t.group = L - t.tile * w.ngroups;
     const int tz = t.tile / w.ntx;
     t.x0 = (t.tile - tz * w.ntx) * TX;
     t.z0 = tz * TZ;

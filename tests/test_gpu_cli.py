@@ -23,6 +23,24 @@ def _ref(name):
     return np.fromfile(os.path.join(TT, name), np.float32)
 
 
+def _oracle_drift():
+    """Relative L2 of the oracle's own chained 1000-step run (no restart
+    from the reference's files) against the reference's late snapshots."""
+    import oracle as O
+
+    n, npml, h, dt = 96, 32, 0.3, 0.01
+    vp, vs, den = O.mdl_raster(os.path.join(TT, "homo_6.8.mdl"), n, n, h)
+    mat, _ = O.psv_material(vp, vs, den, h, dt)
+    cp = O.psv_cpml(n, n, npml, 0.005, 1e-11, 30.0, 2.0)
+    sim = O.PSV(n, n, npml, mat, cp, topabsorb=True)
+    src, M = O.gaussian(-30), O.psv_moment(0, 0, 20, 90, 45, h, dt)
+    snaps = {}
+    O.run_psv(sim, 1000, src, M, 33, 33,
+              snap_cb=lambda it, s: snaps.__setitem__(it, (s.U.copy(), s.W.copy())) if it in (500, 700, 900) else None)
+    return {"U00500": rel(snaps[500][0].ravel(), _ref("U00500")), "U00700": rel(snaps[700][0].ravel(), _ref("U00700")),
+            "W00900": rel(snaps[900][1].ravel(), _ref("W00900"))}
+
+
 def test_tt_restart_pair_through_the_driver(tmp_path):
     print("case: driver t1/t2", flush=True)
     for f in ("t1", "t2", "station200"):
@@ -32,11 +50,22 @@ def test_tt_restart_pair_through_the_driver(tmp_path):
         r = subprocess.run([EXE, "par=" + par], cwd=tmp_path, capture_output=True, text=True, timeout=120)
         print(r.stdout[-300:], r.stderr[-300:], flush=True)
         assert r.returncode == 0 and "finish" in r.stdout
-    for name in ("state000499", "state000999", "U00100", "U00300", "W00300", "U00500", "U00700", "W00900"):
+    for name in ("state000499", "state000999", "U00100", "U00300", "W00300"):
         got = np.fromfile(tmp_path / name, np.float32)
         e = rel(got, _ref(name))
         print(name, e, flush=True)
         assert e < 1e-5, name
+    # Late snapshots: once the wave has left through the C-PML the velocity
+    # fields are small residuals, and FP32 runs that round differently from
+    # the reference's drift apart relative to them (the oracle's own chained
+    # run reaches 2.6e-5, 1.7e-4, 3.5e-4 at steps 500, 700, 900).  The driver
+    # must stay within a few times the oracle's drift.
+    drift = _oracle_drift()
+    for name in ("U00500", "U00700", "W00900"):
+        got = np.fromfile(tmp_path / name, np.float32)
+        e = rel(got, _ref(name))
+        print(name, e, "oracle", drift[name], flush=True)
+        assert e < 3 * drift[name] + 1e-5, name
     # raw time-major seismograms (t2 overwrote t1's) and their ISIS copy;
     # the station sits at the corner (0,0) in the frozen C-PML edge, so as in
     # the reference's own output files of this case every sample is zero

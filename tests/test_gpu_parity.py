@@ -187,7 +187,7 @@ def test_all_tiles_bitwise_equal():
         assert st == ref
 
 
-FUSED = (10, 11, 12)  # fused P-SV step, segment heights 32, 64, 128
+FUSED = (10, 11, 12, 13)  # fused P-SV step: segments of 32/64/128 rows in blocks of 4, 64 rows in 2
 
 
 @pytest.mark.parametrize("nx,nz,npml,top,kind,storage", [
