@@ -74,11 +74,14 @@ def sync():
         pass
 
 
-def run_gpu(wl, steps, warmup, autotune=True):
+def run_gpu(wl, steps, warmup, autotune=True, tile=None):
     ctx = wl.context(max_frames=warmup + steps)
-    tile, tune_ms = (ctx.tile, None)
-    if autotune:
-        tile, tune_ms = ctx.autotune(4)
+    tune_ms = None
+    if tile is not None:
+        ctx.tile = tile
+    elif autotune:
+        _, tune_ms = ctx.autotune(4)
+    tile = ctx.tile
     # warm-up: also builds the HIP graphs
     ctx.run(0, warmup, 1)
     ctx.sync()
@@ -271,6 +274,7 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tile", type=int, default=None, help="use this tile shape instead of tuning")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (shots and requests shard "
@@ -286,7 +290,7 @@ def main():
     from fd23_amd import _lib
 
     wl = make_workload(args.workload, args.storage).build()
-    r = run_gpu(wl, args.steps, args.warmup, autotune=not args.no_autotune)
+    r = run_gpu(wl, args.steps, args.warmup, autotune=not args.no_autotune, tile=args.tile)
     cells = float(wl.nshots) * wl.nx * wl.nz * args.steps
     gcell = cells / r["elapsed"] / 1e9
     ms = r["elapsed"] / args.steps * 1e3

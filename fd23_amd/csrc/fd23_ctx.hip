@@ -97,7 +97,8 @@ struct fd23_ctx {
     void *alt_psi[8] = {};
     int cur = 0;            // which set is current (keys the captured graphs)
     bool tables_dirty = false;
-    void *mat[5] = {};
+    void *mat[5] = {};      // row 0, col 0 (4 zero rows above and below, zero guards)
+    void *mat_alloc[5] = {};
     void *prof = nullptr;   // [npsi][3][n] compute type
     long prof_off[8] = {};  // element offset of profile p (b, then c, then k)
     int prof_n[8] = {};
@@ -178,6 +179,11 @@ void fd23_ctx_invalidate_graphs(fd23_ctx *c) {
 const float *fd23_ctx_scl(fd23_ctx *c) { return c->scl; }
 
 static bool is_psv(const fd23_ctx *c) { return c->d.wave == FD23_WAVE_PSV; }
+// material arrays: nz rows of pitch px with HALO zero rows above and
+// HALO+1 below and kGuard zero elements at both ends, so that clamped row
+// indices in [-HALO, nz+HALO] and any column of the strips read zeros
+static size_t mat_elems_for(int nz, int px) { return (size_t)(nz + 2 * HALO + 1) * px + 2 * kGuard; }
+static size_t mat_elems(const fd23_ctx *c) { return mat_elems_for(c->d.nz, c->px); }
 
 static size_t layout(const fd23_desc *d, int *px, int *rows, long *sstride, long *xpsi, long *zpsi, int *esize,
                      int *csize) {
@@ -198,7 +204,7 @@ static size_t layout(const fd23_desc *d, int *px, int *rows, long *sstride, long
     size_t bytes = 0;
     bytes += (size_t)nf * ((size_t)S * (*sstride) + 2 * kGuard) * (*esize);
     bytes += (size_t)np * S * ((*xpsi) + (*zpsi)) * (*esize);
-    bytes += (size_t)nm * (size_t)d->nz * (*px) * (*csize);
+    bytes += (size_t)nm * ((size_t)(d->nz + 2 * HALO + 1) * (*px) + 2 * kGuard) * (*csize);
     bytes += (size_t)2 * np * 3 * (d->nx + d->nz) * (*csize);
     bytes += (size_t)d->max_frames * S * nc * d->max_rec * (*csize);
     bytes += (size_t)S * nc * d->max_rec * sizeof(long);
@@ -228,7 +234,7 @@ static void free_ctx(fd23_ctx *c) {
     for (auto p : c->psi) if (p) hipFree(p);
     for (auto p : c->alt_alloc) if (p) hipFree(p);
     for (auto p : c->alt_psi) if (p) hipFree(p);
-    for (auto p : c->mat) if (p) hipFree(p);
+    for (auto p : c->mat_alloc) if (p) hipFree(p);
     void *others[] = {c->prof, c->d_fieldtab, c->d_comptab, c->d_compfield, c->scl, c->amax, c->clock,
                       c->d_taps, c->d_src, c->d_stf, c->d_roff, c->d_nrec, c->seis, c->d_tap_start,
                       c->d_rec_start, c->d_tapsT, c->d_recs, c->d_ftap_start, c->d_frec_start, c->d_ftaps,
@@ -299,7 +305,10 @@ extern "C" int fd23_ctx_create(const fd23_desc *desc, fd23_ctx **out) {
         size_t n = std::max<size_t>(1, (size_t)S * (xdir ? c->xpsi : c->zpsi)) * c->esize;
         ALLOC(c->psi[p], n);
     }
-    for (int m = 0; m < c->nmat; m++) ALLOC(c->mat[m], (size_t)d.nz * c->px * c->csize);
+    for (int m = 0; m < c->nmat; m++) {
+        ALLOC(c->mat_alloc[m], mat_elems(c) * c->csize);
+        c->mat[m] = (char *)c->mat_alloc[m] + (size_t)(kGuard + (long)HALO * c->px) * c->csize;
+    }
     {
         long off = 0;
         for (int p = 0; p < c->npsi; p++) {
@@ -390,8 +399,15 @@ extern "C" int fd23_set_cpml(fd23_ctx *c, const double *packed) {
 // per-field power-of-two scales for FP16/BF16 storage
 extern "C" int fd23_set_scales(fd23_ctx *c, const float *scales, int n) {
     if (!c || !scales || n < 1 || n > 16) return fail(FD23_EINVAL, "bad scales");
-    for (int i = 0; i < n; i++)
-        if (!(scales[i] > 0) || !std::isfinite(1.0f / scales[i])) return fail(FD23_EINVAL, "scale %d is not usable", i);
+    // the kernels invert a scale by negating its exponent, so scales are
+    // powers of two in [2^-kScaleExpMax, 2^kScaleExpMax]
+    for (int i = 0; i < n; i++) {
+        int e = 0;
+        const float m = std::frexp(scales[i], &e);
+        if (!(scales[i] > 0) || m != 0.5f || e - 1 < -kScaleExpMax || e - 1 > kScaleExpMax)
+            return fail(FD23_EINVAL, "scale %d (%g) is not a power of two in [2^-%d, 2^%d]", i, scales[i],
+                        kScaleExpMax, kScaleExpMax);
+    }
     // current and next scales alike
     HIPCHK(hipMemcpyAsync(c->scl, scales, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->scl + kNextScale, scales, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
@@ -795,7 +811,7 @@ static Events make_events(const fd23_ctx *c, int k, int itrecord) {
     return e;
 }
 
-template <class S, int RZ, int SPB> static void launch_stress_t(fd23_ctx *c, const Events &ev) {
+template <class S, int RZ, int SPB> static void launch_stress_t(fd23_ctx *c, const Events &ev, unsigned *amax) {
     using C = typename Store<S>::C;
     const Grid g = make_grid(c);
     dim3 grid;
@@ -808,18 +824,18 @@ template <class S, int RZ, int SPB> static void launch_stress_t(fd23_ctx *c, con
         PsvStressPsi<S> ps{(S *)c->psi[2], (S *)c->psi[3], (S *)c->psi[6], (S *)c->psi[7]};
         PsvStressProf<C> pr{prof_of<C>(c, 2), prof_of<C>(c, 3), prof_of<C>(c, 6), prof_of<C>(c, 7)};
         hipLaunchKernelGGL((k_psv_stress<S, RZ, SPB>), grid, block, 0, c->stream, f, m, ps, pr, g, w, ev, c->scl,
-                           c->amax);
+                           amax);
     } else {
         ShFields<S> f{(S *)c->fields[0], (S *)c->fields[1], (S *)c->fields[2]};
         ShStressMat<C> m{(const C *)c->mat[1], (const C *)c->mat[2]};
         ShPsi2<S> ps{(S *)c->psi[1], (S *)c->psi[3]};
         ShProf2<C> pr{prof_of<C>(c, 1), prof_of<C>(c, 3)};
         hipLaunchKernelGGL((k_sh_stress<S, RZ, SPB>), grid, block, 0, c->stream, f, m, ps, pr, g, w, ev, c->scl,
-                           c->amax);
+                           amax);
     }
 }
 
-template <class S, int RZ, int SPB> static void launch_velocity_t(fd23_ctx *c, const Events &ev) {
+template <class S, int RZ, int SPB> static void launch_velocity_t(fd23_ctx *c, const Events &ev, unsigned *amax) {
     using C = typename Store<S>::C;
     const Grid g = make_grid(c);
     dim3 grid;
@@ -832,21 +848,21 @@ template <class S, int RZ, int SPB> static void launch_velocity_t(fd23_ctx *c, c
         PsvVelPsi<S> ps{(S *)c->psi[0], (S *)c->psi[1], (S *)c->psi[5], (S *)c->psi[4]};
         PsvVelProf<C> pr{prof_of<C>(c, 0), prof_of<C>(c, 1), prof_of<C>(c, 5), prof_of<C>(c, 4)};
         hipLaunchKernelGGL((k_psv_velocity<S, RZ, SPB>), grid, block, 0, c->stream, f, m, ps, pr, g, w, ev,
-                           c->scl, c->amax);
+                           c->scl, amax);
     } else {
         ShFields<S> f{(S *)c->fields[0], (S *)c->fields[1], (S *)c->fields[2]};
         ShPsi2<S> ps{(S *)c->psi[0], (S *)c->psi[2]};
         ShProf2<C> pr{prof_of<C>(c, 0), prof_of<C>(c, 2)};
         hipLaunchKernelGGL((k_sh_velocity<S, RZ, SPB>), grid, block, 0, c->stream, f, (const C *)c->mat[0], ps,
-                           pr, g, w, ev, c->scl, c->amax);
+                           pr, g, w, ev, c->scl, amax);
     }
 }
 
-template <class S> static void launch_pass(fd23_ctx *c, bool stress, const Events &ev) {
+template <class S> static void launch_pass(fd23_ctx *c, bool stress, const Events &ev, unsigned *amax) {
     const TileCfg t = kTiles[pass_tile(c)];
 #define TILE_CASE(R, P)                                                                        \
     if (t.rz == R && t.spb == P) {                                                             \
-        if (stress) launch_stress_t<S, R, P>(c, ev); else launch_velocity_t<S, R, P>(c, ev); \
+        if (stress) launch_stress_t<S, R, P>(c, ev, amax); else launch_velocity_t<S, R, P>(c, ev, amax); \
         return;                                                                                \
     }
     TILE_CASE(4, 1) TILE_CASE(2, 1) TILE_CASE(8, 1) TILE_CASE(4, 2) TILE_CASE(4, 4) TILE_CASE(4, 8)
@@ -854,16 +870,20 @@ template <class S> static void launch_pass(fd23_ctx *c, bool stress, const Event
 #undef TILE_CASE
 }
 
-static void dispatch_pass(fd23_ctx *c, bool stress, const Events &ev) {
+// track: gather the arrays' maxima for the scale check that follows this
+// step (scaled storage only)
+static void dispatch_pass(fd23_ctx *c, bool stress, const Events &ev, bool track = false) {
+    unsigned *amax = track ? c->amax : nullptr;
     switch (c->d.storage) {
-    case FD23_STORE_F16: launch_pass<__half>(c, stress, ev); break;
-    case FD23_STORE_BF16: launch_pass<__hip_bfloat16>(c, stress, ev); break;
-    case FD23_STORE_F64: launch_pass<double>(c, stress, ev); break;
-    default: launch_pass<float>(c, stress, ev); break;
+    case FD23_STORE_F16: launch_pass<__half>(c, stress, ev, amax); break;
+    case FD23_STORE_BF16: launch_pass<__hip_bfloat16>(c, stress, ev, amax); break;
+    case FD23_STORE_F64: launch_pass<double>(c, stress, ev, nullptr); break;
+    default: launch_pass<float>(c, stress, ev, nullptr); break;
     }
 }
 
-template <class S, int H, int R> static void launch_fused_t(fd23_ctx *c, int k, int itrecord, bool record) {
+template <class S, int H, int R>
+static void launch_fused_t(fd23_ctx *c, int k, int itrecord, bool record, unsigned *amax) {
     using C = typename Store<S>::C;
     const Grid g = make_grid(c);
     FusedWork w;
@@ -897,24 +917,25 @@ template <class S, int H, int R> static void launch_fused_t(fd23_ctx *c, int k, 
     PsvStressProf<C> spr{prof_of<C>(c, 2), prof_of<C>(c, 3), prof_of<C>(c, 6), prof_of<C>(c, 7)};
     PsvVelProf<C> vpr{prof_of<C>(c, 0), prof_of<C>(c, 1), prof_of<C>(c, 5), prof_of<C>(c, 4)};
     hipLaunchKernelGGL((k_psv_fused<S, H, R>), grid, dim3(FW), 0, c->stream, fields(c->fields), fields(c->alt_fields),
-                       m, spa, spb, vpa, vpb, spr, vpr, g, w, ev, c->scl, c->amax);
+                       m, spa, spb, vpa, vpb, spr, vpr, g, w, ev, c->scl, amax);
 }
 
-template <class S> static void launch_fused_h(fd23_ctx *c, int k, int itrecord, bool record) {
+template <class S> static void launch_fused_h(fd23_ctx *c, int k, int itrecord, bool record, unsigned *amax) {
     const TileCfg t = kTiles[c->tile];
-    if (t.fr == 2) launch_fused_t<S, 64, 2>(c, k, itrecord, record);
-    else if (t.fh == 32) launch_fused_t<S, 32, 4>(c, k, itrecord, record);
-    else if (t.fh == 64) launch_fused_t<S, 64, 4>(c, k, itrecord, record);
-    else launch_fused_t<S, 128, 4>(c, k, itrecord, record);
+    if (t.fr == 2) launch_fused_t<S, 64, 2>(c, k, itrecord, record, amax);
+    else if (t.fh == 32) launch_fused_t<S, 32, 4>(c, k, itrecord, record, amax);
+    else if (t.fh == 64) launch_fused_t<S, 64, 4>(c, k, itrecord, record, amax);
+    else launch_fused_t<S, 128, 4>(c, k, itrecord, record, amax);
 }
 
 // one fused step: reads the current set, writes the other, then swaps
-static void dispatch_fused(fd23_ctx *c, int k, int itrecord, bool record = true) {
+static void dispatch_fused(fd23_ctx *c, int k, int itrecord, bool record = true, bool track = false) {
+    unsigned *amax = track ? c->amax : nullptr;
     switch (c->d.storage) {
-    case FD23_STORE_F16: launch_fused_h<__half>(c, k, itrecord, record); break;
-    case FD23_STORE_BF16: launch_fused_h<__hip_bfloat16>(c, k, itrecord, record); break;
-    case FD23_STORE_F64: launch_fused_h<double>(c, k, itrecord, record); break;
-    default: launch_fused_h<float>(c, k, itrecord, record); break;
+    case FD23_STORE_F16: launch_fused_h<__half>(c, k, itrecord, record, amax); break;
+    case FD23_STORE_BF16: launch_fused_h<__hip_bfloat16>(c, k, itrecord, record, amax); break;
+    case FD23_STORE_F64: launch_fused_h<double>(c, k, itrecord, record, nullptr); break;
+    default: launch_fused_h<float>(c, k, itrecord, record, nullptr); break;
     }
     swap_sets(c);
 }
@@ -980,16 +1001,19 @@ static bool scaled_storage(const fd23_ctx *c) {
 }
 static const int kScaleEvery = 8;            // steps between scale checks
 // stored max band (log2): grow the scale when the stored maximum passes
-// 2^11 (FP16 overflows at 2^16, so a field may grow 32x between checks);
-// shrink it only when the maximum has decayed below 2^-4 (FP16 keeps its
-// full relative precision down to 2^-14); a change puts the maximum at 2^8
-static const int kScaleLo = -4, kScaleHi = 11, kScaleTarget = 8;
+// 2^11 (FP16 overflows at 2^16, so a field may grow 32x between checks),
+// putting the maximum at 2^8.  Scales never shrink: a wavefront's leading
+// edge (values many decades below its peak) would otherwise pull a scale
+// down just before the peak arrives and overflow; the first scales come
+// from the sources, and FP16 keeps full relative precision 22 binades below
+// the 2^8 target.
+static const int kScaleLo = -1000, kScaleHi = 11, kScaleTarget = 8;
 
 static float pow2_scale(double est) {
     if (!(est > 0) || !std::isfinite(est)) return 1.0f;
     int e;
     std::frexp(est, &e);
-    return std::ldexp(1.0f, e - kScaleTarget);
+    return std::ldexp(1.0f, std::max(-kScaleExpMax, std::min(kScaleExpMax, e - kScaleTarget)));
 }
 
 // First scales from the sources: stresses are of the order of the running
@@ -1031,7 +1055,8 @@ static int init_scales(fd23_ctx *c) {
 }
 
 // FP16/BF16 storage: every kScaleEvery steps the control kernel picks the
-// next scale of each array from its running max; each pass reads arrays in
+// next scale of each array from its maximum, which the passes of the step
+// before the check gather (the other steps skip it); each pass reads arrays in
 // the current scales and writes what it updates in the next ones, and a
 // one-thread commit after the pass makes those the current scales.  The
 // arrays are never rescaled in a pass of their own.
@@ -1059,7 +1084,8 @@ extern "C" int fd23_step_stress(fd23_ctx *c) {
     int r = ready(c);
     if (r) return r;
     if (c->box && (r = fd23_box_launch_pre_stress(c, -1))) return r;
-    dispatch_pass(c, true, kNoEvents);
+    // a scale check follows the velocity pass of every kScaleEvery-th step
+    dispatch_pass(c, true, kNoEvents, (c->staged_steps + 1) % kScaleEvery == 0);
     if (scaled_storage(c)) enqueue_commit(c, true);
     if (c->box && (r = fd23_box_launch_post_stress(c, -1))) return r;
     return check_launch();
@@ -1073,7 +1099,7 @@ extern "C" int fd23_inject(fd23_ctx *c, int it) {
 extern "C" int fd23_step_velocity(fd23_ctx *c) {
     int r = ready(c);
     if (r) return r;
-    dispatch_pass(c, false, kNoEvents);
+    dispatch_pass(c, false, kNoEvents, (c->staged_steps + 1) % kScaleEvery == 0);
     if (scaled_storage(c)) {
         enqueue_commit(c, false);
         if (++c->staged_steps % kScaleEvery == 0) enqueue_scale_control(c);
@@ -1095,8 +1121,9 @@ static int enqueue_step(fd23_ctx *c, int k, int itrecord) {
     // a scale check ends every kScaleEvery-th step of a chunk, so the passes
     // of steps k = 0, kScaleEvery, ... commit the scales they wrote
     const bool commit = scaled_storage(c) && k % kScaleEvery == 0;
+    const bool track = (k + 1) % kScaleEvery == 0;
     if (use_fused(c)) {
-        dispatch_fused(c, k, itrecord);
+        dispatch_fused(c, k, itrecord, true, track);
         if (commit) {
             enqueue_commit(c, true);
             enqueue_commit(c, false);
@@ -1105,10 +1132,10 @@ static int enqueue_step(fd23_ctx *c, int k, int itrecord) {
         return FD23_OK;
     }
     if (c->box && (r = fd23_box_launch_pre_stress(c, k))) return r;
-    dispatch_pass(c, true, ev);
+    dispatch_pass(c, true, ev, track);
     if (commit) enqueue_commit(c, true);
     if (c->box && (r = fd23_box_launch_post_stress(c, k))) return r;
-    dispatch_pass(c, false, ev);
+    dispatch_pass(c, false, ev, track);
     if (commit) enqueue_commit(c, false);
     if (c->box && (r = fd23_box_launch_post_velocity(c, k))) return r;
     if (scaled_storage(c) && (k + 1) % kScaleEvery == 0) enqueue_scale_control(c);

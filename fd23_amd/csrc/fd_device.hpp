@@ -105,17 +105,37 @@ __device__ __forceinline__ C dp(const C *f, int s, C c0, C c1, C c2, C c3) {
     return mad(-c3, f[4 * s] - f[-3 * s], r);
 }
 
+// 1/s for a power of two s = 2^e, |e| <= 126: the exponent negated, exact
+// (the host only accepts power-of-two scales)
+__device__ __forceinline__ float inv_pow2(float s) { return __uint_as_float(0x7f000000u - __float_as_uint(s)); }
+
+// Largest of a non-negative value over the 64 lanes of the wavefront, in
+// every lane.  Non-negative floats order like their bit patterns as
+// integers.  DPP steps: swap lanes in pairs and in halves of each quad,
+// mirror each half-row and each row (every lane then holds its row's max),
+// then broadcast lane 15 into rows 1 and 3 and lane 31 into rows 2 and 3, so
+// that lane 63 holds the wave's max.
+__device__ __forceinline__ float wave_max_nonneg(float v) {
+    int x = __float_as_int(v);
+    x = max(x, __builtin_amdgcn_update_dpp(x, x, 0xb1, 0xf, 0xf, false));  // quad_perm [1,0,3,2]
+    x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x4e, 0xf, 0xf, false));  // quad_perm [2,3,0,1]
+    x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x141, 0xf, 0xf, false)); // row_half_mirror
+    x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x140, 0xf, 0xf, false)); // row_mirror
+    x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x142, 0xa, 0xf, false)); // row_bcast:15
+    x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x143, 0xc, 0xf, false)); // row_bcast:31
+    return __int_as_float(__builtin_amdgcn_readlane(x, 63));
+}
+
 // Running max of |v| per array (scaled storage): reduced across the 64
-// lanes of the wavefront, then one atomicMax on the float bits (non-negative
-// floats order like unsigned integers).
-// Each array's maximum is spread over kAmaxSpread words picked by the
-// workgroup index, so that the atomics of a pass do not all queue on one
-// address; the scale check reduces the words.
+// lanes of the wavefront, then one atomicMax on the float bits.  Each
+// array's maximum is spread over kAmaxSpread words picked by the workgroup
+// index, so that the atomics of a pass do not all queue on one address; the
+// scale check reduces the words.  amax is null on steps that no scale check
+// follows: the maxima are only gathered on the step before each check.
 constexpr int kAmaxSpread = 256;
 __device__ __forceinline__ void amax_update(unsigned *amax, int slot, float v) {
-    v = fabsf(v);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    if (!amax) return;
+    v = wave_max_nonneg(fabsf(v));
     if ((threadIdx.x & 63) == 0 && v > 0.f)
         atomicMax(&amax[slot * kAmaxSpread + (blockIdx.x & (kAmaxSpread - 1))], __float_as_uint(v));
 }

@@ -115,28 +115,36 @@ __global__ __launch_bounds__(FW) void k_psv_fused(PsvFields<S> fa, PsvFields<S> 
     PsiScl4 ssc{{1.f, 1.f, 1.f, 1.f}, {1.f, 1.f, 1.f, 1.f}}, vsc{{1.f, 1.f, 1.f, 1.f}, {1.f, 1.f, 1.f, 1.f}};
     if (ST::scaled) {
         sc_u = scl[SC_U]; sc_w = scl[SC_W]; sc_xx = scl[SC_TXX]; sc_zz = scl[SC_TZZ]; sc_xz = scl[SC_TXZ];
-        wi_u = 1.f / scl[kNextScale + SC_U]; wi_w = 1.f / scl[kNextScale + SC_W];
-        wi_xx = 1.f / scl[kNextScale + SC_TXX]; wi_zz = 1.f / scl[kNextScale + SC_TZZ];
-        wi_xz = 1.f / scl[kNextScale + SC_TXZ];
+        wi_u = inv_pow2(scl[kNextScale + SC_U]); wi_w = inv_pow2(scl[kNextScale + SC_W]);
+        wi_xx = inv_pow2(scl[kNextScale + SC_TXX]); wi_zz = inv_pow2(scl[kNextScale + SC_TZZ]);
+        wi_xz = inv_pow2(scl[kNextScale + SC_TXZ]);
         const int so[4] = {SC_PU_X, SC_PW_X, SC_PU_Z, SC_PW_Z}, vo[4] = {SC_PTXX_X, SC_PTXZ_X, SC_PTXZ_Z, SC_PTZZ_Z};
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            ssc.r[i] = scl[so[i]]; ssc.w[i] = 1.f / scl[kNextScale + so[i]];
-            vsc.r[i] = scl[vo[i]]; vsc.w[i] = 1.f / scl[kNextScale + vo[i]];
+            ssc.r[i] = scl[so[i]]; ssc.w[i] = inv_pow2(scl[kNextScale + so[i]]);
+            vsc.r[i] = scl[vo[i]]; vsc.w[i] = inv_pow2(scl[kNextScale + vo[i]]);
         }
     }
     float mx[5] = {0.f, 0.f, 0.f, 0.f, 0.f}; // U W Txx Tzz Txz
     float smp[4] = {0.f, 0.f, 0.f, 0.f}, vmp[4] = {0.f, 0.f, 0.f, 0.f};
 
-    // row loads (zero outside the domain rows; columns outside are pads)
+    // Loads are unconditional, so that all loads of a block are in flight at
+    // once: a row outside the domain is clamped to the nearest zero pad row
+    // (fields and material both have HALO zero rows above row 0 and more
+    // below row nz-1), so it reads zero with no test.  Columns left of the
+    // domain read zero pads; columns right of it may read the next row, but
+    // only lanes outside the domain use them and their results are dropped.
     auto ldrow = [&](const S *f, int row, int col, float s) -> C {
-        return (row >= 0 && row < nz) ? ST::ld(f, base + (long)row * px + col, s) : C(0);
+        const int rc = min(max(row, -HALO), nz + HALO);
+        return ST::ld(f, base + (long)rc * px + col, s);
     };
     auto ldmat = [&](const C *f, int row) -> C {
-        return (row >= 0 && row < nz && xin) ? f[(long)row * px + xc] : C(0);
+        const int rc = min(max(row, -HALO), nz + HALO);
+        return f[(long)rc * px + xc];
     };
     // the 8 extra LDS columns: lanes 0-3 -> x0-8..x0-5, lanes 4-7 -> x0+60..x0+63
-    const int xcol = lane < 4 ? x0 - 2 * HALO + lane : x0 + FO + HALO + (lane - 4);
+    // (the other lanes load their own column and drop it)
+    const int xcol = lane < 4 ? x0 - 2 * HALO + lane : lane < 8 ? x0 + FO + HALO + (lane - 4) : xc;
 
     // Row block b: stresses of rows rs0 .. rs0+R-1 (rs0 = zv0-4+R*b) and
     // velocities of rows rs0-4 .. rs0+R-5.  Queues, lowest row first:
@@ -159,8 +167,8 @@ __global__ __launch_bounds__(FW) void k_psv_fused(PsvFields<S> fa, PsvFields<S> 
 #pragma unroll
     for (int k = 0; k < R; k++) {
         const int rs = rs0 + k, rv = rs - HALO;
-        hU[k] = lane < 8 ? ldrow(fa.U, rs, xcol, sc_u) : C(0);
-        hW[k] = lane < 8 ? ldrow(fa.W, rs, xcol, sc_w) : C(0);
+        hU[k] = ldrow(fa.U, rs, xcol, sc_u);
+        hW[k] = ldrow(fa.W, rs, xcol, sc_w);
         oxx[k] = ldrow(fa.Txx, rs, xc, sc_xx);
         ozz[k] = ldrow(fa.Tzz, rs, xc, sc_zz);
         oxz[k] = ldrow(fa.Txz, rs, xc, sc_xz);
@@ -182,8 +190,8 @@ __global__ __launch_bounds__(FW) void k_psv_fused(PsvFields<S> fa, PsvFields<S> 
             const int rs = rs0 + R + k, rv = rs - HALO;
             nU[k] = ldrow(fa.U, rs + 3, xc, sc_u);
             nW[k] = ldrow(fa.W, rs + 4, xc, sc_w);
-            hU_n[k] = lane < 8 ? ldrow(fa.U, rs, xcol, sc_u) : C(0);
-            hW_n[k] = lane < 8 ? ldrow(fa.W, rs, xcol, sc_w) : C(0);
+            hU_n[k] = ldrow(fa.U, rs, xcol, sc_u);
+            hW_n[k] = ldrow(fa.W, rs, xcol, sc_w);
             oxx_n[k] = ldrow(fa.Txx, rs, xc, sc_xx);
             ozz_n[k] = ldrow(fa.Tzz, rs, xc, sc_zz);
             oxz_n[k] = ldrow(fa.Txz, rs, xc, sc_xz);

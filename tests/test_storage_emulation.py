@@ -1,0 +1,42 @@
+"""The reduced-precision storage emulation used as the FP16/BF16 bar:
+its rounding matches library conversions, FP32 storage stays at FP32
+rounding level, and FP16 storage lands at the expected floor."""
+import numpy as np
+import torch
+
+from fd23_amd import _lib
+from helpers import Problem, emulated_storage, rel, round_bf16, round_storage
+
+
+def test_bf16_rounding_matches_torch():
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.standard_normal(4096) * 10.0 ** rng.integers(-6, 6, 4096),
+                        [1.0, 1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -1.0 - 2.0 ** -8, 0.0]])
+    x32 = x.astype(np.float32)
+    want = torch.from_numpy(x32).to(torch.bfloat16).to(torch.float64).numpy()
+    assert np.array_equal(round_bf16(x32.astype(np.float64)), want)
+    # ties go to the even mantissa
+    assert round_bf16(np.array([1.0 + 2.0 ** -8]))[0] == 1.0
+    assert round_bf16(np.array([1.0 + 3 * 2.0 ** -8]))[0] == 1.0 + 2.0 ** -6
+
+
+def test_f16_rounding_keeps_scale_and_ulp():
+    a = np.linspace(-3e-7, 5e-7, 1001)
+    b = a.copy()
+    round_storage(b, "f16")
+    # largest value scaled into [2^8, 2^9): relative rounding at most 2^-11
+    big = np.abs(a) > 1e-9
+    assert np.max(np.abs(b[big] - a[big]) / np.abs(a[big])) <= 2.0 ** -11 * (1 + 1e-12)
+    assert np.array_equal(np.sign(b[big]), np.sign(a[big]))
+
+
+def test_emulated_storage_floors():
+    p = Problem(_lib.WAVE_PSV, 120, 100, 12, 10.0, 8e-4, 200, seed=11, kind=_lib.SRC_LINE, f64=True,
+                sources=[(60, 10)], rec_z=30)
+    _, ref = p.oracle()
+    err = {fmt: max(rel(e, r) for e, r in zip(emulated_storage(p, fmt), ref)) for fmt in ("f32", "f16", "bf16")}
+    print(err)
+    assert err["f32"] < 1e-6
+    # one rounding per array and step: FP16 well above FP32, BF16 about 8x FP16
+    assert 1e-4 < err["f16"] < 2e-2
+    assert 4 * err["f16"] < err["bf16"] < 0.2
