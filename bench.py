@@ -75,6 +75,8 @@ def sync():
 
 
 def run_gpu(wl, steps, warmup, autotune=True, tile=None):
+    from fd23_amd import _lib
+
     ctx = wl.context(max_frames=warmup + steps)
     tune_ms = None
     if tile is not None:
@@ -93,10 +95,49 @@ def run_gpu(wl, steps, warmup, autotune=True, tile=None):
     el = time.perf_counter() - t0
     seis = ctx.seismograms(0, 0, warmup + steps)
     finite = bool(np.isfinite(seis).all())
-    fused = ctx.tile_shape()[2] > 0 and wl.wave == 0
+    kind = ctx.tile_kind()
+    fused = kind == _lib.TILE_FUSED and wl.wave == _lib.WAVE_PSV
+    names = kernel_names(ctx, wl)
     s_ms, v_ms = ctx.time_passes(20)
     ctx.close()
-    return dict(elapsed=el, stress_ms=s_ms, vel_ms=v_ms, tile=tile, tune_ms=tune_ms, finite=finite, fused=fused)
+    return dict(elapsed=el, stress_ms=s_ms, vel_ms=v_ms, tile=tile, tune_ms=tune_ms, finite=finite, fused=fused,
+                kernels=names)
+
+
+STORE_TYPE = {0: "float", 1: "__half", 2: "__hip_bfloat16", 3: "double"}
+
+
+def kernel_names(ctx, wl):
+    """Kernel names of the (stress, velocity) passes as profilers print
+    them; the fused step is one kernel."""
+    from fd23_amd import _lib
+
+    rz, spb, seg, blk = ctx.tile_shape()
+    kind = ctx.tile_kind()
+    t = STORE_TYPE[wl.storage]
+    if wl.wave == _lib.WAVE_SH:
+        if kind != _lib.TILE_TILED:  # the fused step is P-SV only: the tiled kernels run
+            rz, spb = ctx.tile_shape(0)[:2]
+        pre = "k_sh" if wl.wave == _lib.WAVE_SH else "k_psv"
+        return ("fd23::%s_stress<%s, %d, %d>" % (pre, t, rz, spb), "fd23::%s_velocity<%s, %d, %d>" % (pre, t, rz, spb))
+    if kind == _lib.TILE_FUSED:
+        n = "fd23::k_psv_fused<%s, %d, %d>" % (t, seg, blk)
+        return (n, n)
+    return ("fd23::k_psv_stress<%s, %d, %d>" % (t, rz, spb), "fd23::k_psv_velocity<%s, %d, %d>" % (t, rz, spb))
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of a kernel on a workload, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE summaries kept beside this file (scaled by the
+    copy calibration stated there); None when that kernel was not profiled."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = table.get("%s|%s" % (workload, kernel))
+    return None if e is None else int(e["hbm_bytes"])
 
 
 def cpu_baseline(wl, budget_s=12.0, budget_1t=4.0):
@@ -296,13 +337,13 @@ def main():
     ms = r["elapsed"] / args.steps * 1e3
     sb, vb = pass_bytes(wl)
     if r["fused"]:
-        name, nbytes, kms = "fused", fused_bytes(wl), r["stress_ms"]
+        name, nbytes, kms, kname = "fused", fused_bytes(wl), r["stress_ms"], r["kernels"][0]
         step_bytes = nbytes
     elif r["stress_ms"] >= r["vel_ms"]:
-        name, nbytes, kms = "stress", sb, r["stress_ms"]
+        name, nbytes, kms, kname = "stress", sb, r["stress_ms"], r["kernels"][0]
         step_bytes = sb + vb
     else:
-        name, nbytes, kms = "velocity", vb, r["vel_ms"]
+        name, nbytes, kms, kname = "velocity", vb, r["vel_ms"], r["kernels"][1]
         step_bytes = sb + vb
     achieved = nbytes / (kms * 1e-3) / 1e9
     sname = {v: k for k, v in STORAGE.items()}[wl.storage]
@@ -326,7 +367,8 @@ def main():
         "hbm_fraction_step": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "bytes_per_step": step_bytes,
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(wl.name, kname), "kernel_name": kname,
                      "kernel_ms": round(kms, 5), "algorithmic_bytes": nbytes,
                      "stress_ms": None if r["fused"] else round(r["stress_ms"], 5),
                      "velocity_ms": None if r["fused"] else round(r["vel_ms"], 5)},

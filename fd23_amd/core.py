@@ -175,11 +175,19 @@ class Context:
         check(self.lib.fd23_set_tile(self.h, t))
 
     def tile_shape(self, tile=None):
-        """(rz, spb, fused_rows) of a tile index (default: the current one)."""
+        """(rz, spb, seg_rows, block_rows) of a tile index (default: the
+        current one); seg_rows and block_rows are 0 for tiled kinds."""
         t = self.tile if tile is None else tile
-        a, b, f = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        check(self.lib.fd23_tile_shape(t, ctypes.byref(a), ctypes.byref(b), ctypes.byref(f)))
-        return a.value, b.value, f.value
+        v = [ctypes.c_int() for _ in range(4)]
+        check(self.lib.fd23_tile_shape(t, *(ctypes.byref(x) for x in v)))
+        return tuple(x.value for x in v)
+
+    def tile_kind(self, tile=None):
+        """TILE_TILED or TILE_FUSED for a tile index."""
+        k = self.lib.fd23_tile_kind(self.tile if tile is None else tile)
+        if k < 0:
+            check(k)
+        return k
 
     def autotune(self, steps=4):
         best = ctypes.c_double()

@@ -57,12 +57,17 @@ static const TileCfg kTiles[] = {{4, 1, 0, 0},  {2, 1, 0, 0},  {8, 1, 0, 0},  {4
 static const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 extern "C" int fd23_num_tiles(void) { return kNumTiles; }
-extern "C" int fd23_tile_shape(int tile, int *rz, int *spb, int *fused_rows) {
+extern "C" int fd23_tile_shape(int tile, int *rz, int *spb, int *seg_rows, int *block_rows) {
     if (tile < 0 || tile >= kNumTiles) return fail(FD23_EINVAL, "bad tile %d", tile);
     if (rz) *rz = kTiles[tile].rz;
     if (spb) *spb = kTiles[tile].spb;
-    if (fused_rows) *fused_rows = kTiles[tile].fh;
+    if (seg_rows) *seg_rows = kTiles[tile].fh;
+    if (block_rows) *block_rows = kTiles[tile].fr;
     return FD23_OK;
+}
+extern "C" int fd23_tile_kind(int tile) {
+    if (tile < 0 || tile >= kNumTiles) return fail(FD23_EINVAL, "bad tile %d", tile);
+    return kTiles[tile].fh > 0 ? FD23_TILE_FUSED : FD23_TILE_TILED;
 }
 
 constexpr int kMaxTZ = 32;     // largest tile height (BY * max RZ)
@@ -128,7 +133,7 @@ struct fd23_ctx {
     RecF *d_frecs = nullptr;
     size_t ftap_start_cap = 0, frec_start_cap = 0, ftaps_cap = 0, frecs_cap = 0;
     bool fevents_dirty = true;
-    int fevents_h = -1;
+    int fevents_tile = -1; // tile the lists were built for
 
     // per-tile lists of the fused graph loop
     int *d_tap_start = nullptr, *d_rec_start = nullptr;
@@ -658,8 +663,8 @@ static int build_events(fd23_ctx *c) {
 // whose 64 stress columns (its 56 own columns and 4 on each side) hold
 // the tap -- and receivers per (segment, strip, shot) sorted by row.
 static int build_fused_events(fd23_ctx *c) {
+    if (!c->fevents_dirty && c->fevents_tile == c->tile) return FD23_OK;
     const int H = kTiles[c->tile].fh;
-    if (!c->fevents_dirty && c->fevents_h == H) return FD23_OK;
     const int nstrip = (c->d.nx + FO - 1) / FO, nseg = (c->d.nz + H - 1) / H;
     const int S = c->d.nshots;
     std::vector<std::vector<TapF>> tb((size_t)nstrip * S);
@@ -709,7 +714,7 @@ static int build_fused_events(fd23_ctx *c) {
     if ((r = upload_vec(c, c->d_frecs, c->frecs_cap, recs))) return r;
     HIPCHK(hipStreamSynchronize(c->stream));
     c->fevents_dirty = false;
-    c->fevents_h = H;
+    c->fevents_tile = c->tile;
     for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
     c->graphs.clear();
     return FD23_OK;

@@ -162,18 +162,25 @@ int fd23_box_get_residual(fd23_ctx *ctx, int shot, void *out);
 int fd23_set_tile(fd23_ctx *ctx, int tile);
 int fd23_get_tile(const fd23_ctx *ctx);
 int fd23_num_tiles(void);
-/* shape of a tile index: two-pass tiles have rows-per-thread rz (tile
- * height 4*rz) and spb shots per workgroup; fused_rows > 0 marks the
- * fused P-SV step (stress and velocity in one pass over two buffer sets,
- * segments of fused_rows rows); it applies to fd23_run of P-SV contexts
- * without a box, the staged entry points keep the two-pass kernels      */
-int fd23_tile_shape(int tile, int *rz, int *spb, int *fused_rows);
+/* shape of a tile index.  Kinds: FD23_TILE_TILED, two-pass kernels on 2D
+ * tiles of 64 columns x 4*rz rows (rz rows per thread, spb shots per
+ * workgroup) staged through LDS; FD23_TILE_FUSED, the fused P-SV step
+ * (stress and velocity in one pass over two buffer sets, segments of
+ * seg_rows rows in blocks of block_rows; 0 for tiled kinds).  The fused
+ * step applies to fd23_run of P-SV contexts without a box; the staged
+ * entry points, the box and SH run the tiled kernels.                   */
+enum { FD23_TILE_TILED = 0, FD23_TILE_FUSED = 1 };
+int fd23_tile_shape(int tile, int *rz, int *spb, int *seg_rows, int *block_rows);
+/* FD23_TILE_* of a tile index, negative for a bad index                 */
+int fd23_tile_kind(int tile);
 int fd23_autotune(fd23_ctx *ctx, int steps, double *best_ms);
 /* FP16/BF16 storage: per-array power-of-two scales (stored = value/scale);
  * P-SV order U W Txx Tzz Txz then psi Txx_x Txz_x U_x W_x Tzz_z Txz_z U_z
  * W_z; SH order V Tx Tz then psi Tx_x V_x Tz_z V_z.  Set before stepping
- * to override the estimate from the sources; get returns the scales and
- * the running max |value| per array since the last scale check.        */
+ * to override the estimate from the sources (each scale a power of two in
+ * [2^-100, 2^100], else FD23_EINVAL); get returns the scales and the max
+ * |value| per array gathered for the next scale check (on the step before
+ * each check).                                                          */
 int fd23_set_scales(fd23_ctx *ctx, const float *scales, int n);
 int fd23_get_scales(fd23_ctx *ctx, float *scales, unsigned *amax_bits, int n);
 /* algorithmic HBM bytes of one stress+velocity step for this context   */

@@ -196,9 +196,10 @@ FUSED = (10, 11, 12, 13)  # fused P-SV step: segments of 32/64/128 rows in block
     (121, 140, 9, False, _lib.SRC_LINE, _lib.STORE_F64),
 ])
 def test_fused_step_bitwise_equals_two_pass(nx, nz, npml, top, kind, storage):
-    """The fused step runs the two-pass kernels' per-cell code: states and
+    """The fused step runs the tiled kernels' per-cell code: states and
     seismograms must be identical, across buffer swaps (odd step counts),
-    split runs, and staged steps taken after fused ones."""
+    split runs, and staged steps taken after fused ones; receivers sit on
+    strip edges (strips own 56 columns)."""
     _print("case: fused", nx, nz, npml, top, kind, storage)
     srcs = [(nx // 2, 7), (5, 3), (nx - 6, nz // 2)]
     p = Problem(_lib.WAVE_PSV, nx, nz, npml, 10.0, 7e-4, 75, seed=21, topabsorb=top, kind=kind,
