@@ -108,8 +108,10 @@ STORE_TYPE = {0: "float", 1: "__half", 2: "__hip_bfloat16", 3: "double"}
 
 
 def kernel_names(ctx, wl):
-    """Kernel names of the (stress, velocity) passes as profilers print
-    them; the fused step is one kernel."""
+    """Kernels of the (stress, velocity) passes as profilers print them, a
+    tuple per pass: split tile shapes launch the interior tiles and the edge
+    tiles of a P-SV pass as two instances of one kernel; the fused step is
+    one kernel for both."""
     from fd23_amd import _lib
 
     rz, spb, seg, blk = ctx.tile_shape()
@@ -118,26 +120,33 @@ def kernel_names(ctx, wl):
     if wl.wave == _lib.WAVE_SH:
         if kind != _lib.TILE_TILED:  # the fused step is P-SV only: the tiled kernels run
             rz, spb = ctx.tile_shape(0)[:2]
-        pre = "k_sh" if wl.wave == _lib.WAVE_SH else "k_psv"
-        return ("fd23::%s_stress<%s, %d, %d>" % (pre, t, rz, spb), "fd23::%s_velocity<%s, %d, %d>" % (pre, t, rz, spb))
+        return (("fd23::k_sh_stress<%s, %d, %d>" % (t, rz, spb),), ("fd23::k_sh_velocity<%s, %d, %d>" % (t, rz, spb),))
     if kind == _lib.TILE_FUSED:
-        n = "fd23::k_psv_fused<%s, %d, %d>" % (t, seg, blk)
+        n = ("fd23::k_psv_fused<%s, %d, %d>" % (t, seg, blk),)
         return (n, n)
-    return ("fd23::k_psv_stress<%s, %d, %d>" % (t, rz, spb), "fd23::k_psv_velocity<%s, %d, %d>" % (t, rz, spb))
+    regions = (1, 2) if ctx.tile_split() else (0,)
+    return tuple(tuple("fd23::k_psv_%s<%s, %d, %d, %d>" % (p, t, rz, spb, region) for region in regions)
+                 for p in ("stress", "velocity"))
 
 
-def measured_traffic(workload, kernel):
-    """HBM bytes per launch of a kernel on a workload, from the rocprofv3
-    FETCH_SIZE / WRITE_SIZE summaries kept beside this file (scaled by the
-    copy calibration stated there); None when that kernel was not profiled."""
+def measured_traffic(workload, kernels):
+    """HBM bytes per pass (summed over its kernels) on a workload, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summaries kept beside this file
+    (scaled by the copy calibration stated there); None when a kernel of
+    the pass was not profiled."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
     try:
         with open(path) as f:
             table = json.load(f)
     except (OSError, ValueError):
         return None
-    e = table.get("%s|%s" % (workload, kernel))
-    return None if e is None else int(e["hbm_bytes"])
+    total = 0
+    for k in kernels:
+        e = table.get("%s|%s" % (workload, k))
+        if e is None:
+            return None
+        total += int(e["hbm_bytes"])
+    return total
 
 
 def cpu_baseline(wl, budget_s=12.0, budget_1t=4.0):
@@ -368,7 +377,7 @@ def main():
         "bytes_per_step": step_bytes,
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(wl.name, kname), "kernel_name": kname,
+                     "traffic": measured_traffic(wl.name, kname), "kernel_name": list(kname),
                      "kernel_ms": round(kms, 5), "algorithmic_bytes": nbytes,
                      "stress_ms": None if r["fused"] else round(r["stress_ms"], 5),
                      "velocity_ms": None if r["fused"] else round(r["vel_ms"], 5)},

@@ -89,6 +89,7 @@ SIGNATURES = {
     "fd23_num_tiles": (c_int, []),
     "fd23_tile_shape": (c_int, [c_int, P_int, P_int, P_int, P_int]),
     "fd23_tile_kind": (c_int, [c_int]),
+    "fd23_tile_split": (c_int, [c_int]),
     "fd23_autotune": (c_int, [c_vp, c_int, P_dbl]),
     "fd23_bytes_per_step": (c_dbl, [c_vp]),
     "fd23_time_passes": (c_int, [c_vp, c_int, P_dbl, P_dbl]),

@@ -189,6 +189,13 @@ class Context:
             check(k)
         return k
 
+    def tile_split(self, tile=None):
+        """True when the tile shape launches interior and edge tiles apart."""
+        k = self.lib.fd23_tile_split(self.tile if tile is None else tile)
+        if k < 0:
+            check(k)
+        return bool(k)
+
     def autotune(self, steps=4):
         best = ctypes.c_double()
         check(self.lib.fd23_autotune(self.h, steps, ctypes.byref(best)))

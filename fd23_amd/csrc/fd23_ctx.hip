@@ -48,12 +48,19 @@ extern "C" void fd23_set_error_(const char *msg) { g_err = msg; }
 // tile shapes: two-pass tiles (rows per thread RZ, tile height 4*RZ, shots
 // per workgroup SPB) and fused P-SV steps (segment height FH rows)
 // ---------------------------------------------------------------------------
+// split: P-SV passes launch their interior tiles and their edge tiles as
+// two kernels, each compiled for one path (fewer registers, more waves per
+// SIMD; one more launch per pass, which small grids feel)
 struct TileCfg {
-    int rz, spb, fh, fr; // fused: segment height fh, rows per block fr
+    int rz, spb, fh, fr, split; // fused: segment height fh, rows per block fr
 };
-static const TileCfg kTiles[] = {{4, 1, 0, 0},  {2, 1, 0, 0},  {8, 1, 0, 0},  {4, 2, 0, 0},   {4, 4, 0, 0},
-                                 {4, 8, 0, 0},  {8, 2, 0, 0},  {8, 4, 0, 0},  {2, 4, 0, 0},   {2, 8, 0, 0},
-                                 {0, 1, 32, 4}, {0, 1, 64, 4}, {0, 1, 128, 4}, {0, 1, 64, 2}};
+// tile 0, the shape untuned contexts run (the serving engine, the driver),
+// is the one the autotuner picks on most workloads measured
+static const TileCfg kTiles[] = {
+    {2, 1, 0, 0, 0},   {4, 1, 0, 0, 0},   {8, 1, 0, 0, 0},    {4, 2, 0, 0, 0},   {4, 4, 0, 0, 0},
+    {4, 8, 0, 0, 0},   {8, 2, 0, 0, 0},   {8, 4, 0, 0, 0},    {2, 4, 0, 0, 0},   {2, 8, 0, 0, 0},
+    {0, 1, 32, 4, 0},  {0, 1, 64, 4, 0},  {0, 1, 128, 4, 0},  {0, 1, 64, 2, 0},  {2, 1, 0, 0, 1},
+    {4, 1, 0, 0, 1}};
 static const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 extern "C" int fd23_num_tiles(void) { return kNumTiles; }
@@ -64,6 +71,10 @@ extern "C" int fd23_tile_shape(int tile, int *rz, int *spb, int *seg_rows, int *
     if (seg_rows) *seg_rows = kTiles[tile].fh;
     if (block_rows) *block_rows = kTiles[tile].fr;
     return FD23_OK;
+}
+extern "C" int fd23_tile_split(int tile) {
+    if (tile < 0 || tile >= kNumTiles) return fail(FD23_EINVAL, "bad tile %d", tile);
+    return kTiles[tile].split;
 }
 extern "C" int fd23_tile_kind(int tile) {
     if (tile < 0 || tile >= kNumTiles) return fail(FD23_EINVAL, "bad tile %d", tile);
@@ -787,13 +798,26 @@ template <class C> static Prof<C> prof_of(const fd23_ctx *c, int p) {
     return Prof<C>{base, base + n, base + 2 * n};
 }
 
-static Work make_work(const fd23_ctx *c, int RZ, int SPB, dim3 &grid) {
+// Work of one launch over a region of the tile grid.  The interior
+// rectangle holds the tiles whose cells are all at full order with no C-PML
+// and no free surface (the test of tile_interior, separable in x and z);
+// an empty rectangle leaves every tile to the edge launch.
+static Work make_work(const fd23_ctx *c, int RZ, int SPB, dim3 &grid, int region = REGION_ALL) {
     Work w;
+    const int TZ = BY * RZ;
     w.ntx = (c->d.nx + TX - 1) / TX;
-    w.ntz = (c->d.nz + BY * RZ - 1) / (BY * RZ);
+    w.ntz = (c->d.nz + TZ - 1) / TZ;
     w.ngroups = (c->d.nshots + SPB - 1) / SPB;
-    w.nwork = w.ntx * w.ntz * w.ngroups;
-    grid = dim3(round_up(w.nwork, 8), 1, 1);
+    const int lo = std::max(c->d.npml, 4), ztop = c->d.topabsorb ? lo : 4;
+    w.ix0 = (lo + TX - 1) / TX;
+    w.ix1 = c->d.nx - lo - TX >= 0 ? (c->d.nx - lo - TX) / TX + 1 : 0;
+    w.iz0 = (ztop + TZ - 1) / TZ;
+    w.iz1 = c->d.nz - lo - TZ >= 0 ? (c->d.nz - lo - TZ) / TZ + 1 : 0;
+    if (w.ix1 <= w.ix0 || w.iz1 <= w.iz0) w.ix0 = w.ix1 = w.iz0 = w.iz1 = 0;
+    const int inner = (w.ix1 - w.ix0) * (w.iz1 - w.iz0);
+    w.ntiles = region == REGION_INTERIOR ? inner : region == REGION_EDGE ? w.ntx * w.ntz - inner : w.ntx * w.ntz;
+    w.nwork = w.ntiles * w.ngroups;
+    grid = dim3(round_up(std::max(w.nwork, 1), 8), 1, 1);
     return w;
 }
 
@@ -816,7 +840,8 @@ static Events make_events(const fd23_ctx *c, int k, int itrecord) {
     return e;
 }
 
-template <class S, int RZ, int SPB> static void launch_stress_t(fd23_ctx *c, const Events &ev, unsigned *amax) {
+template <class S, int RZ, int SPB, bool SPLIT>
+static void launch_stress_t(fd23_ctx *c, const Events &ev, unsigned *amax) {
     using C = typename Store<S>::C;
     const Grid g = make_grid(c);
     dim3 grid;
@@ -828,8 +853,20 @@ template <class S, int RZ, int SPB> static void launch_stress_t(fd23_ctx *c, con
         PsvStressMat<C> m{(const C *)c->mat[4], (const C *)c->mat[2], (const C *)c->mat[3]};
         PsvStressPsi<S> ps{(S *)c->psi[2], (S *)c->psi[3], (S *)c->psi[6], (S *)c->psi[7]};
         PsvStressProf<C> pr{prof_of<C>(c, 2), prof_of<C>(c, 3), prof_of<C>(c, 6), prof_of<C>(c, 7)};
-        hipLaunchKernelGGL((k_psv_stress<S, RZ, SPB>), grid, block, 0, c->stream, f, m, ps, pr, g, w, ev, c->scl,
-                           amax);
+        if constexpr (SPLIT) {
+            // interior tiles, then the edge tiles (each kernel compiled for one path)
+            dim3 gi, ge;
+            const Work wi = make_work(c, RZ, SPB, gi, REGION_INTERIOR), we = make_work(c, RZ, SPB, ge, REGION_EDGE);
+            if (wi.nwork > 0)
+                hipLaunchKernelGGL((k_psv_stress<S, RZ, SPB, REGION_INTERIOR>), gi, block, 0, c->stream, f, m, ps, pr,
+                                   g, wi, ev, c->scl, amax);
+            if (we.nwork > 0)
+                hipLaunchKernelGGL((k_psv_stress<S, RZ, SPB, REGION_EDGE>), ge, block, 0, c->stream, f, m, ps, pr, g,
+                                   we, ev, c->scl, amax);
+        } else {
+            hipLaunchKernelGGL((k_psv_stress<S, RZ, SPB, REGION_ALL>), grid, block, 0, c->stream, f, m, ps, pr, g, w,
+                               ev, c->scl, amax);
+        }
     } else {
         ShFields<S> f{(S *)c->fields[0], (S *)c->fields[1], (S *)c->fields[2]};
         ShStressMat<C> m{(const C *)c->mat[1], (const C *)c->mat[2]};
@@ -840,7 +877,8 @@ template <class S, int RZ, int SPB> static void launch_stress_t(fd23_ctx *c, con
     }
 }
 
-template <class S, int RZ, int SPB> static void launch_velocity_t(fd23_ctx *c, const Events &ev, unsigned *amax) {
+template <class S, int RZ, int SPB, bool SPLIT>
+static void launch_velocity_t(fd23_ctx *c, const Events &ev, unsigned *amax) {
     using C = typename Store<S>::C;
     const Grid g = make_grid(c);
     dim3 grid;
@@ -852,8 +890,19 @@ template <class S, int RZ, int SPB> static void launch_velocity_t(fd23_ctx *c, c
         PsvVelMat<C> m{(const C *)c->mat[0], (const C *)c->mat[1]};
         PsvVelPsi<S> ps{(S *)c->psi[0], (S *)c->psi[1], (S *)c->psi[5], (S *)c->psi[4]};
         PsvVelProf<C> pr{prof_of<C>(c, 0), prof_of<C>(c, 1), prof_of<C>(c, 5), prof_of<C>(c, 4)};
-        hipLaunchKernelGGL((k_psv_velocity<S, RZ, SPB>), grid, block, 0, c->stream, f, m, ps, pr, g, w, ev,
-                           c->scl, amax);
+        if constexpr (SPLIT) {
+            dim3 gi, ge;
+            const Work wi = make_work(c, RZ, SPB, gi, REGION_INTERIOR), we = make_work(c, RZ, SPB, ge, REGION_EDGE);
+            if (wi.nwork > 0)
+                hipLaunchKernelGGL((k_psv_velocity<S, RZ, SPB, REGION_INTERIOR>), gi, block, 0, c->stream, f, m, ps,
+                                   pr, g, wi, ev, c->scl, amax);
+            if (we.nwork > 0)
+                hipLaunchKernelGGL((k_psv_velocity<S, RZ, SPB, REGION_EDGE>), ge, block, 0, c->stream, f, m, ps, pr,
+                                   g, we, ev, c->scl, amax);
+        } else {
+            hipLaunchKernelGGL((k_psv_velocity<S, RZ, SPB, REGION_ALL>), grid, block, 0, c->stream, f, m, ps, pr, g,
+                               w, ev, c->scl, amax);
+        }
     } else {
         ShFields<S> f{(S *)c->fields[0], (S *)c->fields[1], (S *)c->fields[2]};
         ShPsi2<S> ps{(S *)c->psi[0], (S *)c->psi[2]};
@@ -865,13 +914,15 @@ template <class S, int RZ, int SPB> static void launch_velocity_t(fd23_ctx *c, c
 
 template <class S> static void launch_pass(fd23_ctx *c, bool stress, const Events &ev, unsigned *amax) {
     const TileCfg t = kTiles[pass_tile(c)];
-#define TILE_CASE(R, P)                                                                        \
-    if (t.rz == R && t.spb == P) {                                                             \
-        if (stress) launch_stress_t<S, R, P>(c, ev, amax); else launch_velocity_t<S, R, P>(c, ev, amax); \
-        return;                                                                                \
+    const bool split = t.split && is_psv(c);
+#define TILE_CASE(R, P, X)                                                                                 \
+    if (t.rz == R && t.spb == P && split == X) {                                                           \
+        if (stress) launch_stress_t<S, R, P, X>(c, ev, amax); else launch_velocity_t<S, R, P, X>(c, ev, amax); \
+        return;                                                                                            \
     }
-    TILE_CASE(4, 1) TILE_CASE(2, 1) TILE_CASE(8, 1) TILE_CASE(4, 2) TILE_CASE(4, 4) TILE_CASE(4, 8)
-    TILE_CASE(8, 2) TILE_CASE(8, 4) TILE_CASE(2, 4) TILE_CASE(2, 8)
+    TILE_CASE(4, 1, false) TILE_CASE(2, 1, false) TILE_CASE(8, 1, false) TILE_CASE(4, 2, false)
+    TILE_CASE(4, 4, false) TILE_CASE(4, 8, false) TILE_CASE(8, 2, false) TILE_CASE(8, 4, false)
+    TILE_CASE(2, 4, false) TILE_CASE(2, 8, false) TILE_CASE(2, 1, true) TILE_CASE(4, 1, true)
 #undef TILE_CASE
 }
 
@@ -1461,6 +1512,7 @@ extern "C" int fd23_autotune(fd23_ctx *c, int steps, double *best_ms) {
     double bt = 1e30;
     for (int t = 0; t < kNumTiles; t++) {
         if (kTiles[t].spb > 1 && c->d.nshots < kTiles[t].spb) continue;
+        if (kTiles[t].split && !is_psv(c)) continue; // SH runs split shapes unsplit
         if (tile_fused(t) && (!is_psv(c) || c->box)) continue;
         c->tile = t;
         double a, b;

@@ -55,24 +55,57 @@ __device__ __forceinline__ long zpsi_idx(int ix, int iz, int nx, int nz, int npm
 // ---------------------------------------------------------------------------
 // work mapping
 // ---------------------------------------------------------------------------
+// A launch covers one region of the tile grid: all tiles, the rectangle of
+// interior tiles [ix0, ix1) x [iz0, iz1), or the frame of edge tiles around
+// it.  Interior and edge tiles are launched apart so that each kernel is
+// compiled for one path: the interior kernel then needs a third fewer
+// registers than one holding both paths, and keeps more waves per SIMD.
+enum { REGION_ALL = 0, REGION_INTERIOR = 1, REGION_EDGE = 2 };
 struct Work {
-    int ntx, ntz, ngroups; // tiles in x, in z, shot groups
-    int nwork;             // ntx*ntz*ngroups (logical workgroups)
+    int ntx, ntz, ngroups; // tiles of the whole grid in x, in z; shot groups
+    int ix0, ix1, iz0, iz1; // interior rectangle (tile indices)
+    int ntiles;            // tiles of this launch's region
+    int nwork;             // ntiles*ngroups (logical workgroups)
 };
 struct TileId {
-    int tile, group, x0, z0;
+    int tile, group, x0, z0; // tile: index in the whole grid (ntx per row)
 };
-template <int TZ> __device__ __forceinline__ bool map_tile(const Work &w, TileId &t) {
+template <int TZ, int REGION = REGION_ALL> __device__ __forceinline__ bool map_tile(const Work &w, TileId &t) {
     const int q = gridDim.x >> 3; // grid padded to a multiple of 8
     const int L = (int)(blockIdx.x & 7) * q + (int)(blockIdx.x >> 3);
     if (L >= w.nwork) return false;
     // tiles of one shot group are consecutive, so neighbouring tiles (which
     // read each other's halos) run together on one XCD and share its L2
-    const int ntiles = w.ntx * w.ntz;
-    t.group = L / ntiles;
-    t.tile = L - t.group * ntiles;
-    const int tz = t.tile / w.ntx;
-    t.x0 = (t.tile - tz * w.ntx) * TX;
+    t.group = L / w.ntiles;
+    int k = L - t.group * w.ntiles;
+    int tx, tz;
+    if (REGION == REGION_INTERIOR) {
+        const int wx = w.ix1 - w.ix0;
+        tz = k / wx;
+        tx = w.ix0 + (k - tz * wx);
+        tz += w.iz0;
+    } else if (REGION == REGION_EDGE) {
+        // rows above the rectangle, the side pieces of its rows, rows below
+        const int top = w.iz0 * w.ntx, side = w.ix0 + (w.ntx - w.ix1), mid = (w.iz1 - w.iz0) * side;
+        if (k < top) {
+            tz = k / w.ntx;
+            tx = k - tz * w.ntx;
+        } else if (k - top < mid) {
+            k -= top;
+            const int r = k / side, c = k - r * side;
+            tz = w.iz0 + r;
+            tx = c < w.ix0 ? c : w.ix1 + (c - w.ix0);
+        } else {
+            k -= top + mid;
+            tz = w.iz1 + k / w.ntx;
+            tx = k - (tz - w.iz1) * w.ntx;
+        }
+    } else {
+        tz = k / w.ntx;
+        tx = k - tz * w.ntx;
+    }
+    t.tile = tz * w.ntx + tx;
+    t.x0 = tx * TX;
     t.z0 = tz * TZ;
     return true;
 }
@@ -276,7 +309,7 @@ __device__ __forceinline__ void psv_stress_exact(const Grid &g, int ix, int iz, 
     }
 }
 
-template <class S, int RZ, int SPB>
+template <class S, int RZ, int SPB, int REGION>
 __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStressMat<typename Store<S>::C> m,
                                                        PsvStressPsi<S> ps,
                                                        PsvStressProf<typename Store<S>::C> pr, Grid g, Work w,
@@ -290,13 +323,15 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
     __shared__ C sW[(TZ + 2 * HALO) * LW];
 
     TileId t;
-    if (!map_tile<TZ>(w, t)) return;
+    if (!map_tile<TZ, REGION>(w, t)) return;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int x0 = t.x0, z0 = t.z0;
     const int nx = g.nx, nz = g.nz, px = g.px;
     const int gx = x0 + tx;
     const int lz0 = ty * RZ;
-    const bool interior = tile_interior(g, x0, z0, TZ);
+    const bool interior = REGION == REGION_INTERIOR  ? true
+                          : REGION == REGION_EDGE    ? false
+                                                     : tile_interior(g, x0, z0, TZ);
     const int it = ev.clock ? ev.clock[0] + ev.k : 0;
 
     // stored arrays are read in the current scales scl[0..15] and the
@@ -521,7 +556,7 @@ __device__ __forceinline__ void psv_velocity_exact(const Grid &g, int ix, int iz
     }
 }
 
-template <class S, int RZ, int SPB>
+template <class S, int RZ, int SPB, int REGION>
 __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelMat<typename Store<S>::C> m,
                                                          PsvVelPsi<S> ps,
                                                          PsvVelProf<typename Store<S>::C> pr, Grid g, Work w,
@@ -536,13 +571,15 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
     __shared__ C sXZ[(TZ + 2 * HALO) * LW];
 
     TileId t;
-    if (!map_tile<TZ>(w, t)) return;
+    if (!map_tile<TZ, REGION>(w, t)) return;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int x0 = t.x0, z0 = t.z0;
     const int nx = g.nx, nz = g.nz, px = g.px;
     const int gx = x0 + tx;
     const int lz0 = ty * RZ;
-    const bool interior = tile_interior(g, x0, z0, TZ);
+    const bool interior = REGION == REGION_INTERIOR  ? true
+                          : REGION == REGION_EDGE    ? false
+                                                     : tile_interior(g, x0, z0, TZ);
     const int it = ev.clock ? ev.clock[0] + ev.k : 0;
 
     float sc_u = 1.f, sc_w = 1.f, sc_xx = 1.f, sc_zz = 1.f, sc_xz = 1.f;

@@ -173,6 +173,10 @@ enum { FD23_TILE_TILED = 0, FD23_TILE_FUSED = 1 };
 int fd23_tile_shape(int tile, int *rz, int *spb, int *seg_rows, int *block_rows);
 /* FD23_TILE_* of a tile index, negative for a bad index                 */
 int fd23_tile_kind(int tile);
+/* 1 when a tiled shape launches the interior tiles and the edge tiles of
+ * each P-SV pass as two kernels, each compiled for one path (fewer
+ * registers per lane, one more launch per pass); 0 otherwise           */
+int fd23_tile_split(int tile);
 int fd23_autotune(fd23_ctx *ctx, int steps, double *best_ms);
 /* FP16/BF16 storage: per-array power-of-two scales (stored = value/scale);
  * P-SV order U W Txx Tzz Txz then psi Txx_x Txz_x U_x W_x Tzz_z Txz_z U_z

@@ -83,6 +83,8 @@ PSV_CASES = [
     (131, 77, 10, True, _lib.SRC_LINE, False),
     (200, 64, 16, False, _lib.SRC_P, False),
     (97, 120, 8, True, _lib.SRC_LINE, True),
+    (24, 16, 3, False, _lib.SRC_LINE, False),  # the smallest grid side (16), smaller than one tile
+    (65, 33, 5, True, _lib.SRC_LINE, False),   # one column and one row past a tile
 ]
 
 
@@ -113,6 +115,7 @@ SH_CASES = [
     (150, 90, 12, False, _lib.SRC_LINE),
     (131, 77, 10, True, _lib.SRC_P),
     (120, 100, 9, True, _lib.SRC_FORCE),
+    (17, 16, 3, False, _lib.SRC_FORCE),  # the smallest grid side (16), smaller than one tile
 ]
 
 
