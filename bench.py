@@ -96,7 +96,7 @@ def run_gpu(wl, steps, warmup, autotune=True, tile=None):
     seis = ctx.seismograms(0, 0, warmup + steps)
     finite = bool(np.isfinite(seis).all())
     kind = ctx.tile_kind()
-    fused = kind == _lib.TILE_FUSED and wl.wave == _lib.WAVE_PSV
+    fused = kind == _lib.TILE_FUSED and wl.wave == _lib.WAVE_PSV and wl.punch_after is None
     names = kernel_names(ctx, wl)
     s_ms, v_ms = ctx.time_passes(20)
     ctx.close()
@@ -121,10 +121,15 @@ def kernel_names(ctx, wl):
         if kind != _lib.TILE_TILED:  # the fused step is P-SV only: the tiled kernels run
             rz, spb = ctx.tile_shape(0)[:2]
         return (("fd23::k_sh_stress<%s, %d, %d>" % (t, rz, spb),), ("fd23::k_sh_velocity<%s, %d, %d>" % (t, rz, spb),))
-    if kind == _lib.TILE_FUSED:
+    # a first-arrival window runs the whole-grid tiled kernels (tile 0's
+    # shape in place of a fused one)
+    window = wl.punch_after is not None
+    if kind == _lib.TILE_FUSED and not window:
         n = ("fd23::k_psv_fused<%s, %d, %d>" % (t, seg, blk),)
         return (n, n)
-    regions = (1, 2) if ctx.tile_split() else (0,)
+    if kind == _lib.TILE_FUSED:
+        rz, spb = ctx.tile_shape(0)[:2]
+    regions = (1, 2) if ctx.tile_split() and not window else (0,)
     return tuple(tuple("fd23::k_psv_%s<%s, %d, %d, %d>" % (p, t, rz, spb, region) for region in regions)
                  for p in ("stress", "velocity"))
 
@@ -222,6 +227,8 @@ def make_workload(name, storage=None):
         wl = W.config_psv_precision(_lib.STORE_F32)
     elif name in ("f16", "bf16"):
         wl = W.config_psv_precision(STORAGE[name])
+    elif name == "punch":
+        wl = W.config_psv_punch()
     else:
         wl = W.config_sh_400x200()
     if storage is not None:
@@ -308,13 +315,43 @@ def bench_serve(nreq, max_active):
     }
 
 
+def bench_material(reps=5):
+    """Material set-up of the 4096x2048 P-SV model: the host precompute
+    plus its upload (fd23_psv_material + fd23_set_material) against the
+    device precompute (fd23_set_model: three input arrays uploaded, one
+    kernel), median of reps, the context already created."""
+    from fd23_amd import _lib, host, workloads as W
+    from fd23_amd.core import Context
+
+    wl = W.config_psv_precision(_lib.STORE_F32)
+    vp, vs, den = W.layered_grf_model(wl.nx, wl.nz, wl.dx, wl.seed)
+    t_host, t_dev = [], []
+    with Context(_lib.WAVE_PSV, wl.nx, wl.nz, wl.npml) as ctx:
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            mat, _ = host.psv_material(vp, vs, den, wl.dx, wl.dt)
+            ctx.set_material(mat)
+            t_host.append(time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            ctx.set_model(vp, vs, den, wl.dx, wl.dt)
+            t_dev.append(time.perf_counter() - t0)
+        same = all(np.array_equal(ctx.material(i), mat[i]) for i in range(5))
+    h, d = float(np.median(t_host)) * 1e3, float(np.median(t_dev)) * 1e3
+    return {"metric": "material set-up time", "value": round(d, 3), "unit": "ms", "n_gpus": 1, "steps": None,
+            "warmup": 0, "ms_per_step": None, "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic (seeded layered + GRF model)",
+            "config": {"workload": "material_4096x2048", "nx": wl.nx, "nz": wl.nz, "reps": reps},
+            "host_ms": round(h, 3), "device_ms": round(d, 3), "speedup": round(h / d, 2), "bitwise_equal": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="psv2000",
-                    choices=["psv2000", "batched", "single1024", "c4", "f16", "bf16", "sh400", "serve"])
+                    choices=["psv2000", "batched", "single1024", "c4", "f16", "bf16", "sh400", "serve", "punch",
+                             "material"])
     ap.add_argument("--requests", type=int, default=40, help="serve: requests in the stream")
     ap.add_argument("--max-active", type=int, default=8, help="serve: resident contexts")
     ap.add_argument("--storage", choices=sorted(STORAGE), default=None)
@@ -335,6 +372,9 @@ def main():
         return
     if args.workload == "serve":
         print(json.dumps(bench_serve(args.requests, args.max_active)), flush=True)
+        return
+    if args.workload == "material":
+        print(json.dumps(bench_material()), flush=True)
         return
 
     from fd23_amd import _lib
@@ -383,6 +423,17 @@ def main():
                      "velocity_ms": None if r["fused"] else round(r["vel_ms"], 5)},
         "finite": r["finite"],
     }
+    if wl.punch_after is not None:
+        # the same steps without the window, for the speed-up it gives
+        before, after = wl.punch_before, wl.punch_after
+        wl.punch_before = wl.punch_after = None
+        r0 = run_gpu(wl, args.steps, args.warmup, autotune=False, tile=r["tile"])
+        wl.punch_before, wl.punch_after = before, after
+        out["first_arrival_window"] = {
+            "before_s": before, "after_s": after,
+            "active_fraction": round(wl.punch_active_fraction(args.warmup, args.steps), 4),
+            "ms_per_step_without": round(r0["elapsed"] / args.steps * 1e3, 5),
+            "speedup": round(r0["elapsed"] / r["elapsed"], 3)}
     if args.accuracy:
         out["error_vs_fp64"] = accuracy_vs_f64(args.workload, sname, args.accuracy_steps or wl.nt)
     if not args.no_cpu and wl.wave == _lib.WAVE_PSV and wl.nshots == 1:

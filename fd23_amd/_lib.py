@@ -47,7 +47,8 @@ class Request(ctypes.Structure):
                 ("nrec", c_int), ("rx", c_vp), ("rz", c_vp), ("box", c_int), ("box_x0", c_int),
                 ("box_z0", c_int), ("box_bx", c_int), ("box_bz", c_int), ("box_incident", c_int),
                 ("box_angle", c_dbl), ("box_f0", c_dbl), ("box_t0", c_dbl), ("box_amp", c_dbl),
-                ("box_vp", c_dbl), ("box_vs", c_dbl), ("box_rho", c_dbl), ("model_key", ctypes.c_ulonglong)]
+                ("box_vp", c_dbl), ("box_vs", c_dbl), ("box_rho", c_dbl), ("model_key", ctypes.c_ulonglong),
+                ("model", ctypes.POINTER(c_vp))]
 
 
 class ServerStats(ctypes.Structure):
@@ -66,6 +67,8 @@ SIGNATURES = {
     "fd23_ctx_device_bytes": (c_sz, [c_vp]),
     "fd23_desc_device_bytes": (c_sz, [ctypes.POINTER(Desc)]),
     "fd23_set_material": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
+    "fd23_set_model": (c_int, [c_vp, c_vp, c_vp, c_vp, c_dbl, c_dbl, ctypes.POINTER(c_dbl)]),
+    "fd23_get_material": (c_int, [c_vp, c_int, c_vp]),
     "fd23_set_cpml": (c_int, [c_vp, c_vp]),
     "fd23_set_source": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_int, c_vp]),
     "fd23_set_receivers": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_int]),
@@ -75,6 +78,7 @@ SIGNATURES = {
     "fd23_record": (c_int, [c_vp, c_int, c_int]),
     "fd23_run": (c_int, [c_vp, c_int, c_int, c_int]),
     "fd23_sync": (c_int, [c_vp]),
+    "fd23_set_punch": (c_int, [c_vp, c_vp, c_flt, c_flt, c_flt]),
     "fd23_get_seismograms": (c_int, [c_vp, c_int, c_int, c_vp, c_int]),
     "fd23_get_field": (c_int, [c_vp, c_int, c_int, c_vp]),
     "fd23_set_field": (c_int, [c_vp, c_int, c_int, c_vp]),

@@ -10,7 +10,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfd23.so")
 
-SOURCES = ["fd23_ctx.hip", "fd23_box.hip", "host_precompute.cpp", "par.cpp", "fd23_serve.cpp"]
+SOURCES = ["fd23_ctx.hip", "fd23_box.hip", "fd23_material.hip", "host_precompute.cpp", "par.cpp", "fd23_serve.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -73,6 +73,23 @@ class Context:
         ptrs = (ctypes.c_void_p * len(keep))(*[a.ctypes.data for a in keep])
         check(self.lib.fd23_set_material(self.h, ptrs))
 
+    def set_model(self, vp, vs, den, h, dt):
+        """Material computed on the device from vp, vs, den (nz x nx); returns
+        the stability number (raises Fd23Error above 1).  SH ignores vp."""
+        arrs = [None if a is None else np.ascontiguousarray(a, self.cdtype) for a in (vp, vs, den)]
+        for a in arrs:
+            assert a is None or a.shape == (self.nz, self.nx), a.shape
+        stab = ctypes.c_double(0.0)
+        p = [None if a is None else a.ctypes.data for a in arrs]
+        check(self.lib.fd23_set_model(self.h, p[0], p[1], p[2], h, dt, ctypes.byref(stab)))
+        return stab.value
+
+    def material(self, index):
+        """One material array (fd23_set_material order) as nz x nx."""
+        out = np.zeros((self.nz, self.nx), self.cdtype)
+        check(self.lib.fd23_get_material(self.h, index, _ptr(out)))
+        return out
+
     def set_cpml(self, packed):
         packed = np.ascontiguousarray(packed, np.float64)
         n = self.lib.fd23_cpml_pack_len(self.wave, self.nx, self.nz)
@@ -112,6 +129,17 @@ class Context:
 
     def record(self, it, itrecord=1):
         check(self.lib.fd23_record(self.h, it, itrecord))
+
+    def set_punch(self, times, before=0.0, after=0.0, dt=1.0):
+        """First-arrival window from an nz x nx travel-time grid (None
+        removes it); see fd23_set_punch."""
+        if times is None:
+            check(self.lib.fd23_set_punch(self.h, None, 0.0, 0.0, 1.0))
+            return
+        t = np.ascontiguousarray(times, np.float32)
+        if t.shape != (self.nz, self.nx):
+            raise ValueError(f"travel times {t.shape} != (nz, nx) = {(self.nz, self.nx)}")
+        check(self.lib.fd23_set_punch(self.h, _ptr(t), before, after, dt))
 
     def run(self, it0, nt, itrecord=1):
         check(self.lib.fd23_run(self.h, it0, nt, itrecord))

@@ -15,7 +15,8 @@
 //   raw time-major seismograms {output}_U/_W or _V -> ISIS files.
 // Keys the one-GPU build reads but does not need: gpuid, ngpu, usetable.
 // Additional key: storage=f32|f16|bf16|f64 (default f32).  usepunch=1
-// (first-arrival block skipping) is not supported and is rejected.
+// reads before, after and timefile (an nz*nx float32 travel-time grid) and
+// sets the first-arrival window (SH/slide/cu_slide.cu:7-61).
 #include "fd23.h"
 
 #include <chrono>
@@ -140,16 +141,14 @@ int main(int argc, char **argv) {
 
     // ---- parameters (P-SV param.cpp:94-193, SH param.cpp:12-84) --------
     int restart = 0, nt0 = 0, savestate = 0, itprint = 100, itrecord = 1, ntsnap;
+    int usepunch = 0;
     int v;
     if (psv) {
         par.has_i("restart", restart);
         if (restart) nt0 = par.mi("nt0");
         par.has_i("savestate", savestate);
     } else {
-        if (par.mi("usepunch")) {
-            fprintf(stderr, "usepunch=1 (first-arrival block skipping) is not supported by this build\n");
-            return -1;
-        }
+        usepunch = par.mi("usepunch");
         par.mi("ngpu");
     }
     const int nx = par.mi("nx"), nz = par.mi("nz"), nt = par.mi("nt");
@@ -223,14 +222,28 @@ int main(int argc, char **argv) {
     ntsnap = par.mi("ntsnap");
     std::string snapfile;
     if (!psv) par.has_s("ntsnapfile", snapfile);
-    if (psv && par.has_i("usepunch", v) && v == 1) {
-        fprintf(stderr, "usepunch=1 (first-arrival block skipping) is not supported by this build\n");
-        return -1;
+    if (psv) par.has_i("usepunch", usepunch);
+    float before = 0, after = 0;
+    std::string timefile;
+    if (usepunch == 1) { // P-SV param.cpp:187-191, SH param.cpp:80-84
+        before = par.mf("before");
+        after = par.mf("after");
+        timefile = par.ms("timefile");
     }
     const std::set<int> extra_snaps = snap_list(snapfile);
 
     // ---- model, material, C-PML -----------------------------------------
     const size_t N = (size_t)nx * nz;
+    std::vector<float> times;
+    if (usepunch == 1) { // the travel-time grid, nz rows of nx float32 (cu_slide.cu:24-39)
+        times.resize(N);
+        std::ifstream f(timefile, std::ios::binary);
+        f.read((char *)times.data(), (std::streamsize)(N * sizeof(float)));
+        if (!f) {
+            fprintf(stderr, "cannot read %zu travel times from %s\n", N, timefile.c_str());
+            return -1;
+        }
+    }
     std::vector<float> vp(N), vs(N), den(N);
     CHECK(fd23_model_read(model.c_str(), nx, nz, h, vp.data(), vs.data(), den.data()));
     const bool f64 = storage == FD23_STORE_F64;
@@ -327,6 +340,7 @@ int main(int argc, char **argv) {
     CHECK(fd23_set_source(ctx, 0, kind, xs, zs, stf.data(), lsrc, M));
     for (size_t c = 0; c < rx.size(); c++)
         CHECK(fd23_set_receivers(ctx, 0, (int)c, rx[c].data(), rz[c].data(), (int)rx[c].size()));
+    if (usepunch == 1) CHECK(fd23_set_punch(ctx, times.data(), before, after, dt));
 
     const size_t sbytes = fd23_state_bytes(ctx);
     if (restart) {

@@ -124,6 +124,9 @@ struct fd23_ctx {
     float *scl = nullptr;
     unsigned *amax = nullptr;
     int staged_steps = 0; // steps taken through the staged entry points
+    // first-arrival window per 32 x 16 block (fd23_set_punch)
+    bool punch = false;
+    int *d_pfirst = nullptr, *d_plast = nullptr;
     int *clock = nullptr;
     bool scales_init = false;
     double bmax = 0;   // largest buoyancy-type material value (scale estimate)
@@ -178,7 +181,7 @@ int fd23_box_launch_post_velocity(fd23_ctx *c, int k);
 void fd23_box_free(fd23_ctx *c);
 void fd23_box_set_step(fd23_ctx *c, int it);
 
-// accessors used by the box module
+// accessors used by the box and material modules
 hipStream_t fd23_ctx_stream(fd23_ctx *c) { return c->stream; }
 void *fd23_ctx_field(fd23_ctx *c, int internal) { return c->fields[internal]; }
 void *fd23_ctx_material(fd23_ctx *c, int i) { return c->mat[i]; }
@@ -193,6 +196,11 @@ void fd23_ctx_invalidate_graphs(fd23_ctx *c) {
     c->graphs.clear();
 }
 const float *fd23_ctx_scl(fd23_ctx *c) { return c->scl; }
+// the material module computed the arrays on the device
+void fd23_ctx_material_ready(fd23_ctx *c, double bmax) {
+    c->bmax = bmax;
+    c->have_mat = true;
+}
 
 static bool is_psv(const fd23_ctx *c) { return c->d.wave == FD23_WAVE_PSV; }
 // material arrays: nz rows of pitch px with HALO zero rows above and
@@ -254,7 +262,7 @@ static void free_ctx(fd23_ctx *c) {
     void *others[] = {c->prof, c->d_fieldtab, c->d_comptab, c->d_compfield, c->scl, c->amax, c->clock,
                       c->d_taps, c->d_src, c->d_stf, c->d_roff, c->d_nrec, c->seis, c->d_tap_start,
                       c->d_rec_start, c->d_tapsT, c->d_recs, c->d_ftap_start, c->d_frec_start, c->d_ftaps,
-                      c->d_frecs};
+                      c->d_frecs, c->d_pfirst, c->d_plast};
     for (auto p : others) if (p) hipFree(p);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -393,6 +401,14 @@ extern "C" int fd23_set_material(fd23_ctx *c, const void *const *arrays) {
             bm = std::max(bm, std::fabs(c->csize == 8 ? ((const double *)arrays[m])[q] : (double)((const float *)arrays[m])[q]));
     c->bmax = bm;
     c->have_mat = true;
+    return FD23_OK;
+}
+
+extern "C" int fd23_get_material(fd23_ctx *c, int index, void *out) {
+    if (!c || !out || index < 0 || index >= c->nmat) return fail(FD23_EINVAL, "bad arguments");
+    HIPCHK(hipMemcpy2DAsync(out, (size_t)c->d.nx * c->csize, c->mat[index], (size_t)c->px * c->csize,
+                            (size_t)c->d.nx * c->csize, c->d.nz, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return FD23_OK;
 }
 
@@ -597,7 +613,7 @@ extern "C" int fd23_set_receivers(fd23_ctx *c, int shot, int comp, const int *rx
 static bool tile_fused(int t) { return kTiles[t].fh > 0; }
 // the fused step runs in fd23_run for P-SV contexts without a box; the
 // staged entry points, the box and SH always use two-pass tiles
-static bool use_fused(const fd23_ctx *c) { return tile_fused(c->tile) && is_psv(c) && !c->box; }
+static bool use_fused(const fd23_ctx *c) { return tile_fused(c->tile) && is_psv(c) && !c->box && !c->punch; }
 static int pass_tile(const fd23_ctx *c) { return tile_fused(c->tile) ? 0 : c->tile; }
 static int tile_rows(const fd23_ctx *c) { return BY * kTiles[pass_tile(c)].rz; }
 
@@ -789,6 +805,9 @@ static Grid make_grid(const fd23_ctx *c) {
     g.nshots = c->d.nshots;
     g.xpsi_stride = c->xpsi;
     g.zpsi_stride = c->zpsi;
+    g.pfirst = c->punch ? c->d_pfirst : nullptr;
+    g.plast = c->punch ? c->d_plast : nullptr;
+    g.pnxb = c->d.nx / kPunchBX;
     return g;
 }
 
@@ -853,17 +872,25 @@ static void launch_stress_t(fd23_ctx *c, const Events &ev, unsigned *amax) {
         PsvStressMat<C> m{(const C *)c->mat[4], (const C *)c->mat[2], (const C *)c->mat[3]};
         PsvStressPsi<S> ps{(S *)c->psi[2], (S *)c->psi[3], (S *)c->psi[6], (S *)c->psi[7]};
         PsvStressProf<C> pr{prof_of<C>(c, 2), prof_of<C>(c, 3), prof_of<C>(c, 6), prof_of<C>(c, 7)};
+        bool split_done = false;
         if constexpr (SPLIT) {
-            // interior tiles, then the edge tiles (each kernel compiled for one path)
-            dim3 gi, ge;
-            const Work wi = make_work(c, RZ, SPB, gi, REGION_INTERIOR), we = make_work(c, RZ, SPB, ge, REGION_EDGE);
-            if (wi.nwork > 0)
-                hipLaunchKernelGGL((k_psv_stress<S, RZ, SPB, REGION_INTERIOR>), gi, block, 0, c->stream, f, m, ps, pr,
-                                   g, wi, ev, c->scl, amax);
-            if (we.nwork > 0)
-                hipLaunchKernelGGL((k_psv_stress<S, RZ, SPB, REGION_EDGE>), ge, block, 0, c->stream, f, m, ps, pr, g,
-                                   we, ev, c->scl, amax);
-        } else {
+            // the interior-only kernel has no first-arrival window check: with a
+            // window the whole-grid kernel runs
+            if (!c->punch) {
+                // interior tiles, then the edge tiles (each kernel compiled for one path)
+                dim3 gi, ge;
+                const Work wi = make_work(c, RZ, SPB, gi, REGION_INTERIOR);
+                const Work we = make_work(c, RZ, SPB, ge, REGION_EDGE);
+                if (wi.nwork > 0)
+                    hipLaunchKernelGGL((k_psv_stress<S, RZ, SPB, REGION_INTERIOR>), gi, block, 0, c->stream, f, m, ps,
+                                       pr, g, wi, ev, c->scl, amax);
+                if (we.nwork > 0)
+                    hipLaunchKernelGGL((k_psv_stress<S, RZ, SPB, REGION_EDGE>), ge, block, 0, c->stream, f, m, ps, pr,
+                                       g, we, ev, c->scl, amax);
+                split_done = true;
+            }
+        }
+        if (!split_done) {
             hipLaunchKernelGGL((k_psv_stress<S, RZ, SPB, REGION_ALL>), grid, block, 0, c->stream, f, m, ps, pr, g, w,
                                ev, c->scl, amax);
         }
@@ -890,16 +917,24 @@ static void launch_velocity_t(fd23_ctx *c, const Events &ev, unsigned *amax) {
         PsvVelMat<C> m{(const C *)c->mat[0], (const C *)c->mat[1]};
         PsvVelPsi<S> ps{(S *)c->psi[0], (S *)c->psi[1], (S *)c->psi[5], (S *)c->psi[4]};
         PsvVelProf<C> pr{prof_of<C>(c, 0), prof_of<C>(c, 1), prof_of<C>(c, 5), prof_of<C>(c, 4)};
+        bool split_done = false;
         if constexpr (SPLIT) {
-            dim3 gi, ge;
-            const Work wi = make_work(c, RZ, SPB, gi, REGION_INTERIOR), we = make_work(c, RZ, SPB, ge, REGION_EDGE);
-            if (wi.nwork > 0)
-                hipLaunchKernelGGL((k_psv_velocity<S, RZ, SPB, REGION_INTERIOR>), gi, block, 0, c->stream, f, m, ps,
-                                   pr, g, wi, ev, c->scl, amax);
-            if (we.nwork > 0)
-                hipLaunchKernelGGL((k_psv_velocity<S, RZ, SPB, REGION_EDGE>), ge, block, 0, c->stream, f, m, ps, pr,
-                                   g, we, ev, c->scl, amax);
-        } else {
+            // the interior-only kernel has no first-arrival window check: with a
+            // window the whole-grid kernel runs
+            if (!c->punch) {
+                dim3 gi, ge;
+                const Work wi = make_work(c, RZ, SPB, gi, REGION_INTERIOR);
+                const Work we = make_work(c, RZ, SPB, ge, REGION_EDGE);
+                if (wi.nwork > 0)
+                    hipLaunchKernelGGL((k_psv_velocity<S, RZ, SPB, REGION_INTERIOR>), gi, block, 0, c->stream, f, m,
+                                       ps, pr, g, wi, ev, c->scl, amax);
+                if (we.nwork > 0)
+                    hipLaunchKernelGGL((k_psv_velocity<S, RZ, SPB, REGION_EDGE>), ge, block, 0, c->stream, f, m, ps,
+                                       pr, g, we, ev, c->scl, amax);
+                split_done = true;
+            }
+        }
+        if (!split_done) {
             hipLaunchKernelGGL((k_psv_velocity<S, RZ, SPB, REGION_ALL>), grid, block, 0, c->stream, f, m, ps, pr, g,
                                w, ev, c->scl, amax);
         }
@@ -1136,9 +1171,55 @@ static void enqueue_commit(fd23_ctx *c, bool stress) {
     hipLaunchKernelGGL(k_commit_scales, dim3(1), dim3(64), 0, c->stream, c->scl, pass_mask(c, stress));
 }
 
+// First-arrival window, SH/slide/cu_slide.cu:7-61: per block of 32 x 16
+// cells the step window from the block's earliest and latest arrival time,
+// in float arithmetic and truncated to int as the reference does.
+extern "C" int fd23_set_punch(fd23_ctx *c, const float *times, float before, float after, float dt) {
+    if (!c) return fail(FD23_EINVAL, "null context");
+    if (!times) {
+        c->punch = false;
+        fd23_ctx_invalidate_graphs(c);
+        return FD23_OK;
+    }
+    // FP16/BF16 storage rewrites every cell each step in the next step's
+    // scale; a cell left alone would be read back in the wrong one
+    if (scaled_storage(c)) return fail(FD23_EINVAL, "a first-arrival window needs FP32 or FP64 storage");
+    const int nx = c->d.nx, nz = c->d.nz;
+    if (nx % kPunchBX || nz % kPunchBZ)
+        return fail(FD23_EINVAL, "a first-arrival window needs nx a multiple of %d and nz of %d", kPunchBX, kPunchBZ);
+    if (!(dt > 0)) return fail(FD23_EINVAL, "bad dt for the first-arrival window");
+    const int nxb = nx / kPunchBX, nzb = nz / kPunchBZ;
+    std::vector<int> first((size_t)nxb * nzb), last((size_t)nxb * nzb);
+    for (int k = 0; k < nzb; k++)
+        for (int i = 0; i < nxb; i++) {
+            float lo = 1e28f, hi = -1e28f;
+            for (int kk = 0; kk < kPunchBZ; kk++)
+                for (int ii = 0; ii < kPunchBX; ii++) {
+                    const float t = times[(size_t)(k * kPunchBZ + kk) * nx + i * kPunchBX + ii];
+                    lo = std::min(lo, t);
+                    hi = std::max(hi, t);
+                }
+            first[(size_t)k * nxb + i] = (int)((lo - before) / dt);
+            last[(size_t)k * nxb + i] = (int)((hi + after) / dt);
+        }
+    int r;
+    size_t cap_f = c->d_pfirst ? first.size() : 0, cap_l = c->d_plast ? last.size() : 0;
+    if ((r = upload_vec(c, c->d_pfirst, cap_f, first)) || (r = upload_vec(c, c->d_plast, cap_l, last))) return r;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->punch = true;
+    fd23_ctx_invalidate_graphs(c);
+    return FD23_OK;
+}
+
+// the first-arrival window needs the step index, which only the graph
+// loop keeps on the device
+static int no_punch(const fd23_ctx *c) {
+    return c->punch ? fail(FD23_EINVAL, "a first-arrival window is set: step with fd23_run") : FD23_OK;
+}
+
 extern "C" int fd23_step_stress(fd23_ctx *c) {
     int r = ready(c);
-    if (r) return r;
+    if (r || (r = no_punch(c))) return r;
     if (c->box && (r = fd23_box_launch_pre_stress(c, -1))) return r;
     // a scale check follows the velocity pass of every kScaleEvery-th step
     dispatch_pass(c, true, kNoEvents, (c->staged_steps + 1) % kScaleEvery == 0);
@@ -1154,7 +1235,7 @@ extern "C" int fd23_inject(fd23_ctx *c, int it) {
 }
 extern "C" int fd23_step_velocity(fd23_ctx *c) {
     int r = ready(c);
-    if (r) return r;
+    if (r || (r = no_punch(c))) return r;
     dispatch_pass(c, false, kNoEvents, (c->staged_steps + 1) % kScaleEvery == 0);
     if (scaled_storage(c)) {
         enqueue_commit(c, false);

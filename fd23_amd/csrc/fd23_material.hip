@@ -1,0 +1,228 @@
+// Material precompute on the device: the context's material arrays from
+// vp, vs and den, the same values as the host precompute
+// (host_precompute.cpp, checked against the reference's own host code),
+// computed by one pointwise kernel instead of a host pass over the grid.
+//
+// P-SV, PSV/clean_nobox/material/material.cpp:35-240: Lame parameters per
+// cell (T arithmetic, differences in double), the water treatment of the
+// 5-cell left/right/bottom edges (vs/vp < 0.01: mu = vp^2/3 rho), buoyancy
+// BU toward +x (wrapping across rows) and BW toward -z (the last row uses
+// its own density), MUA the harmonic mean of four mu with the reference's
+// repeated m2m3m4 term in the denominator, row 0 as the Mittet free surface
+// (ix in [1, nx-2]: BU = 4/(rho+rho'), LAM = 0, MU halved), then every
+// array times dt/h.  SH, SH/material/material.cpp:76-130: BV = 1/rho and
+// harmonic means of mu toward +x (wrapping) and -z (row 0 uses itself).
+// The stability number needs the largest vp (vs for SH), and the FP16/BF16
+// scale estimate the largest buoyancy: both are reduced on the device.
+#include "fd23.h"
+#include "fd_device.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#pragma clang fp contract(off)
+
+using namespace fd23;
+
+struct fd23_ctx;
+hipStream_t fd23_ctx_stream(fd23_ctx *c);
+void *fd23_ctx_material(fd23_ctx *c, int i);
+int fd23_ctx_px(fd23_ctx *c);
+const fd23_desc &fd23_ctx_desc(fd23_ctx *c);
+void fd23_ctx_material_ready(fd23_ctx *c, double bmax);
+extern "C" void fd23_set_error_(const char *msg);
+
+namespace {
+
+int merr(int code, const std::string &m) {
+    fd23_set_error_(m.c_str());
+    return code;
+}
+#define MCHK(x)                                                                                      \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return merr(FD23_EHIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// order-preserving map of a double onto unsigned 64-bit integers, so that
+// atomicMax on the keys is a max on the values
+__device__ __forceinline__ unsigned long long order_key(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+double order_value(unsigned long long k) {
+    const unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    double v;
+    memcpy(&v, &b, sizeof v);
+    return v;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = v;
+}
+
+// wave-wide max, then one atomic per wave
+__device__ __forceinline__ void wave_max_to(unsigned long long *dst, double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(dst, order_key(v));
+}
+
+template <class T>
+__device__ __forceinline__ void lame(const T *vp, const T *vs, const T *den, long q, bool edge, T &mu, T &lam) {
+    mu = vs[q] * vs[q] * den[q];
+    lam = (T)(vp[q] * vp[q] * den[q] - 2.0 * mu);
+    if (edge && vs[q] / vp[q] < 0.01) { // water at the model edges
+        mu = (T)(vp[q] * vp[q] / 3.0 * den[q]);
+        lam = (T)(vp[q] * vp[q] * den[q] - 2.0 * mu);
+    }
+}
+
+template <class T>
+__global__ void k_psv_material(const T *__restrict__ vp, const T *__restrict__ vs, const T *__restrict__ den, int nx,
+                               int nz, int px, T fac, T *BU, T *BW, T *MU, T *MUA, T *LAM,
+                               unsigned long long *maxk) {
+    const long N = (long)nx * nz;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double vmax = -1e300, bmax = 0.0;
+    if (q < N) {
+        const int iz = (int)(q / nx), ix = (int)(q - (long)iz * nx);
+        auto edge = [&](int x, int z) { return x < 5 || x >= nx - 5 || z >= nz - 5; };
+        T mu, lam;
+        lame(vp, vs, den, q, edge(ix, iz), mu, lam);
+        T bu = 0, bw = 0, mua = 0;
+        if (iz >= 1) {
+            bu = (T)(2.0 / ((double)den[q] + den[q + 1 < N ? q + 1 : q]));
+            bw = (T)(2.0 / ((double)den[q] + den[q + nx < N ? q - nx : q]));
+            if (ix + 1 < nx && iz + 1 < nz) {
+                T m2, m3, m4, l_;
+                lame(vp, vs, den, q + 1, edge(ix + 1, iz), m2, l_);
+                lame(vp, vs, den, q - nx, edge(ix, iz - 1), m3, l_);
+                lame(vp, vs, den, q + 1 - nx, edge(ix + 1, iz - 1), m4, l_);
+                const double d1 = mu, d2 = m2, d3 = m3, d4 = m4;
+                const double dd = fmax(d2 * d3 * d4 + d1 * d3 * d4 + d1 * d2 * d4 + d2 * d3 * d4, 1E-10);
+                mua = (T)(4.0 * d1 * d2 * d3 * d4 / dd);
+            } else {
+                mua = mu;
+            }
+        } else if (ix >= 1 && ix < nx - 1) { // free surface row
+            bu = (T)(4.0 / ((double)den[ix] + den[ix + 1]));
+            lam = 0;
+            mu = (T)(0.5 * mu);
+        }
+        const long o = (long)iz * px + ix;
+        BU[o] = bu * fac;
+        BW[o] = bw * fac;
+        MU[o] = mu * fac;
+        MUA[o] = mua * fac;
+        LAM[o] = lam * fac;
+        vmax = (double)vp[q];
+        bmax = fmax(fabs((double)(bu * fac)), fabs((double)(bw * fac)));
+    }
+    wave_max_to(&maxk[0], vmax);
+    wave_max_to(&maxk[1], bmax);
+}
+
+__global__ void k_sh_material(const float *__restrict__ vs, const float *__restrict__ den, int nx, int nz, int px,
+                              float fac, float *BV, float *MUX, float *MUZ, unsigned long long *maxk) {
+    const long N = (long)nx * nz;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double vmax = -1e300, bmax = 0.0;
+    if (q < N) {
+        const int iz = (int)(q / nx), ix = (int)(q - (long)iz * nx);
+        auto mu_of = [&](long k) -> double { return vs[k] * vs[k] * den[k]; };
+        const double m1 = mu_of(q);
+        double m2 = q + 1 < N ? mu_of(q + 1) : m1;
+        const float mux = (float)(2.0 * m1 * m2 / fmax(m1 + m2, 1E-10));
+        m2 = q >= nx ? mu_of(q - nx) : m1;
+        const float muz = (float)(2.0 * m1 * m2 / fmax(m1 + m2, 1E-10));
+        const float bv = (float)(1.0 / den[q]) * fac;
+        const long o = (long)iz * px + ix;
+        BV[o] = bv;
+        MUX[o] = mux * fac;
+        MUZ[o] = muz * fac;
+        vmax = (double)vs[q];
+        bmax = fabs((double)bv);
+    }
+    wave_max_to(&maxk[0], vmax);
+    wave_max_to(&maxk[1], bmax);
+}
+
+const double kCoefSum = 1225. / 1024. + 245. / 3072. + 49. / 5120. + 5. / 7168.;
+
+} // namespace
+
+// vp, vs, den: nz*nx host arrays (double for an FP64 P-SV context, float
+// otherwise; SH reads vs and den only).
+extern "C" int fd23_set_model(fd23_ctx *c, const void *vp, const void *vs, const void *den, double h, double dt,
+                              double *stab) {
+    if (!c || !vs || !den) return merr(FD23_EINVAL, "set_model: null argument");
+    const fd23_desc &d = fd23_ctx_desc(c);
+    const bool psv = d.wave == FD23_WAVE_PSV;
+    if (psv && !vp) return merr(FD23_EINVAL, "set_model: null vp");
+    if (d.nx < 2 || d.nz < 2) return merr(FD23_EINVAL, "set_model: grid too small");
+    const bool f64 = psv && d.storage == FD23_STORE_F64;
+    if (!psv && d.storage == FD23_STORE_F64)
+        return merr(FD23_EINVAL, "set_model: SH material is computed in float; pass it with fd23_set_material");
+    const size_t N = (size_t)d.nx * d.nz, es = f64 ? 8 : 4;
+    const int nin = psv ? 3 : 2;
+    hipStream_t st = fd23_ctx_stream(c);
+    // one staging allocation: the two 64-bit reduction keys first (8-byte
+    // aligned, as 64-bit atomics need), then the inputs from byte 256
+    const size_t kIn = 256;
+    char *buf = nullptr;
+    MCHK(hipMalloc((void **)&buf, kIn + nin * N * es));
+    unsigned long long *keys = (unsigned long long *)buf;
+    const void *src[3] = {psv ? vp : vs, psv ? vs : den, den};
+    for (int i = 0; i < nin; i++) {
+        hipError_t e = hipMemcpyAsync(buf + kIn + i * N * es, src[i], N * es, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) {
+            hipStreamSynchronize(st);
+            hipFree(buf);
+            return merr(FD23_EHIP, std::string("set_model copy: ") + hipGetErrorString(e));
+        }
+    }
+    // 0 is below every key (the key of -NaN): the maxima start empty
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, keys, 2, 0ull);
+    const int block = 256;
+    const int grid = (int)((N + block - 1) / block);
+    const int px = fd23_ctx_px(c);
+    if (psv && f64) {
+        const double *in = (const double *)(buf + kIn);
+        hipLaunchKernelGGL(k_psv_material<double>, dim3(grid), dim3(block), 0, st, in, in + N, in + 2 * N, d.nx,
+                           d.nz, px, dt / h, (double *)fd23_ctx_material(c, 0), (double *)fd23_ctx_material(c, 1),
+                           (double *)fd23_ctx_material(c, 2), (double *)fd23_ctx_material(c, 3),
+                           (double *)fd23_ctx_material(c, 4), keys);
+    } else if (psv) {
+        const float *in = (const float *)(buf + kIn);
+        const float ff = (float)dt / (float)h;
+        hipLaunchKernelGGL(k_psv_material<float>, dim3(grid), dim3(block), 0, st, in, in + N, in + 2 * N, d.nx, d.nz,
+                           px, ff, (float *)fd23_ctx_material(c, 0), (float *)fd23_ctx_material(c, 1),
+                           (float *)fd23_ctx_material(c, 2), (float *)fd23_ctx_material(c, 3),
+                           (float *)fd23_ctx_material(c, 4), keys);
+    } else {
+        const float *in = (const float *)(buf + kIn);
+        const float ff = (float)dt / (float)h;
+        hipLaunchKernelGGL(k_sh_material, dim3(grid), dim3(block), 0, st, in, in + N, d.nx, d.nz, px, ff,
+                           (float *)fd23_ctx_material(c, 0), (float *)fd23_ctx_material(c, 1),
+                           (float *)fd23_ctx_material(c, 2), keys);
+    }
+    unsigned long long hk[2] = {0, 0};
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(hk, keys, sizeof hk, hipMemcpyDeviceToHost, st);
+    const hipError_t es_ = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = es_;
+    hipFree(buf);
+    if (e != hipSuccess) return merr(FD23_EHIP, std::string("set_model: ") + hipGetErrorString(e));
+    const double vmax = std::max(order_value(hk[0]), -99999.0), bmax = order_value(hk[1]);
+    fd23_ctx_material_ready(c, bmax);
+    // stability number as the host precompute forms it
+    const double s = psv && f64 ? vmax * dt * kCoefSum * std::sqrt(2.0) / h
+                                : vmax * (double)(float)dt * kCoefSum * std::sqrt(2.0) / (double)(float)h;
+    if (stab) *stab = s;
+    if (s > 1) return merr(FD23_ESTAB, "model stability number " + std::to_string(s) + " > 1");
+    return FD23_OK;
+}

@@ -43,8 +43,9 @@ double seconds(Clock::time_point a, Clock::time_point b) { return std::chrono::d
 struct Request {
     long long id = 0;
     fd23_request r{};             // scalar fields; pointers re-aimed at the copies below
-    std::vector<std::vector<char>> mat;
+    std::vector<std::vector<char>> mat; // material arrays, or vp vs den with from_model
     std::vector<const void *> matp;
+    bool from_model = false;
     std::vector<double> cpml, stf, M;
     std::vector<int> xs, zs, rx, rz;
     int ncomp = 1, nframes = 0, storage = 0;
@@ -141,7 +142,13 @@ int fd23_server::start_group(Group &g) {
     fd23_desc d = make_desc(q0, g.nshots, streams[g.slot]);
     int r;
     if ((r = fd23_ctx_create(&d, &g.ctx))) return r;
-    if ((r = fd23_set_material(g.ctx, q0.matp.data()))) return r;
+    if (q0.from_model) {
+        double stab;
+        r = fd23_set_model(g.ctx, q0.matp[0], q0.matp[1], q0.matp[2], q0.r.dx, q0.r.dt, &stab);
+    } else {
+        r = fd23_set_material(g.ctx, q0.matp.data());
+    }
+    if (r) return r;
     if ((r = fd23_set_cpml(g.ctx, q0.cpml.data()))) return r;
     int shot = 0;
     for (auto &q : g.reqs) {
@@ -459,7 +466,7 @@ extern "C" int fd23_server_submit(fd23_server *s, const fd23_request *r, long lo
     const bool psv = r->wave == FD23_WAVE_PSV;
     if ((r->wave != FD23_WAVE_PSV && r->wave != FD23_WAVE_SH) || r->storage < FD23_STORE_F32 ||
         r->storage > FD23_STORE_F64 || r->nshots < 1 || r->nt < 1 || r->itrecord < 1 || r->nrec < 0 ||
-        !r->material || !r->cpml || (r->src_kind >= 0 && (!r->xs || !r->zs || !r->M || r->lsrc < 0 ||
+        (!r->material && !r->model) || !r->cpml || (r->src_kind >= 0 && (!r->xs || !r->zs || !r->M || r->lsrc < 0 ||
                                                           (r->lsrc > 0 && !r->stf))) ||
         (r->nrec > 0 && (!r->rx || !r->rz)) || (r->box && !psv)) {
         fd23_set_error_("bad request");
@@ -474,20 +481,39 @@ extern "C" int fd23_server_submit(fd23_server *s, const fd23_request *r, long lo
     q->csize = q->storage == FD23_STORE_F64 ? 8 : 4;
     const size_t n = (size_t)r->nx * r->nz;
     const size_t msize = r->storage == FD23_STORE_F64 ? 8 : 4;
-    const int nm = material_count(r->wave);
-    for (int i = 0; i < nm; i++) {
-        if (!r->material[i]) {
-            fd23_set_error_("null material array");
+    q->from_model = !r->material;
+    if (q->from_model) { // vp vs den: SH leaves vp empty
+        if ((!r->model[1] || !r->model[2]) || (psv && !r->model[0])) {
+            fd23_set_error_("null model array");
             return FD23_EINVAL;
         }
-        const char *p = (const char *)r->material[i];
-        q->mat.emplace_back(p, p + n * msize);
+        if (!psv && r->storage == FD23_STORE_F64) {
+            fd23_set_error_("SH material from the model is computed in float: pass the material");
+            return FD23_EINVAL;
+        }
+        for (int i = 0; i < 3; i++) {
+            const char *p = (const char *)r->model[i];
+            if (p)
+                q->mat.emplace_back(p, p + n * msize);
+            else
+                q->mat.emplace_back();
+        }
+    } else {
+        const int nm = material_count(r->wave);
+        for (int i = 0; i < nm; i++) {
+            if (!r->material[i]) {
+                fd23_set_error_("null material array");
+                return FD23_EINVAL;
+            }
+            const char *p = (const char *)r->material[i];
+            q->mat.emplace_back(p, p + n * msize);
+        }
     }
     if (msize == 8 && q->csize == 4) {
         fd23_set_error_("FP64 material needs FP64 storage");
         return FD23_EINVAL;
     }
-    for (auto &m : q->mat) q->matp.push_back(m.data());
+    for (auto &m : q->mat) q->matp.push_back(m.empty() ? nullptr : m.data());
     q->cpml.assign(r->cpml, r->cpml + fd23_cpml_pack_len(r->wave, r->nx, r->nz));
     if (r->src_kind >= 0) {
         q->xs.assign(r->xs, r->xs + r->nshots);
@@ -500,6 +526,7 @@ extern "C" int fd23_server_submit(fd23_server *s, const fd23_request *r, long lo
         q->rz.assign(r->rz, r->rz + r->nrec);
     }
     q->r.material = nullptr;
+    q->r.model = nullptr;
     q->r.cpml = nullptr;
     q->r.xs = q->r.zs = q->r.rx = q->r.rz = nullptr;
     q->r.stf = q->r.M = nullptr;

@@ -150,7 +150,19 @@ struct Grid {
     int nshots;
     long xpsi_stride;   // elements between shots in an x-psi array (2*npml*nz)
     long zpsi_stride;   // ... in a z-psi array (2*npml*nx)
+    // first-arrival window (null: every cell is updated every step): per
+    // block of 32 x 16 cells, a cell is updated at step it only when
+    // pfirst < it < plast of its block (SH/fd/cu_fd.cu:18-25)
+    const int *pfirst, *plast;
+    int pnxb;           // blocks per row, nx / 32
 };
+
+constexpr int kPunchBX = 32, kPunchBZ = 16; // block of the window, SH/param/const.h:20-21
+// ix < nx and iz < nz (nx and nz are multiples of the block)
+__device__ __forceinline__ bool punch_open(const Grid &g, int ix, int iz, int it) {
+    const int b = (int)((unsigned)iz / kPunchBZ) * g.pnxb + (int)((unsigned)ix / kPunchBX);
+    return g.pfirst[b] < it && it < g.plast[b];
+}
 
 template <class S> struct PsvFields {
     S *U, *W, *Txx, *Tzz, *Txz; // point at shot 0, row 0, column 0

@@ -146,6 +146,48 @@ struct Events {
     long frame_size;
 };
 
+// first-arrival window (the graph loop applies it: the step index is known
+// there only).  Bit r of rows_open is set when row r of the thread's RZ
+// rows is updated this step; rows off the grid are clear.
+__device__ __forceinline__ bool punch_on(const Grid &g, const Events &ev) {
+    return g.pfirst != nullptr && ev.clock != nullptr;
+}
+template <int RZ>
+__device__ __forceinline__ unsigned rows_open(const Grid &g, bool pun, int gx, int gz0, int it) {
+    if (!pun) return ~0u;
+    unsigned m = 0;
+#pragma unroll
+    for (int r = 0; r < RZ; r++)
+        if (gx < g.nx && gz0 + r < g.nz && punch_open(g, gx, gz0 + r, it)) m |= 1u << r;
+    return m;
+}
+
+// what a workgroup does under a window (uniform per workgroup): nothing when
+// every block its tile covers is closed and no shot of its group has a source
+// tap or a receiver in the tile; the interior path when every block is open;
+// else the exact path with the per-row mask
+enum { kTileIdle = 0, kTileMixed = 1, kTileOpen = 2 };
+template <int TZ, int SPB>
+__device__ __forceinline__ int tile_window(const Grid &g, const Events &ev, const TileId &t, int it) {
+    const int bx1 = (min(t.x0 + TX, g.nx) - 1) / kPunchBX, bz1 = (min(t.z0 + TZ, g.nz) - 1) / kPunchBZ;
+    int nopen = 0, nblk = 0;
+    for (int bz = t.z0 / kPunchBZ; bz <= bz1; bz++)
+        for (int bx = t.x0 / kPunchBX; bx <= bx1; bx++) {
+            const int b = bz * g.pnxb + bx;
+            nopen += g.pfirst[b] < it && it < g.plast[b];
+            nblk++;
+        }
+    if (nopen == nblk) return kTileOpen;
+    if (nopen > 0) return kTileMixed;
+    const int s1 = min(t.group * SPB + SPB, g.nshots);
+    for (int s = t.group * SPB; s < s1; s++) {
+        const int key = t.tile * g.nshots + s;
+        if (ev.tap_start[key + 1] > ev.tap_start[key] || ev.rec_start[key + 1] > ev.rec_start[key])
+            return kTileMixed;
+    }
+    return kTileIdle;
+}
+
 // source sample of a shot at step it, or false when inactive (it >= lsrc
 // without clamping), PSV/clean_nobox/point_source/cu_point_source.cu:47,110-112
 __device__ __forceinline__ bool src_sample(const SrcInfo &si, const double *stf, int it, double &sv) {
@@ -329,10 +371,19 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
     const int nx = g.nx, nz = g.nz, px = g.px;
     const int gx = x0 + tx;
     const int lz0 = ty * RZ;
-    const bool interior = REGION == REGION_INTERIOR  ? true
-                          : REGION == REGION_EDGE    ? false
-                                                     : tile_interior(g, x0, z0, TZ);
+    bool interior = REGION == REGION_INTERIOR  ? true
+                    : REGION == REGION_EDGE    ? false
+                                               : tile_interior(g, x0, z0, TZ);
     const int it = ev.clock ? ev.clock[0] + ev.k : 0;
+    // first-arrival window: the interior-only kernel is never launched with
+    // one; a tile with a closed block takes the exact path, which masks rows
+    const bool pun = REGION != REGION_INTERIOR && punch_on(g, ev); // uniform
+    if (pun) {
+        const int tw = tile_window<TZ, SPB>(g, ev, t, it);
+        if (tw == kTileIdle) return;
+        interior = interior && tw == kTileOpen;
+    }
+    const unsigned upd = rows_open<RZ>(g, pun, gx, z0 + lz0, it);
 
     // stored arrays are read in the current scales scl[0..15] and the
     // updated ones written in the next scales scl[16..31] (equal except
@@ -405,7 +456,7 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
             for (int r = 0; r < RZ; r++) {
                 const int lz = lz0 + r;
                 const int ix = gx, iz = z0 + lz;
-                if (ix >= nx || iz >= nz) continue;
+                if (ix >= nx || iz >= nz || !(upd >> r & 1u)) continue;
                 C c0, c1, c2, c3;
                 Coef<C>::get(order_at(ix, iz, nx, nz), c0, c1, c2, c3);
                 const C *pu = &sU[(lz + HALO) * LW + tx + HALO];
@@ -577,10 +628,19 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
     const int nx = g.nx, nz = g.nz, px = g.px;
     const int gx = x0 + tx;
     const int lz0 = ty * RZ;
-    const bool interior = REGION == REGION_INTERIOR  ? true
-                          : REGION == REGION_EDGE    ? false
-                                                     : tile_interior(g, x0, z0, TZ);
+    bool interior = REGION == REGION_INTERIOR  ? true
+                    : REGION == REGION_EDGE    ? false
+                                               : tile_interior(g, x0, z0, TZ);
     const int it = ev.clock ? ev.clock[0] + ev.k : 0;
+    // first-arrival window: the interior-only kernel is never launched with
+    // one; a tile with a closed block takes the exact path, which masks rows
+    const bool pun = REGION != REGION_INTERIOR && punch_on(g, ev); // uniform
+    if (pun) {
+        const int tw = tile_window<TZ, SPB>(g, ev, t, it);
+        if (tw == kTileIdle) return;
+        interior = interior && tw == kTileOpen;
+    }
+    const unsigned upd = rows_open<RZ>(g, pun, gx, z0 + lz0, it);
 
     float sc_u = 1.f, sc_w = 1.f, sc_xx = 1.f, sc_zz = 1.f, sc_xz = 1.f;
     float wi_u = 1.f, wi_w = 1.f;
@@ -649,7 +709,7 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
             for (int r = 0; r < RZ; r++) {
                 const int lz = lz0 + r;
                 const int ix = gx, iz = z0 + lz;
-                if (ix >= nx || iz >= nz) continue;
+                if (ix >= nx || iz >= nz || !(upd >> r & 1u)) continue;
                 C c0, c1, c2, c3;
                 Coef<C>::get(order_at(ix, iz, nx, nz), c0, c1, c2, c3);
                 const int o = (lz + HALO) * LW + tx + HALO;
@@ -719,8 +779,17 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
     const int nx = g.nx, nz = g.nz, px = g.px, npml = g.npml;
     const int gx = x0 + tx;
     const int lz0 = ty * RZ;
-    const bool interior = tile_interior(g, x0, z0, TZ);
+    bool interior = tile_interior(g, x0, z0, TZ);
     const int it = ev.clock ? ev.clock[0] + ev.k : 0;
+    // first-arrival window: a tile with a closed block takes the exact
+    // path, which masks rows
+    const bool pun = punch_on(g, ev); // uniform
+    if (pun) {
+        const int tw = tile_window<TZ, SPB>(g, ev, t, it);
+        if (tw == kTileIdle) return;
+        interior = interior && tw == kTileOpen;
+    }
+    const unsigned upd = rows_open<RZ>(g, pun, gx, z0 + lz0, it);
 
     float sc_v = 1.f, sc_x = 1.f, sc_z = 1.f, wi_x = 1.f, wi_z = 1.f;
     if (ST::scaled) {
@@ -779,7 +848,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
                 const int lz = lz0 + r;
                 const int ix = gx, iz = z0 + lz;
                 // early return of SH/fd/cu_fd.cu:140
-                if (ix >= nx || iz >= nz || ix == 0 || ix >= nx - 1 || iz >= nz - 1) continue;
+                if (ix >= nx || iz >= nz || ix == 0 || ix >= nx - 1 || iz >= nz - 1 || !(upd >> r & 1u)) continue;
                 C c0, c1, c2, c3;
                 Coef<C>::get(order_at(ix, iz, nx, nz), c0, c1, c2, c3);
                 const C *pv = &sV[(lz + HALO) * LW + tx + HALO];
@@ -872,8 +941,19 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
     const int nx = g.nx, nz = g.nz, px = g.px, npml = g.npml;
     const int gx = x0 + tx;
     const int lz0 = ty * RZ;
-    const bool interior = tile_interior(g, x0, z0, TZ);
+    bool interior = tile_interior(g, x0, z0, TZ);
     const int it = ev.clock ? ev.clock[0] + ev.k : 0;
+    // first-arrival window: a tile with a closed block takes the exact
+    // path, which masks rows
+    const bool pun = punch_on(g, ev); // uniform
+    if (pun) {
+        const int tw = tile_window<TZ, SPB>(g, ev, t, it);
+        if (tw == kTileIdle) return;
+        interior = interior && tw == kTileOpen;
+    }
+    const unsigned upd = rows_open<RZ>(g, pun, gx, z0 + lz0, it);
+    // the surface imaging of column gx runs when its row-0 block is open
+    const bool img_open = (rows_open<1>(g, pun, gx, 0, it) & 1u) != 0;
 
     float sc_v = 1.f, sc_x = 1.f, sc_z = 1.f, wi_v = 1.f;
     if (ST::scaled) {
@@ -907,13 +987,13 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
         // HALO - z0) takes -Tz[row 1], and the tile owning row 0 writes it back
         const bool img = z0 - HALO <= 0 && z0 + TZ + HALO > 0;
         C img_v = C(0);
-        if (img && ty == 0) img_v = -ST::ld(f.Tz, base + (long)px + gx, sc_z);
+        if (img && ty == 0 && img_open) img_v = -ST::ld(f.Tz, base + (long)px + gx, sc_z);
         if (s > 0) __syncthreads();
         rX.store(sX);
         rZ.store(sZ);
         if (img) {
             __syncthreads();
-            if (ty == 0) {
+            if (ty == 0 && img_open) {
                 sZ[(HALO - z0) * LW + tx + HALO] = img_v;
                 if (z0 == 0 && gx < nx) ST::st(f.Tz, base + gx, img_v, inv_pow2(sc_z));
             }
@@ -959,7 +1039,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
             for (int r = 0; r < RZ; r++) {
                 const int lz = lz0 + r;
                 const int ix = gx, iz = z0 + lz;
-                if (ix >= nx || iz >= nz) continue;
+                if (ix >= nx || iz >= nz || !(upd >> r & 1u)) continue;
                 C c0, c1, c2, c3;
                 Coef<C>::get(order_at(ix, iz, nx, nz), c0, c1, c2, c3);
                 const int o = (lz + HALO) * LW + tx + HALO;

@@ -38,7 +38,7 @@ class Solve:
     nt: int
     dx: float
     dt: float
-    material: list                 # scaled arrays (fd23_set_material order)
+    material: list                 # scaled arrays (fd23_set_material order), or None with model
     cpml: np.ndarray               # fd23_cpml_pack layout
     xs: np.ndarray
     zs: np.ndarray
@@ -52,6 +52,7 @@ class Solve:
     box: dict | None = None        # x0 z0 bx bz incident angle f0 t0 amp vp vs rho
     model_key: int = 0
     tag: str = ""
+    model: list | None = None      # vp, vs, den: the engine computes the material on the device
 
     @property
     def nshots(self):
@@ -96,8 +97,16 @@ class Server:
     def submit(self, s: Solve) -> int:
         f64 = s.storage == _lib.STORE_F64
         mdt = np.float64 if f64 else np.float32
-        mats = [np.ascontiguousarray(m, mdt) for m in s.material]
-        ptrs = (ctypes.c_void_p * len(mats))(*[m.ctypes.data for m in mats])
+        if s.material is not None:
+            mats = [np.ascontiguousarray(m, mdt) for m in s.material]
+            ptrs = ctypes.cast((ctypes.c_void_p * len(mats))(*[m.ctypes.data for m in mats]),
+                               ctypes.POINTER(ctypes.c_void_p))
+            mptrs = None
+        else:
+            mats = [None if m is None else np.ascontiguousarray(m, mdt) for m in s.model]
+            mptrs = ctypes.cast((ctypes.c_void_p * 3)(*[None if m is None else m.ctypes.data for m in mats]),
+                                ctypes.POINTER(ctypes.c_void_p))
+            ptrs = None
         cp = np.ascontiguousarray(s.cpml, np.float64)
         xs = np.ascontiguousarray(s.xs, np.int32)
         zs = np.ascontiguousarray(s.zs, np.int32)
@@ -108,11 +117,11 @@ class Server:
         b = s.box or {}
         r = _lib.Request(
             s.wave, s.storage, s.nx, s.nz, s.npml, int(bool(s.topabsorb)), s.nshots, s.nt, s.itrecord,
-            s.dx, s.dt, ctypes.cast(ptrs, ctypes.POINTER(ctypes.c_void_p)), _vp(cp), s.src_kind, _vp(xs), _vp(zs),
+            s.dx, s.dt, ptrs, _vp(cp), s.src_kind, _vp(xs), _vp(zs),
             _vp(stf), stf.size, _vp(M), rx.size, _vp(rx), _vp(rz), int(bool(s.box)), b.get("x0", 0),
             b.get("z0", 0), b.get("bx", 0), b.get("bz", 0), b.get("incident", 0), b.get("angle", 0.0),
             b.get("f0", 0.0), b.get("t0", 0.0), b.get("amp", 0.0), b.get("vp", 0.0), b.get("vs", 0.0),
-            b.get("rho", 0.0), int(s.model_key))
+            b.get("rho", 0.0), int(s.model_key), mptrs)
         rid = ctypes.c_longlong()
         check(self.lib.fd23_server_submit(self.h, ctypes.byref(r), ctypes.byref(rid)))
         # FP16/BF16 requests with a box run at FP32 storage
@@ -151,21 +160,26 @@ GRIDS = ((256, 128), (512, 256), (1024, 512), (2048, 1024))
 
 
 def make_solve(wave, storage, nx, nz, dx, nt, nshots, seed, npml=20, f0=10.0, box=None, model_key=None,
-               cfl=0.6, tag=""):
+               cfl=0.6, tag="", device_material=False):
     """A request on a layered+GRF model (the batched-config recipe) with a
     CFL-safe time step dt = cfl*dx/(vpmax*sum|c|*sqrt 2), Ricker f0 sources
-    at z=8 spread between the C-PML strips, receiver pairs at z=4."""
+    at z=8 spread between the C-PML strips, receiver pairs at z=4.  With
+    device_material the request carries vp, vs, den and the engine computes
+    the material on the device."""
     from .workloads import layered_grf_model
 
     vp, vs, den = layered_grf_model(nx, nz, dx, seed)
     vpmax = float(vp.max())
     dt = cfl * dx / (vpmax * SIGMA_C * np.sqrt(2.0))
+    mat = None
     if wave == _lib.WAVE_PSV:
-        mat, _ = host.psv_material(vp, vs, den, dx, dt)
+        if not device_material:
+            mat, _ = host.psv_material(vp, vs, den, dx, dt)
         M = host.psv_moment(_lib.SRC_LINE, 0.0, 0.0, 0.0, 0.0, dx, dt, explosion=True)
         kind = _lib.SRC_LINE
     else:
-        mat, _ = host.sh_material(vs, den, dx, dt)
+        if not device_material:
+            mat, _ = host.sh_material(vs, den, dx, dt)
         M = host.sh_moment(_lib.SRC_LINE, 0.0, 90.0, 0.0, 0.0, dx, dt)
         kind = _lib.SRC_FORCE
     packed = host.cpml_pack(wave, nx, nz, npml, dt, 1e-6, vpmax / dx, f0)
@@ -183,8 +197,9 @@ def make_solve(wave, storage, nx, nz, dx, nt, nshots, seed, npml=20, f0=10.0, bo
         box = dict(box, vp=float(vp[sl].mean()), vs=float(vs[sl].mean()), rho=float(den[sl].mean()))
         box.setdefault("t0", 1.2 / box["f0"])
         box.setdefault("amp", 1.0)
-    return Solve(wave, storage, nx, nz, npml, nt, dx, dt, list(mat), packed, xs, zs, stf, M, rx, rz, kind,
-                 box=box, model_key=seed if model_key is None else model_key, tag=tag)
+    model = [vp if wave == _lib.WAVE_PSV else None, vs, den] if device_material else None
+    return Solve(wave, storage, nx, nz, npml, nt, dx, dt, None if mat is None else list(mat), packed, xs, zs, stf,
+                 M, rx, rz, kind, box=box, model_key=seed if model_key is None else model_key, tag=tag, model=model)
 
 
 def serving_stream(n, seed=5005, rate=40.0, storages=(_lib.STORE_F32, _lib.STORE_F16)):

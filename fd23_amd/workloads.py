@@ -83,6 +83,10 @@ class Workload:
     M: np.ndarray = field(default=None, repr=False)
     stab: float = 0.0
     vpmax: float = 0.0
+    # first-arrival window (None: none): seconds before the straight-ray
+    # first arrival at vpmax and after it
+    punch_before: float = None
+    punch_after: float = None
 
     @property
     def nshots(self):
@@ -137,7 +141,27 @@ class Workload:
                            self.M)
             for c in range(ncomp):
                 ctx.set_receivers(s, c, rx, rz)
+        if self.punch_after is not None:
+            ctx.set_punch(self.travel_times(), self.punch_before, self.punch_after, self.dt)
         return ctx
+
+    def travel_times(self):
+        """Straight-ray first-arrival times (s) from shot 0's source at the
+        fastest speed of the model, nz x nx float32."""
+        xs, zs = self.sources[0]
+        z, x = np.mgrid[0:self.nz, 0:self.nx]
+        return (np.hypot(x - xs, z - zs) * self.dx / self.vpmax).astype(np.float32)
+
+    def punch_active_fraction(self, it0, nt):
+        """Fraction of (32x16 block, step) pairs inside the window over
+        steps [it0, it0+nt): the share of cell updates the passes perform."""
+        t = self.travel_times().reshape(self.nz // 16, 16, self.nx // 32, 32)
+        f32 = np.float32
+        first = ((t.min(axis=(1, 3)) - f32(self.punch_before)) / f32(self.dt)).astype(np.int64)
+        last = ((t.max(axis=(1, 3)) + f32(self.punch_after)) / f32(self.dt)).astype(np.int64)
+        lo = np.maximum(first + 1, it0)
+        hi = np.minimum(last - 1, it0 + nt - 1)
+        return float(np.clip(hi - lo + 1, 0, None).sum() / (first.size * nt))
 
     def cell_updates(self, nt=None):
         return float(self.nshots) * self.nx * self.nz * (self.nt if nt is None else nt)
@@ -174,3 +198,14 @@ def config_psv_precision(storage=_lib.STORE_F16, nshots=8):
     return Workload("psv_4096x2048_x%d" % nshots, _lib.WAVE_PSV, 4096, 2048, 4.0, 2.5e-4, 6000, 32, False,
                     20.0, [(x, 8) for x in xs], rec_x, np.full(rec_x.size, 4, np.int32), explosion=True,
                     seed=4004, storage=storage)
+
+
+def config_psv_punch():
+    """P-SV FP32 with a first-arrival window: the 2000x1000 configuration's
+    physics on 2048x1024 (block multiples), 5000 steps; blocks update from
+    20 ms before the straight-ray first arrival at the fastest P speed to
+    300 ms after it."""
+    rec_x = np.arange(1, 2048, 2, dtype=np.int32)
+    return Workload("psv_2048x1024_punch_fp32", _lib.WAVE_PSV, 2048, 1024, 5.0, 3e-4, 5000, 32, False, 15.0,
+                    [(1024, 10)], rec_x, np.full(rec_x.size, 5, np.int32), seed=2002, punch_before=0.02,
+                    punch_after=0.3)

@@ -90,6 +90,20 @@ size_t fd23_desc_device_bytes(const fd23_desc *desc);
  * BU, BW, MU, MUA, LAM; SH order BV, MUX, MUZ.  Each nx*nz, x fastest,
  * float (or double for FD23_STORE_F64).  Replaces init_gpu_full's H2D.  */
 int fd23_set_material(fd23_ctx *ctx, const void *const *arrays);
+/* The material computed on the device from the model: vp, vs, den nz*nx
+ * (double for an FP64 P-SV context, float otherwise; SH reads vs and den
+ * and takes FP32, FP16 or BF16 storage).  The arrays equal those of
+ * fd23_psv_material[_f64] / fd23_sh_material bitwise, with the same
+ * stability number (FD23_ESTAB above 1, the material still set); one
+ * pointwise kernel replaces the host pass over the grid.  Replaces
+ * MATERIAL::get_model + init_gpu_full (PSV/clean_nobox/material/
+ * material.cpp:35-240, cu_material.cu:8-50; SH/material/material.cpp:
+ * 31-135).                                                              */
+int fd23_set_model(fd23_ctx *ctx, const void *vp, const void *vs, const void *den, double h, double dt,
+                   double *stab);
+/* One material array (index in the fd23_set_material order) copied back
+ * as nz*nx values (float, or double for F64); no reference counterpart. */
+int fd23_get_material(fd23_ctx *ctx, int index, void *out);
 
 /* C-PML profiles as b, c, k(=1/kappa) triples, double, packed in order
  * P-SV: Txx_x, Txz_x, U_x, W_x (nx each), Tzz_z, Txz_z, U_z, W_z (nz each);
@@ -126,6 +140,21 @@ int fd23_record(fd23_ctx *ctx, int it, int itrecord);
  * Equivalent to the singlegpu() loop body (main.cpp:115-168).           */
 int fd23_run(fd23_ctx *ctx, int it0, int nt, int itrecord);
 int fd23_sync(fd23_ctx *ctx);
+
+/* First-arrival window ("punch", usepunch=1): times is the nz*nx float
+ * travel-time grid (row-major, the timefile layout); per block of 32x16
+ * cells, first = (int)((min t - before)/dt), last = (int)((max t +
+ * after)/dt), in float, and both passes update the block's cells only
+ * at steps first < it < last; sources and receivers are unaffected.
+ * Replaces SLIDE::SLIDE (SH/slide/cu_slide.cu:7-61) and the kernels'
+ * CHECK_IF_NEED_UPDATE (SH/fd/cu_fd.cu:18-25, PSV/psv_clean_double/fd/
+ * cu_fd.cu:16-23).  Needs nx % 32 == 0, nz % 16 == 0 and FP32 or FP64
+ * storage (else FD23_EINVAL); applies to fd23_run (the staged entry
+ * points return FD23_EINVAL while a window is set).  times = NULL
+ * removes the window.  Workgroups whose blocks are all outside their
+ * window and hold no source tap or receiver return at once.           */
+int fd23_set_punch(fd23_ctx *ctx, const float *times, float before,
+                   float after, float dt);
 
 /* ---- outputs ---------------------------------------------------------
  * Seismograms of one shot/component, time-major [nframes][n] floats

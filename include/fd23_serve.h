@@ -66,6 +66,10 @@ typedef struct fd23_request {
     int box_x0, box_z0, box_bx, box_bz, box_incident;
     double box_angle, box_f0, box_t0, box_amp, box_vp, box_vs, box_rho;
     unsigned long long model_key; /* equal non-zero keys: same material     */
+    /* vp, vs, den (nx*nz, double for F64 P-SV, else float; SH reads vs and
+     * den) in place of material (which is then NULL): the engine computes
+     * the material on the device at admission (fd23_set_model) with dx, dt */
+    const void *const *model;
 } fd23_request;
 
 #define FD23_REQ_QUEUED 0

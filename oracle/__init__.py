@@ -17,6 +17,8 @@ Reference files followed (paths relative to the reference root):
   loop order                 PSV/clean_nobox/main.cpp:115-168, SH/main.cpp:93-141
   state%06d layout           PSV/clean_nobox/main.cpp:253-318
   .mdl models                SH/model/model.cpp:175-469
+  first-arrival window       SH/slide/cu_slide.cu:7-61, SH/fd/cu_fd.cu:18-25,
+                             PSV/psv_clean_double/fd/cu_fd.cu:16-23, main.cpp:120
 """
 from __future__ import annotations
 
@@ -306,9 +308,65 @@ def mdl_raster(path, nx, nz, h, dtype=np.float32):
 
 
 # --------------------------------------------------------------------------
+# first-arrival window ("punch")
+# --------------------------------------------------------------------------
+PUNCH_BX, PUNCH_BZ = 32, 16
+
+
+def punch_window(times, before, after, dt):
+    """Per 32 x 16 block, the first and last step of the window,
+    SH/slide/cu_slide.cu:27-53: the block's min and max of the float32
+    travel-time grid (nz x nx), (min - before) / dt and (max + after) / dt
+    in float32, truncated toward zero."""
+    t = np.ascontiguousarray(times, np.float32)
+    nz, nx = t.shape
+    nzb, nxb = nz // PUNCH_BZ, nx // PUNCH_BX
+    blk = t[:nzb * PUNCH_BZ, :nxb * PUNCH_BX].reshape(nzb, PUNCH_BZ, nxb, PUNCH_BX)
+    lo = blk.min(axis=(1, 3))
+    hi = blk.max(axis=(1, 3))
+    f32 = np.float32
+    first = ((lo - f32(before)) / f32(dt)).astype(np.int32)
+    last = ((hi + f32(after)) / f32(dt)).astype(np.int32)
+    return first, last
+
+
+def punch_mask(first, last, it):
+    """Per cell, whether the passes update it at step it: first < it < last
+    of its block (the kernels return otherwise, SH/fd/cu_fd.cu:18-25)."""
+    open_b = (first < it) & (it < last)
+    return np.ascontiguousarray(np.repeat(np.repeat(open_b, PUNCH_BZ, axis=0), PUNCH_BX, axis=1)
+                                .astype(np.uint8))
+
+
+# --------------------------------------------------------------------------
 # stepping
 # --------------------------------------------------------------------------
-class PSV:
+class _Window:
+    """First-arrival window shared by the P-SV and SH state classes: with a
+    window set, each pass takes the step index it."""
+    window = None
+
+    def set_window(self, times, before, after, dt):
+        if times is None:
+            self.window = None
+            return
+        if self.nx % PUNCH_BX or self.nz % PUNCH_BZ:
+            raise ValueError("first-arrival window needs nx % 32 == 0 and nz % 16 == 0")
+        if np.shape(times) != (self.nz, self.nx):
+            raise ValueError(f"travel times {np.shape(times)} != (nz, nx)")
+        self.window = punch_window(times, before, after, dt)
+
+    def _open(self, it):
+        if self.window is None:
+            return None
+        if it is None:
+            raise ValueError("a first-arrival window is set: pass the step index")
+        m = punch_mask(*self.window, it)
+        self._mask = m
+        return m.ctypes.data_as(ctypes.c_void_p)
+
+
+class PSV(_Window):
     """State and passes of the P-SV loop at one precision."""
 
     def __init__(self, nx, nz, npml, mat, cpml, topabsorb=False, dtype=np.float32):
@@ -330,7 +388,7 @@ class PSV:
         for f in (self._stress, self._vel, self._inj, self._rec):
             f.restype = None
 
-    def stress(self):
+    def stress(self, it=None):
         names = ("U_x", "W_x", "U_z", "W_z")
         ps = [self.psi[n] for n in names]
         bs = [self.cpml[n][0] for n in names]
@@ -340,9 +398,9 @@ class PSV:
         self._stress(ctypes.c_int(self.nx), ctypes.c_int(self.nz), ctypes.c_int(self.npml),
                      ctypes.c_int(self.topabsorb), _p(self.U), _p(self.W), _p(self.Txx),
                      _p(self.Tzz), _p(self.Txz), _p(m["LAM"]), _p(m["MU"]), _p(m["MUA"]),
-                     _ptrs(ps), _ptrs(bs), _ptrs(cs), _ptrs(ks))
+                     _ptrs(ps), _ptrs(bs), _ptrs(cs), _ptrs(ks), self._open(it))
 
-    def velocity(self):
+    def velocity(self, it=None):
         names = ("Txx_x", "Txz_x", "Txz_z", "Tzz_z")
         ps = [self.psi[n] for n in names]
         bs = [self.cpml[n][0] for n in names]
@@ -352,7 +410,7 @@ class PSV:
         self._vel(ctypes.c_int(self.nx), ctypes.c_int(self.nz), ctypes.c_int(self.npml),
                   ctypes.c_int(self.topabsorb), _p(self.Txx), _p(self.Tzz), _p(self.Txz),
                   _p(self.U), _p(self.W), _p(m["BU"]), _p(m["BW"]),
-                  _ptrs(ps), _ptrs(bs), _ptrs(cs), _ptrs(ks))
+                  _ptrs(ps), _ptrs(bs), _ptrs(cs), _ptrs(ks), self._open(it))
 
     def inject(self, kind, it, src, M, xs, zs):
         src = np.ascontiguousarray(src, self.dtype)
@@ -392,7 +450,7 @@ class PSV:
             off += n
 
 
-class SH:
+class SH(_Window):
     def __init__(self, nx, nz, npml, mat, cpml, topabsorb=False, dtype=np.float32):
         self.nx, self.nz, self.npml = nx, nz, npml
         self.topabsorb = int(bool(topabsorb))
@@ -412,7 +470,7 @@ class SH:
         for f in (self._stress, self._vel, self._inj, self._rec):
             f.restype = None
 
-    def stress(self):
+    def stress(self, it=None):
         names = ("V_x", "V_z")
         m = self.mat
         self._stress(ctypes.c_int(self.nx), ctypes.c_int(self.nz), ctypes.c_int(self.npml),
@@ -420,9 +478,9 @@ class SH:
                      _p(m["MUX"]), _p(m["MUZ"]), _ptrs([self.psi[n] for n in names]),
                      _ptrs([self.cpml[n][0] for n in names]),
                      _ptrs([self.cpml[n][1] for n in names]),
-                     _ptrs([self.cpml[n][2] for n in names]))
+                     _ptrs([self.cpml[n][2] for n in names]), self._open(it))
 
-    def velocity(self):
+    def velocity(self, it=None):
         names = ("Tx_x", "Tz_z")
         m = self.mat
         self._vel(ctypes.c_int(self.nx), ctypes.c_int(self.nz), ctypes.c_int(self.npml),
@@ -430,7 +488,7 @@ class SH:
                   _p(m["BV"]), _ptrs([self.psi[n] for n in names]),
                   _ptrs([self.cpml[n][0] for n in names]),
                   _ptrs([self.cpml[n][1] for n in names]),
-                  _ptrs([self.cpml[n][2] for n in names]))
+                  _ptrs([self.cpml[n][2] for n in names]), self._open(it))
 
     def inject(self, kind, it, src, M, xs, zs):
         src = np.ascontiguousarray(src, self.dtype)
@@ -458,9 +516,9 @@ def run_psv(sim, nt, src, M, xs, zs, kind=0, rx=None, rz=None, rxw=None, rzw=Non
     if rxw is None:
         rxw, rzw = rx, rz
     for it in range(it0, it0 + nt):
-        sim.stress()
+        sim.stress(it)
         sim.inject(kind, it, src, M, xs, zs)
-        sim.velocity()
+        sim.velocity(it)
         if rx is not None and it % itrecord == 0:
             su.append(sim.record("U", rx, rz))
             sw.append(sim.record("W", rxw, rzw))
@@ -475,9 +533,9 @@ def run_sh(sim, nt, src, M, xs, zs, kind=0, rx=None, rz=None, itrecord=1):
     """The per-step order of SH/main.cpp:93-141; returns V seismograms."""
     sv = []
     for it in range(nt):
-        sim.stress()
+        sim.stress(it)
         sim.inject(kind, it, src, M, xs, zs)
-        sim.velocity()
+        sim.velocity(it)
         if rx is not None and it % itrecord == 0:
             sv.append(sim.record("V", rx, rz))
     return np.array(sv) if rx is not None else None
@@ -578,9 +636,9 @@ def run_psv_box(sim, box, nt, rx=None, rz=None, it0=0):
     su, sw = [], []
     for it in range(it0, it0 + nt):
         box.pre_stress(it)
-        sim.stress()
+        sim.stress(it)
         box.post_stress(it)
-        sim.velocity()
+        sim.velocity(it)
         if rx is not None:
             su.append(sim.record("U", rx, rz))
             sw.append(sim.record("W", rx, rz))

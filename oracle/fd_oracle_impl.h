@@ -269,7 +269,8 @@ void F(or_psv_stress)(int nx, int nz, int npml, int topabsorb, const REAL *U,
                       const REAL *W, REAL *Txx, REAL *Tzz, REAL *Txz,
                       const REAL *LAM, const REAL *MU, const REAL *MUA,
                       REAL *const *psi, const REAL *const *b,
-                      const REAL *const *c, const REAL *const *k)
+                      const REAL *const *c, const REAL *const *k,
+                      const unsigned char *open)
 {
     F(init_coef_)();
     REAL *pUx = psi[0], *pWx = psi[1], *pUz = psi[2], *pWz = psi[3];
@@ -277,6 +278,7 @@ void F(or_psv_stress)(int nx, int nz, int npml, int topabsorb, const REAL *U,
     for (int iz = 0; iz < nz; iz++)
         for (int ix = 0; ix < nx; ix++) {
             const size_t in = (size_t)iz * nx + ix;
+            if (open && !open[in]) continue; /* fd/cu_fd.cu:16-23 */
             const int ord = F(ordof_)(ix, iz, nx, nz);
             const REAL *cf = F(coef_)[ord];
             const REAL lam = LAM[in], mu = MU[in], mua = MUA[in];
@@ -326,7 +328,7 @@ void F(or_psv_velocity)(int nx, int nz, int npml, int topabsorb, const REAL *Txx
                         const REAL *Tzz, const REAL *Txz, REAL *U, REAL *W,
                         const REAL *BU, const REAL *BW, REAL *const *psi,
                         const REAL *const *b, const REAL *const *c,
-                        const REAL *const *k)
+                        const REAL *const *k, const unsigned char *open)
 {
     F(init_coef_)();
     REAL *pTxx_x = psi[0], *pTxz_x = psi[1], *pTxz_z = psi[2], *pTzz_z = psi[3];
@@ -334,6 +336,7 @@ void F(or_psv_velocity)(int nx, int nz, int npml, int topabsorb, const REAL *Txx
     for (int iz = 0; iz < nz; iz++)
         for (int ix = 0; ix < nx; ix++) {
             const size_t in = (size_t)iz * nx + ix;
+            if (open && !open[in]) continue;
             const int ord = F(ordof_)(ix, iz, nx, nz);
             const REAL *cf = F(coef_)[ord];
             const REAL bu = BU[in], bw = BW[in];
@@ -374,13 +377,15 @@ void F(or_psv_velocity)(int nx, int nz, int npml, int topabsorb, const REAL *Txx
 /* SH stress pass: SH/fd/cu_fd.cu:77-171.  psi order [0]=V_x [1]=V_z. */
 void F(or_sh_stress)(int nx, int nz, int npml, int topabsorb, const REAL *V, REAL *Tx,
                      REAL *Tz, const REAL *MUX, const REAL *MUZ, REAL *const *psi,
-                     const REAL *const *b, const REAL *const *c, const REAL *const *k)
+                     const REAL *const *b, const REAL *const *c, const REAL *const *k,
+                     const unsigned char *open)
 {
     F(init_coef_)();
     REAL *pVx = psi[0], *pVz = psi[1];
 #pragma omp parallel for schedule(static)
     for (int iz = 0; iz < nz; iz++)
         for (int ix = 0; ix < nx; ix++) {
+            if (open && !open[(size_t)iz * nx + ix]) continue; /* SH/fd/cu_fd.cu:106 */
             /* early return after the derivatives (:140) */
             if (ix == 0 || ix >= nx - 1 || iz >= nz - 1) continue;
             const size_t in = (size_t)iz * nx + ix;
@@ -413,16 +418,20 @@ void F(or_sh_stress)(int nx, int nz, int npml, int topabsorb, const REAL *V, REA
  * psi order [0]=Tx_x [1]=Tz_z. */
 void F(or_sh_velocity)(int nx, int nz, int npml, int topabsorb, REAL *Tx, REAL *Tz,
                        REAL *V, const REAL *BV, REAL *const *psi,
-                       const REAL *const *b, const REAL *const *c, const REAL *const *k)
+                       const REAL *const *b, const REAL *const *c, const REAL *const *k,
+                       const unsigned char *open)
 {
     F(init_coef_)();
     REAL *pTx = psi[0], *pTz = psi[1];
+    /* the imaging runs after the window check (:206, :226) */
     if (nz > 1)
-        for (int ix = 0; ix < nx; ix++) Tz[ix] = -Tz[ix + nx];
+        for (int ix = 0; ix < nx; ix++)
+            if (!open || open[ix]) Tz[ix] = -Tz[ix + nx];
 #pragma omp parallel for schedule(static)
     for (int iz = 0; iz < nz; iz++)
         for (int ix = 0; ix < nx; ix++) {
             const size_t in = (size_t)iz * nx + ix;
+            if (open && !open[in]) continue;
             const int ord = F(ordof_)(ix, iz, nx, nz);
             const REAL *cf = F(coef_)[ord];
             const REAL bv = BV[in];

@@ -50,6 +50,7 @@ class Problem:
         self.h, self.dt, self.nt = h, dt, nt
         self.topabsorb, self.kind, self.f64 = topabsorb, kind, f64
         vp, vs, den = layered_model(nx, nz, seed, vmax=vmax)
+        self.vmax = float(vp.max())
         if f64:
             vp, vs, den = (a.astype(np.float64) for a in (vp, vs, den))
         if wave == _lib.WAVE_PSV:
@@ -91,19 +92,31 @@ class Problem:
                 ctx.set_receivers(s, c, self.rx, self.rz)
         return ctx
 
-    def oracle(self, shot=0):
+    def travel_times(self, shot=0, v=None):
+        """Straight-ray first-arrival times from a shot's source at speed v
+        (default: the fastest P speed of the model), nz x nx float32."""
+        xs, zs = self.sources[shot]
+        z, x = np.mgrid[0:self.nz, 0:self.nx]
+        return (np.hypot(x - xs, z - zs) * self.h / (v or self.vmax)).astype(np.float32)
+
+    def oracle(self, shot=0, window=None):
+        """window: (times, before, after) for a first-arrival window."""
         dt_ = np.float64 if self.f64 else np.float32
         cp = unpack_cpml(self.packed, self.wave, self.nx, self.nz, dt_)
         xs, zs = self.sources[shot]
         if self.wave == _lib.WAVE_PSV:
             mat = dict(zip(("BU", "BW", "MU", "MUA", "LAM"), self.mat))
             sim = O.PSV(self.nx, self.nz, self.npml, mat, cp, self.topabsorb, dt_)
+            if window is not None:
+                sim.set_window(*window, self.dt)
             okind = {(_lib.SRC_LINE, False): 0, (_lib.SRC_P, False): 1}.get((self.kind, False), 0)
             u, w = O.run_psv(sim, self.nt, self.stf.astype(dt_), self.M.astype(dt_), xs, zs, kind=okind,
                              rx=self.rx, rz=self.rz)
             return sim, [u, w]
         mat = dict(zip(("BV", "MUX", "MUZ"), self.mat))
         sim = O.SH(self.nx, self.nz, self.npml, mat, cp, self.topabsorb, dt_)
+        if window is not None:
+            sim.set_window(*window, self.dt)
         v = O.run_sh(sim, self.nt, self.stf.astype(dt_), self.M.astype(dt_), xs, zs, kind=self.kind,
                      rx=self.rx, rz=self.rz)
         return sim, [v]

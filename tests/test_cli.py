@@ -1,7 +1,7 @@
 """The par-file driver (fd23_amd/bin, the reference's nbpsv2d / sh2d
 interface) on the CPU: it reads the reference's own par files, model and
 station files, and stops with an error before any device work when a key
-is missing or a feature is unsupported.  With no GPU the run ends at
+is missing or an input file cannot be read.  With no GPU the run ends at
 context creation, after every host step (par, model, material, C-PML,
 wavelet, moment tensor, receivers) has succeeded."""
 import os
@@ -49,9 +49,16 @@ def test_missing_key_is_reported(ttdir):
     assert r.returncode != 0 and "must specify value for 'nrec'" in r.stderr
 
 
-def test_punch_is_rejected(ttdir):
+def test_punch_keys(ttdir):
+    # usepunch=1 makes before, after and timefile mandatory (param.cpp:187-191)
     r = _run("nbpsv2d", ["par=t1", "usepunch=1"], ttdir)
-    assert r.returncode != 0 and "usepunch" in r.stderr
+    assert r.returncode != 0 and "must specify value for 'before'" in r.stderr
+    r = _run("nbpsv2d", ["par=t1", "usepunch=1", "before=0.1", "after=0.2", "timefile=none.bin"], ttdir)
+    assert r.returncode != 0 and "cannot read" in r.stderr and "none.bin" in r.stderr
+    # a travel-time file one value short
+    (ttdir / "short.bin").write_bytes(b"\0" * (4 * (96 * 96 - 1)))
+    r = _run("nbpsv2d", ["par=t1", "usepunch=1", "before=0.1", "after=0.2", "timefile=short.bin"], ttdir)
+    assert r.returncode != 0 and "cannot read" in r.stderr
 
 
 def test_missing_model_is_an_error(ttdir):

@@ -76,3 +76,43 @@ def test_tt_restart_pair_through_the_driver(tmp_path):
         assert np.array_equal(isis[i, 84:].view(np.float32), raw[:, i])
     assert not raw.any()
     assert os.path.getsize(tmp_path / "0_W") == 500 * nrec * 4
+
+
+def test_tt_with_first_arrival_window(tmp_path):
+    """t1 for 200 steps with usepunch=1: a straight-ray travel-time file
+    from the source at 6.8 km/s, before 0.1 s, after 0.3 s.  The saved
+    state equals the oracle's run with the same window (1e-5)."""
+    import oracle as O
+
+    print("case: driver t1 with usepunch=1", flush=True)
+    n, npml, h, dt, nt = 96, 32, 0.3, 0.01, 200
+    z, x = np.mgrid[0:n, 0:n]
+    times = (np.hypot(x - 33, z - 33) * h / 6.8).astype(np.float32)
+    times.tofile(tmp_path / "tt.bin")
+    par = open(os.path.join(TT, "t1")).read().replace("usepunch=0", "usepunch=1").replace("nt=500", f"nt={nt}")
+    par += "\nbefore=0.1\nafter=0.3\ntimefile=tt.bin\n"
+    (tmp_path / "tp").write_text(par)
+    for f in ("station200",):
+        shutil.copy(os.path.join(TT, f), tmp_path / f)
+    shutil.copy(os.path.join(TT, "homo_6.8.mdl"), tmp_path / "homo.mdl")
+    r = subprocess.run([EXE, "par=tp"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-300:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0 and "finish" in r.stdout
+    vp, vs, den = O.mdl_raster(os.path.join(TT, "homo_6.8.mdl"), n, n, h)
+    mat, _ = O.psv_material(vp, vs, den, h, dt)
+    cp = O.psv_cpml(n, n, npml, 0.005, 1e-11, 30.0, 2.0)
+    sim = O.PSV(n, n, npml, mat, cp, topabsorb=True)
+    sim.set_window(times, 0.1, 0.3, dt)
+    first, last = sim.window
+    assert first.min() < 0 < first.max() and last.min() < nt  # blocks open and close in the run
+    free = O.PSV(n, n, npml, mat, cp, topabsorb=True)
+    src, M = O.gaussian(-30), O.psv_moment(0, 0, 20, 90, 45, h, dt)
+    O.run_psv(sim, nt, src, M, 33, 33)
+    O.run_psv(free, nt, src, M, 33, 33)
+    got = np.fromfile(tmp_path / f"state{nt - 1:06d}", np.float32)
+    want = np.frombuffer(sim.state_bytes(), np.float32)
+    e = rel(got, want)
+    print("state vs oracle with window", e, "vs without", rel(got, np.frombuffer(free.state_bytes(), np.float32)),
+          flush=True)
+    assert e < 1e-5
+    assert rel(want, np.frombuffer(free.state_bytes(), np.float32)) > 1e-3

@@ -138,3 +138,27 @@ def test_admission_by_budget():
             srv.result(bid)
         st = srv.stats()
     assert st["peak_active"] == 1 and st["failed"] == 1
+
+
+def test_requests_with_model_equal_requests_with_material():
+    """A request that carries vp, vs, den (the engine computes the material
+    on the device at admission) returns the same seismograms, bitwise, as
+    the same request carrying the host-computed material."""
+    P, S = _lib.WAVE_PSV, _lib.WAVE_SH
+    print("case: serving requests with the model", flush=True)
+    # (FP64 storage: make_solve's material is computed in float, the
+    # device's from the model in double; test_gpu_material covers FP64)
+    specs = [(P, _lib.STORE_F32, 256, 128, 8.0, 300, 2, 31), (S, _lib.STORE_F32, 192, 96, 6.0, 300, 1, 32),
+             (P, _lib.STORE_F16, 256, 128, 7.0, 300, 2, 34)]
+    with Server(max_active=4, chunk_steps=64) as srv:
+        for spec in specs:
+            a = make_solve(*spec, tag="material")
+            b = make_solve(*spec, tag="model", device_material=True)
+            assert b.material is None and b.model is not None
+            got = []
+            for s in (a, b):
+                rid = srv.submit(s)
+                assert srv.wait(rid, 120.0) == _lib.REQ_DONE
+                got.append(srv.result(rid)[0])
+            assert np.abs(got[0]).max() > 0
+            assert np.array_equal(got[0], got[1]), spec
