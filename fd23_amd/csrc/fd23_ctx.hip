@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstdarg>
 #include <cmath>
 #include <cstdio>
@@ -127,6 +128,9 @@ struct fd23_ctx {
     // first-arrival window per 32 x 16 block (fd23_set_punch)
     bool punch = false;
     int *d_pfirst = nullptr, *d_plast = nullptr;
+    std::vector<int> h_pfirst, h_plast; // per block, row-major over nx/32 blocks
+    int4 *d_pwin = nullptr;             // per (tile, shot), built with the event lists
+    size_t pwin_cap = 0;
     int *clock = nullptr;
     bool scales_init = false;
     double bmax = 0;   // largest buoyancy-type material value (scale estimate)
@@ -262,7 +266,7 @@ static void free_ctx(fd23_ctx *c) {
     void *others[] = {c->prof, c->d_fieldtab, c->d_comptab, c->d_compfield, c->scl, c->amax, c->clock,
                       c->d_taps, c->d_src, c->d_stf, c->d_roff, c->d_nrec, c->seis, c->d_tap_start,
                       c->d_rec_start, c->d_tapsT, c->d_recs, c->d_ftap_start, c->d_frec_start, c->d_ftaps,
-                      c->d_frecs, c->d_pfirst, c->d_plast};
+                      c->d_frecs, c->d_pfirst, c->d_plast, c->d_pwin};
     for (auto p : others) if (p) hipFree(p);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -673,6 +677,33 @@ static int build_events(fd23_ctx *c) {
     ts[nkey] = (int)taps.size();
     rs[nkey] = (int)recs.size();
     int r;
+    if (c->punch) {
+        // per (tile, shot): x < it < y when a block of the tile may be open
+        // (always, when the shot has a tap or receiver there), z < it < w
+        // when every block of the tile is open
+        const int nxb = c->d.nx / kPunchBX;
+        std::vector<int4> win(nkey);
+        for (int tz_ = 0; tz_ < ntz; tz_++)
+            for (int tx_ = 0; tx_ < ntx; tx_++) {
+                int lo = INT_MAX, hi = INT_MIN, olo = INT_MIN, ohi = INT_MAX;
+                const int bz1 = (std::min((tz_ + 1) * TZ, c->d.nz) - 1) / kPunchBZ;
+                const int bx1 = (std::min((tx_ + 1) * TX, c->d.nx) - 1) / kPunchBX;
+                for (int bz = tz_ * TZ / kPunchBZ; bz <= bz1; bz++)
+                    for (int bx = tx_ * TX / kPunchBX; bx <= bx1; bx++) {
+                        const int f = c->h_pfirst[(size_t)bz * nxb + bx], l = c->h_plast[(size_t)bz * nxb + bx];
+                        lo = std::min(lo, f);
+                        hi = std::max(hi, l);
+                        olo = std::max(olo, f);
+                        ohi = std::min(ohi, l);
+                    }
+                for (int s = 0; s < S; s++) {
+                    const size_t k = (size_t)(tz_ * ntx + tx_) * S + s;
+                    const bool events = ts[k + 1] > ts[k] || rs[k + 1] > rs[k];
+                    win[k] = events ? make_int4(INT_MIN, INT_MAX, olo, ohi) : make_int4(lo, hi, olo, ohi);
+                }
+            }
+        if ((r = upload_vec(c, c->d_pwin, c->pwin_cap, win))) return r;
+    }
     if ((r = upload_vec(c, c->d_tap_start, c->tap_start_cap, ts))) return r;
     if ((r = upload_vec(c, c->d_rec_start, c->rec_start_cap, rs))) return r;
     if ((r = upload_vec(c, c->d_tapsT, c->tapsT_cap, taps))) return r;
@@ -856,6 +887,7 @@ static Events make_events(const fd23_ctx *c, int k, int itrecord) {
     e.seis = c->seis;
     e.max_frames = c->seis ? c->d.max_frames : 0;
     e.frame_size = (long)c->d.nshots * c->ncomp * c->d.max_rec;
+    e.pwin = c->punch ? c->d_pwin : nullptr;
     return e;
 }
 
@@ -1206,7 +1238,10 @@ extern "C" int fd23_set_punch(fd23_ctx *c, const float *times, float before, flo
     size_t cap_f = c->d_pfirst ? first.size() : 0, cap_l = c->d_plast ? last.size() : 0;
     if ((r = upload_vec(c, c->d_pfirst, cap_f, first)) || (r = upload_vec(c, c->d_plast, cap_l, last))) return r;
     HIPCHK(hipStreamSynchronize(c->stream));
+    c->h_pfirst = std::move(first);
+    c->h_plast = std::move(last);
     c->punch = true;
+    c->events_dirty = true; // the per-tile window table is built with them
     fd23_ctx_invalidate_graphs(c);
     return FD23_OK;
 }

@@ -144,13 +144,17 @@ struct Events {
     void *seis;           // [max_frames][frame_size] compute type
     int max_frames;
     long frame_size;
+    // first-arrival window per (tile*nshots + shot): x < it < y when the
+    // workgroup has work (a block may be open, or the shot has a tap or
+    // receiver in the tile), z < it < w when every block is open
+    const int4 *pwin;
 };
 
 // first-arrival window (the graph loop applies it: the step index is known
 // there only).  Bit r of rows_open is set when row r of the thread's RZ
 // rows is updated this step; rows off the grid are clear.
 __device__ __forceinline__ bool punch_on(const Grid &g, const Events &ev) {
-    return g.pfirst != nullptr && ev.clock != nullptr;
+    return g.pfirst != nullptr && ev.clock != nullptr && ev.pwin != nullptr;
 }
 template <int RZ>
 __device__ __forceinline__ unsigned rows_open(const Grid &g, bool pun, int gx, int gz0, int it) {
@@ -162,30 +166,23 @@ __device__ __forceinline__ unsigned rows_open(const Grid &g, bool pun, int gx, i
     return m;
 }
 
-// what a workgroup does under a window (uniform per workgroup): nothing when
-// every block its tile covers is closed and no shot of its group has a source
-// tap or a receiver in the tile; the interior path when every block is open;
+// what a workgroup does under a window (uniform per workgroup), from the
+// per-tile table (one scalar load per shot): nothing when no shot of its
+// group has work in the tile; the interior path when every block is open;
 // else the exact path with the per-row mask
 enum { kTileIdle = 0, kTileMixed = 1, kTileOpen = 2 };
-template <int TZ, int SPB>
+template <int SPB>
 __device__ __forceinline__ int tile_window(const Grid &g, const Events &ev, const TileId &t, int it) {
-    const int bx1 = (min(t.x0 + TX, g.nx) - 1) / kPunchBX, bz1 = (min(t.z0 + TZ, g.nz) - 1) / kPunchBZ;
-    int nopen = 0, nblk = 0;
-    for (int bz = t.z0 / kPunchBZ; bz <= bz1; bz++)
-        for (int bx = t.x0 / kPunchBX; bx <= bx1; bx++) {
-            const int b = bz * g.pnxb + bx;
-            nopen += g.pfirst[b] < it && it < g.plast[b];
-            nblk++;
-        }
-    if (nopen == nblk) return kTileOpen;
-    if (nopen > 0) return kTileMixed;
-    const int s1 = min(t.group * SPB + SPB, g.nshots);
-    for (int s = t.group * SPB; s < s1; s++) {
-        const int key = t.tile * g.nshots + s;
-        if (ev.tap_start[key + 1] > ev.tap_start[key] || ev.rec_start[key + 1] > ev.rec_start[key])
-            return kTileMixed;
+    const int k0 = t.tile * g.nshots + t.group * SPB;
+    const int ns = min(SPB, g.nshots - t.group * SPB);
+    const int4 w0 = ev.pwin[k0];
+    bool work = w0.x < it && it < w0.y;
+    for (int s = 1; s < ns && !work; s++) {
+        const int4 w = ev.pwin[k0 + s];
+        work = w.x < it && it < w.y;
     }
-    return kTileIdle;
+    if (!work) return kTileIdle;
+    return w0.z < it && it < w0.w ? kTileOpen : kTileMixed;
 }
 
 // source sample of a shot at step it, or false when inactive (it >= lsrc
@@ -379,7 +376,7 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
     // one; a tile with a closed block takes the exact path, which masks rows
     const bool pun = REGION != REGION_INTERIOR && punch_on(g, ev); // uniform
     if (pun) {
-        const int tw = tile_window<TZ, SPB>(g, ev, t, it);
+        const int tw = tile_window<SPB>(g, ev, t, it);
         if (tw == kTileIdle) return;
         interior = interior && tw == kTileOpen;
     }
@@ -636,7 +633,7 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
     // one; a tile with a closed block takes the exact path, which masks rows
     const bool pun = REGION != REGION_INTERIOR && punch_on(g, ev); // uniform
     if (pun) {
-        const int tw = tile_window<TZ, SPB>(g, ev, t, it);
+        const int tw = tile_window<SPB>(g, ev, t, it);
         if (tw == kTileIdle) return;
         interior = interior && tw == kTileOpen;
     }
@@ -785,7 +782,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
     // path, which masks rows
     const bool pun = punch_on(g, ev); // uniform
     if (pun) {
-        const int tw = tile_window<TZ, SPB>(g, ev, t, it);
+        const int tw = tile_window<SPB>(g, ev, t, it);
         if (tw == kTileIdle) return;
         interior = interior && tw == kTileOpen;
     }
@@ -947,7 +944,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
     // path, which masks rows
     const bool pun = punch_on(g, ev); // uniform
     if (pun) {
-        const int tw = tile_window<TZ, SPB>(g, ev, t, it);
+        const int tw = tile_window<SPB>(g, ev, t, it);
         if (tw == kTileIdle) return;
         interior = interior && tw == kTileOpen;
     }
