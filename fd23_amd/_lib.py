@@ -69,6 +69,7 @@ SIGNATURES = {
     "fd23_set_material": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
     "fd23_set_model": (c_int, [c_vp, c_vp, c_vp, c_vp, c_dbl, c_dbl, ctypes.POINTER(c_dbl)]),
     "fd23_get_material": (c_int, [c_vp, c_int, c_vp]),
+    "fd23_set_model_file": (c_int, [c_vp, ctypes.c_char_p, c_dbl, c_dbl, ctypes.POINTER(c_dbl)]),
     "fd23_set_cpml": (c_int, [c_vp, c_vp]),
     "fd23_set_source": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_int, c_vp]),
     "fd23_set_receivers": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_int]),

@@ -84,6 +84,13 @@ class Context:
         check(self.lib.fd23_set_model(self.h, p[0], p[1], p[2], h, dt, ctypes.byref(stab)))
         return stab.value
 
+    def set_model_file(self, path, h, dt):
+        """A .mdl (rasterised on the device) or raster model file and its
+        material computed on the device; returns the stability number."""
+        stab = ctypes.c_double(0.0)
+        check(self.lib.fd23_set_model_file(self.h, str(path).encode(), h, dt, ctypes.byref(stab)))
+        return stab.value
+
     def material(self, index):
         """One material array (fd23_set_material order) as nz x nx."""
         out = np.zeros((self.nz, self.nx), self.cdtype)

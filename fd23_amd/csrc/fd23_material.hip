@@ -16,6 +16,7 @@
 // scale estimate the largest buoyancy: both are reduced on the device.
 #include "fd23.h"
 #include "fd_device.hpp"
+#include "mdl.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -23,6 +24,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #pragma clang fp contract(off)
 
@@ -151,40 +153,67 @@ __global__ void k_sh_material(const float *__restrict__ vs, const float *__restr
     wave_max_to(&maxk[1], bmax);
 }
 
+// crossing-number test of SH/model/model.cpp:437-469 in the host
+// rasteriser's double arithmetic (host_precompute.cpp: inside)
+__device__ int inside_poly(double xp, double yp, const float *vx, const float *vz, int n) {
+    int nleft = 0;
+    double x2 = vx[n - 1], y2 = vz[n - 1];
+    for (int j = 0; j < n; j++) {
+        const double x1 = x2, y1 = y2;
+        x2 = vx[j];
+        y2 = vz[j];
+        if (fmin(y1, y2) >= yp) continue;
+        if (fmax(y1, y2) < yp) continue;
+        if (y1 == y2) {
+            if (xp > fmax(x1, x2)) continue;
+            if (xp >= fmin(x1, x2)) return 1;
+            nleft++;
+            continue;
+        }
+        const double xi = x1 + (yp - y1) * (x2 - x1) / (y2 - y1);
+        if (xi == xp) return 1;
+        if (xi > xp) nleft++;
+    }
+    return nleft % 2;
+}
+
+struct MdlDev {
+    const float *vx, *vz, *vp, *vs, *den;
+    const int *first, *count, *ix1, *ix2, *iz1, *iz2;
+    int np;
+};
+
+// every cell takes the last polygon that contains it (polygons tested from
+// the last down, the first hit wins), else the background
+template <class T>
+__global__ void k_mdl_raster(MdlDev m, int nx, int nz, double hh, T *vp, T *vs, T *den) {
+    const long N = (long)nx * nz;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N) return;
+    const int iz = (int)(q / nx), ix = (int)(q - (long)iz * nx);
+    int hit = 0;
+    for (int p = m.np - 1; p >= 1 && !hit; p--) {
+        if (m.count[p] == 0 || ix < m.ix1[p] || ix > m.ix2[p] || iz < m.iz1[p] || iz > m.iz2[p]) continue;
+        if (inside_poly(hh * ix, hh * iz, m.vx + m.first[p], m.vz + m.first[p], m.count[p])) hit = p;
+    }
+    vp[q] = (T)m.vp[hit];
+    vs[q] = (T)m.vs[hit];
+    den[q] = (T)m.den[hit];
+}
+
 const double kCoefSum = 1225. / 1024. + 245. / 3072. + 49. / 5120. + 5. / 7168.;
 
 } // namespace
 
-// vp, vs, den: nz*nx host arrays (double for an FP64 P-SV context, float
-// otherwise; SH reads vs and den only).
-extern "C" int fd23_set_model(fd23_ctx *c, const void *vp, const void *vs, const void *den, double h, double dt,
-                              double *stab) {
-    if (!c || !vs || !den) return merr(FD23_EINVAL, "set_model: null argument");
+// the material kernels on device inputs buf + kIn (vp vs den, or vs den for
+// SH), reduction keys at buf; frees buf
+static int material_from_device(fd23_ctx *c, char *buf, size_t kIn, double h, double dt, double *stab) {
     const fd23_desc &d = fd23_ctx_desc(c);
     const bool psv = d.wave == FD23_WAVE_PSV;
-    if (psv && !vp) return merr(FD23_EINVAL, "set_model: null vp");
-    if (d.nx < 2 || d.nz < 2) return merr(FD23_EINVAL, "set_model: grid too small");
     const bool f64 = psv && d.storage == FD23_STORE_F64;
-    if (!psv && d.storage == FD23_STORE_F64)
-        return merr(FD23_EINVAL, "set_model: SH material is computed in float; pass it with fd23_set_material");
-    const size_t N = (size_t)d.nx * d.nz, es = f64 ? 8 : 4;
-    const int nin = psv ? 3 : 2;
+    const size_t N = (size_t)d.nx * d.nz;
     hipStream_t st = fd23_ctx_stream(c);
-    // one staging allocation: the two 64-bit reduction keys first (8-byte
-    // aligned, as 64-bit atomics need), then the inputs from byte 256
-    const size_t kIn = 256;
-    char *buf = nullptr;
-    MCHK(hipMalloc((void **)&buf, kIn + nin * N * es));
     unsigned long long *keys = (unsigned long long *)buf;
-    const void *src[3] = {psv ? vp : vs, psv ? vs : den, den};
-    for (int i = 0; i < nin; i++) {
-        hipError_t e = hipMemcpyAsync(buf + kIn + i * N * es, src[i], N * es, hipMemcpyHostToDevice, st);
-        if (e != hipSuccess) {
-            hipStreamSynchronize(st);
-            hipFree(buf);
-            return merr(FD23_EHIP, std::string("set_model copy: ") + hipGetErrorString(e));
-        }
-    }
     // 0 is below every key (the key of -NaN): the maxima start empty
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, keys, 2, 0ull);
     const int block = 256;
@@ -225,4 +254,109 @@ extern "C" int fd23_set_model(fd23_ctx *c, const void *vp, const void *vs, const
     if (stab) *stab = s;
     if (s > 1) return merr(FD23_ESTAB, "model stability number " + std::to_string(s) + " > 1");
     return FD23_OK;
+}
+
+static int check_model_ctx(fd23_ctx *c) {
+    const fd23_desc &d = fd23_ctx_desc(c);
+    if (d.nx < 2 || d.nz < 2) return merr(FD23_EINVAL, "set_model: grid too small");
+    if (d.wave != FD23_WAVE_PSV && d.storage == FD23_STORE_F64)
+        return merr(FD23_EINVAL, "set_model: SH material is computed in float; pass it with fd23_set_material");
+    return FD23_OK;
+}
+
+// the staging allocation: the two 64-bit reduction keys first (8-byte
+// aligned, as 64-bit atomics need), then the inputs from byte kIn
+static const size_t kIn = 256;
+
+// vp, vs, den: nz*nx host arrays (double for an FP64 P-SV context, float
+// otherwise; SH reads vs and den only).
+extern "C" int fd23_set_model(fd23_ctx *c, const void *vp, const void *vs, const void *den, double h, double dt,
+                              double *stab) {
+    if (!c || !vs || !den) return merr(FD23_EINVAL, "set_model: null argument");
+    const fd23_desc &d = fd23_ctx_desc(c);
+    const bool psv = d.wave == FD23_WAVE_PSV;
+    if (psv && !vp) return merr(FD23_EINVAL, "set_model: null vp");
+    int r = check_model_ctx(c);
+    if (r) return r;
+    const bool f64 = psv && d.storage == FD23_STORE_F64;
+    const size_t N = (size_t)d.nx * d.nz, es = f64 ? 8 : 4;
+    const int nin = psv ? 3 : 2;
+    hipStream_t st = fd23_ctx_stream(c);
+    char *buf = nullptr;
+    MCHK(hipMalloc((void **)&buf, kIn + nin * N * es));
+    const void *src[3] = {psv ? vp : vs, psv ? vs : den, den};
+    for (int i = 0; i < nin; i++) {
+        hipError_t e = hipMemcpyAsync(buf + kIn + i * N * es, src[i], N * es, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) {
+            hipStreamSynchronize(st);
+            hipFree(buf);
+            return merr(FD23_EHIP, std::string("set_model copy: ") + hipGetErrorString(e));
+        }
+    }
+    return material_from_device(c, buf, kIn, h, dt, stab);
+}
+
+// a .mdl model rasterised on the device, then its material; raster files
+// (model.vp/.vs/.den) are read on the host and go through fd23_set_model
+extern "C" int fd23_set_model_file(fd23_ctx *c, const char *model, double h, double dt, double *stab) {
+    if (!c || !model) return merr(FD23_EINVAL, "set_model_file: null argument");
+    int r = check_model_ctx(c);
+    if (r) return r;
+    const fd23_desc &d = fd23_ctx_desc(c);
+    const bool psv = d.wave == FD23_WAVE_PSV;
+    const bool f64 = psv && d.storage == FD23_STORE_F64;
+    const size_t N = (size_t)d.nx * d.nz;
+    const std::string nm(model);
+    if (nm.size() < 4 || nm.compare(nm.size() - 4, 4, ".mdl") != 0) {
+        std::vector<float> v(N), s_(N), rho(N);
+        if ((r = fd23_model_read(model, d.nx, d.nz, h, v.data(), s_.data(), rho.data()))) return r;
+        if (!f64) return fd23_set_model(c, v.data(), s_.data(), rho.data(), h, dt, stab);
+        std::vector<double> a(v.begin(), v.end()), b(s_.begin(), s_.end()), e(rho.begin(), rho.end());
+        return fd23_set_model(c, a.data(), b.data(), e.data(), h, dt, stab);
+    }
+    MdlFlat m;
+    if ((r = fd23_mdl_flatten(model, d.nx, d.nz, h, m))) return r;
+    const int np = (int)m.vp.size();
+    // polygon tables: 2 vertex arrays, 3 property arrays, 6 int arrays
+    std::vector<char> tab;
+    auto put = [&](const void *p, size_t bytes) {
+        const size_t o = tab.size();
+        tab.resize(o + ((bytes + 255) & ~(size_t)255));
+        if (bytes) memcpy(tab.data() + o, p, bytes);
+        return o;
+    };
+    const size_t o_vx = put(m.vx.data(), m.vx.size() * 4), o_vz = put(m.vz.data(), m.vz.size() * 4);
+    const size_t o_vp = put(m.vp.data(), np * 4), o_vs = put(m.vs.data(), np * 4), o_den = put(m.den.data(), np * 4);
+    const size_t o_f = put(m.first.data(), np * 4), o_c = put(m.count.data(), np * 4);
+    const size_t o_x1 = put(m.ix1.data(), np * 4), o_x2 = put(m.ix2.data(), np * 4);
+    const size_t o_z1 = put(m.iz1.data(), np * 4), o_z2 = put(m.iz2.data(), np * 4);
+    const size_t es = f64 ? 8 : 4;
+    const size_t o_tab = kIn + 3 * N * es; // after the keys and the three rasterised arrays
+    hipStream_t st = fd23_ctx_stream(c);
+    char *buf = nullptr;
+    MCHK(hipMalloc((void **)&buf, o_tab + tab.size()));
+    hipError_t e = hipMemcpyAsync(buf + o_tab, tab.data(), tab.size(), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) {
+        hipStreamSynchronize(st);
+        hipFree(buf);
+        return merr(FD23_EHIP, std::string("set_model_file copy: ") + hipGetErrorString(e));
+    }
+    const char *t = buf + o_tab;
+    MdlDev md{(const float *)(t + o_vx), (const float *)(t + o_vz), (const float *)(t + o_vp),
+              (const float *)(t + o_vs), (const float *)(t + o_den), (const int *)(t + o_f),
+              (const int *)(t + o_c), (const int *)(t + o_x1), (const int *)(t + o_x2),
+              (const int *)(t + o_z1), (const int *)(t + o_z2), np};
+    // rasterised vp vs den at buf + kIn (SH: vs den there, vp after them,
+    // the layout material_from_device reads)
+    char *a0 = buf + kIn, *a1 = a0 + N * es, *a2 = a1 + N * es;
+    char *ovp = psv ? a0 : a2, *ovs = psv ? a1 : a0, *oden = psv ? a2 : a1;
+    const int block = 256, grid = (int)((N + block - 1) / block);
+    const double hh = (float)h;
+    if (f64)
+        hipLaunchKernelGGL(k_mdl_raster<double>, dim3(grid), dim3(block), 0, st, md, d.nx, d.nz, hh, (double *)ovp,
+                           (double *)ovs, (double *)oden);
+    else
+        hipLaunchKernelGGL(k_mdl_raster<float>, dim3(grid), dim3(block), 0, st, md, d.nx, d.nz, hh, (float *)ovp,
+                           (float *)ovs, (float *)oden);
+    return material_from_device(c, buf, kIn, h, dt, stab);
 }

@@ -4,6 +4,7 @@
 // formulas, their order of operations and their edge quirks follow the
 // reference files cited at each function.
 #include "fd23.h"
+#include "mdl.hpp"
 
 #include <algorithm>
 #include <cctype>
@@ -427,40 +428,63 @@ static int inside(double xp, double yp, const std::vector<std::pair<float, float
     return nleft % 2;
 }
 
+int fd23_mdl_flatten(const char *path, int nx, int nz, double h, MdlFlat &out) {
+    std::vector<Poly> polys;
+    int r = load_mdl(path, polys);
+    if (r) return r;
+    out = MdlFlat{};
+    const double hh = (float)h;
+    auto clampi = [](int a, int lo, int hi) { return a < lo ? lo : (a > hi ? hi : a); };
+    for (size_t p = 0; p < polys.size(); p++) {
+        const Poly &P = polys[p];
+        float x1 = 1E28f, x2 = -1E28f, z1 = 1E28f, z2 = -1E28f;
+        out.first.push_back((int)out.vx.size());
+        out.count.push_back(p == 0 ? 0 : (int)P.v.size());
+        for (auto &v : P.v) {
+            if (p == 0) break;
+            out.vx.push_back(v.first);
+            out.vz.push_back(v.second);
+            x1 = std::min(x1, v.first);
+            x2 = std::max(x2, v.first);
+            z1 = std::min(z1, v.second);
+            z2 = std::max(z2, v.second);
+        }
+        out.ix1.push_back(clampi((int)(x1 / hh - 1), 0, nx - 1));
+        out.ix2.push_back(clampi((int)(x2 / hh + 1), 0, nx - 1));
+        out.iz1.push_back(clampi((int)(z1 / hh - 1), 0, nz - 1));
+        out.iz2.push_back(clampi((int)(z2 / hh + 1), 0, nz - 1));
+        out.vp.push_back(P.vp);
+        out.vs.push_back(P.vs);
+        out.den.push_back(P.den);
+    }
+    return FD23_OK;
+}
+
 extern "C" int fd23_model_read(const char *name, int nx, int nz, double h, float *vp, float *vs, float *den) {
     if (!name || nx < 1 || nz < 1 || !vp || !vs || !den) return herr(FD23_EINVAL, "model_read: bad arguments");
     const std::string nm(name);
     const size_t N = (size_t)nx * nz;
     if (ends_with(nm, "mdl")) {
-        std::vector<Poly> polys;
-        int r = load_mdl(nm, polys);
+        MdlFlat m;
+        int r = fd23_mdl_flatten(name, nx, nz, h, m);
         if (r) return r;
         for (size_t q = 0; q < N; q++) {
-            vp[q] = polys[0].vp;
-            vs[q] = polys[0].vs;
-            den[q] = polys[0].den;
+            vp[q] = m.vp[0];
+            vs[q] = m.vs[0];
+            den[q] = m.den[0];
         }
         const double hh = (float)h;
-        for (size_t p = 1; p < polys.size(); p++) {
-            const Poly &P = polys[p];
-            if (P.v.empty()) continue;
-            float x1 = 1E28f, x2 = -1E28f, z1 = 1E28f, z2 = -1E28f;
-            for (auto &v : P.v) {
-                x1 = std::min(x1, v.first);
-                x2 = std::max(x2, v.first);
-                z1 = std::min(z1, v.second);
-                z2 = std::max(z2, v.second);
-            }
-            auto clampi = [](int a, int lo, int hi) { return a < lo ? lo : (a > hi ? hi : a); };
-            const int ix1 = clampi((int)(x1 / hh - 1), 0, nx - 1), ix2 = clampi((int)(x2 / hh + 1), 0, nx - 1);
-            const int iz1 = clampi((int)(z1 / hh - 1), 0, nz - 1), iz2 = clampi((int)(z2 / hh + 1), 0, nz - 1);
-            for (int iz = iz1; iz <= iz2; iz++)
-                for (int ix = ix1; ix <= ix2; ix++)
-                    if (inside(hh * ix, hh * iz, P.v)) {
+        for (size_t p = 1; p < m.vp.size(); p++) {
+            if (m.count[p] == 0) continue;
+            std::vector<std::pair<float, float>> v;
+            for (int j = 0; j < m.count[p]; j++) v.push_back({m.vx[m.first[p] + j], m.vz[m.first[p] + j]});
+            for (int iz = m.iz1[p]; iz <= m.iz2[p]; iz++)
+                for (int ix = m.ix1[p]; ix <= m.ix2[p]; ix++)
+                    if (inside(hh * ix, hh * iz, v)) {
                         const size_t q = (size_t)iz * nx + ix;
-                        vp[q] = P.vp;
-                        vs[q] = P.vs;
-                        den[q] = P.den;
+                        vp[q] = m.vp[p];
+                        vs[q] = m.vs[p];
+                        den[q] = m.den[p];
                     }
         }
         return FD23_OK;

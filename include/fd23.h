@@ -101,6 +101,12 @@ int fd23_set_material(fd23_ctx *ctx, const void *const *arrays);
  * 31-135).                                                              */
 int fd23_set_model(fd23_ctx *ctx, const void *vp, const void *vs, const void *den, double h, double dt,
                    double *stab);
+/* The model read from a file and its material computed on the device: a
+ * .mdl polygon model (SH/model/model.cpp:175-237) is rasterised by a
+ * kernel (each cell takes the last polygon containing it), raster files
+ * (model.vp/.vs/.den) are read on the host; then as fd23_set_model.  The
+ * model equals fd23_model_read's bitwise (widened to double for FP64).  */
+int fd23_set_model_file(fd23_ctx *ctx, const char *model, double h, double dt, double *stab);
 /* One material array (index in the fd23_set_material order) copied back
  * as nz*nx values (float, or double for F64); no reference counterpart. */
 int fd23_get_material(fd23_ctx *ctx, int index, void *out);

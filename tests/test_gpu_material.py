@@ -96,3 +96,72 @@ def test_unstable_model_reports_and_sets():
     with Context(_lib.WAVE_SH, 96, 48, 8, storage=_lib.STORE_F64) as ctx:
         with pytest.raises(_lib.Fd23Error, match="SH material"):
             ctx.set_model(None, vs.astype(np.float64), den.astype(np.float64), 10.0, 7e-4)
+
+
+SYNTH_MDL = """BACK 3.0 1.7 2.2
+DEF A 1.0 1.0
+DEF B 9.0 1.0
+# a triangle with a horizontal base on grid rows, vertices on grid points
+POLY 4.0 -0.5 2.4
+\tA B
+\t5.0 6.0
+# overlapping later polygon (wins), water (vs 0)
+POLY 1.5 0.0 1.0
+\t4.0 0.0
+\t12.0 0.0
+\t12.0 3.0
+\t4.0 3.0
+# a sliver with a vertical edge through cell centres and one off the grid
+POLY 5.5 3.1 -0.45
+\t2.5 4.0
+\t2.5 9.5
+\t7.25 7.75
+"""
+
+
+@pytest.mark.parametrize("case", ["fixture", "tt", "synthetic", "synthetic_f64", "synthetic_sh", "raster"])
+def test_model_file_on_device_equals_host(tmp_path, case):
+    """fd23_set_model_file: the .mdl polygons rasterised on the device, then
+    the material, bitwise equal to fd23_model_read + the host precompute;
+    raster files go through the host reader."""
+    import os
+
+    _print("case: model file", case)
+    wave, storage, h, dt = _lib.WAVE_PSV, _lib.STORE_F32, 0.25, 0.01
+    if case == "fixture":
+        g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_host", "psv_mdl.npz"))
+        path, nx, nz = tmp_path / "m.mdl", 64, 48
+        path.write_text(g["mdl"].tobytes().decode())
+    elif case == "tt":
+        path, nx, nz, h = os.path.join(os.path.dirname(__file__), "golden", "ref", "psv_tt", "homo_6.8.mdl"), 96, 96, 0.3
+    elif case == "raster":
+        nx, nz = 70, 40
+        vp, vs, den = layered_model(nx, nz, 9)
+        for ext, a in (("vp", vp), ("vs", vs), ("den", den)):
+            a.astype(np.float32).tofile(tmp_path / ("r." + ext))
+        path, h, dt = tmp_path / "r", 10.0, 7e-4
+    else:
+        path, nx, nz = tmp_path / "s.mdl", 56, 44
+        path.write_text(SYNTH_MDL)
+        h, dt = 0.25, 0.005
+        if case == "synthetic_f64":
+            storage = _lib.STORE_F64
+        if case == "synthetic_sh":
+            wave = _lib.WAVE_SH
+    vp, vs, den = host.model_read(str(path), nx, nz, h)
+    f64 = storage == _lib.STORE_F64
+    if wave == _lib.WAVE_PSV:
+        mat, stab = host.psv_material(*((a.astype(np.float64) for a in (vp, vs, den)) if f64 else (vp, vs, den)),
+                                      h, dt, f64=f64)
+    else:
+        mat, stab = host.sh_material(vs, den, h, dt)
+    if case.startswith("synthetic"):
+        # every branch of the polygon test is taken somewhere on this grid
+        assert len(np.unique(vp)) == 4
+    with Context(wave, nx, nz, 8, storage=storage) as ctx:
+        s = ctx.set_model_file(path, h, dt)
+        got = [ctx.material(i) for i in range(len(mat))]
+    assert s == stab
+    for i, (a, b) in enumerate(zip(got, mat)):
+        bits = np.uint64 if f64 else np.uint32
+        assert np.array_equal(a.view(bits), b.view(bits)), (case, i)
