@@ -67,10 +67,30 @@ __global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = v;
 }
 
-// wave-wide max, then one atomic per wave
-__device__ __forceinline__ void wave_max_to(unsigned long long *dst, double v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    if ((threadIdx.x & 63) == 0) atomicMax(dst, order_key(v));
+// workgroup-wide max of two values (256 threads), then one atomic per
+// workgroup and value: the grid is a few thousand workgroups that stride
+// over the cells, so the two addresses see a few thousand atomics
+constexpr int kMatBlock = 256, kMatGrid = 2048;
+__device__ __forceinline__ void block_max_to(unsigned long long *dst, double a, double b) {
+    __shared__ double sa[kMatBlock / 64], sb[kMatBlock / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+        a = fmax(a, __shfl_xor(a, o));
+        b = fmax(b, __shfl_xor(b, o));
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sa[w] = a;
+        sb[w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < kMatBlock / 64; i++) {
+            a = fmax(a, sa[i]);
+            b = fmax(b, sb[i]);
+        }
+        atomicMax(&dst[0], order_key(a));
+        atomicMax(&dst[1], order_key(b));
+    }
 }
 
 template <class T>
@@ -84,13 +104,13 @@ __device__ __forceinline__ void lame(const T *vp, const T *vs, const T *den, lon
 }
 
 template <class T>
-__global__ void k_psv_material(const T *__restrict__ vp, const T *__restrict__ vs, const T *__restrict__ den, int nx,
-                               int nz, int px, T fac, T *BU, T *BW, T *MU, T *MUA, T *LAM,
-                               unsigned long long *maxk) {
+__global__ __launch_bounds__(kMatBlock) void k_psv_material(const T *__restrict__ vp, const T *__restrict__ vs,
+                                                            const T *__restrict__ den, int nx, int nz, int px,
+                                                            T fac, T *BU, T *BW, T *MU, T *MUA, T *LAM,
+                                                            unsigned long long *maxk) {
     const long N = (long)nx * nz;
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double vmax = -1e300, bmax = 0.0;
-    if (q < N) {
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (long)gridDim.x * blockDim.x) {
         const int iz = (int)(q / nx), ix = (int)(q - (long)iz * nx);
         auto edge = [&](int x, int z) { return x < 5 || x >= nx - 5 || z >= nz - 5; };
         T mu, lam;
@@ -121,19 +141,19 @@ __global__ void k_psv_material(const T *__restrict__ vp, const T *__restrict__ v
         MU[o] = mu * fac;
         MUA[o] = mua * fac;
         LAM[o] = lam * fac;
-        vmax = (double)vp[q];
-        bmax = fmax(fabs((double)(bu * fac)), fabs((double)(bw * fac)));
+        vmax = fmax(vmax, (double)vp[q]);
+        bmax = fmax(bmax, fmax(fabs((double)(bu * fac)), fabs((double)(bw * fac))));
     }
-    wave_max_to(&maxk[0], vmax);
-    wave_max_to(&maxk[1], bmax);
+    block_max_to(maxk, vmax, bmax);
 }
 
-__global__ void k_sh_material(const float *__restrict__ vs, const float *__restrict__ den, int nx, int nz, int px,
-                              float fac, float *BV, float *MUX, float *MUZ, unsigned long long *maxk) {
+__global__ __launch_bounds__(kMatBlock) void k_sh_material(const float *__restrict__ vs,
+                                                           const float *__restrict__ den, int nx, int nz, int px,
+                                                           float fac, float *BV, float *MUX, float *MUZ,
+                                                           unsigned long long *maxk) {
     const long N = (long)nx * nz;
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double vmax = -1e300, bmax = 0.0;
-    if (q < N) {
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += (long)gridDim.x * blockDim.x) {
         const int iz = (int)(q / nx), ix = (int)(q - (long)iz * nx);
         auto mu_of = [&](long k) -> double { return vs[k] * vs[k] * den[k]; };
         const double m1 = mu_of(q);
@@ -146,11 +166,10 @@ __global__ void k_sh_material(const float *__restrict__ vs, const float *__restr
         BV[o] = bv;
         MUX[o] = mux * fac;
         MUZ[o] = muz * fac;
-        vmax = (double)vs[q];
-        bmax = fabs((double)bv);
+        vmax = fmax(vmax, (double)vs[q]);
+        bmax = fmax(bmax, fabs((double)bv));
     }
-    wave_max_to(&maxk[0], vmax);
-    wave_max_to(&maxk[1], bmax);
+    block_max_to(maxk, vmax, bmax);
 }
 
 // crossing-number test of SH/model/model.cpp:437-469 in the host
@@ -216,8 +235,8 @@ static int material_from_device(fd23_ctx *c, char *buf, size_t kIn, double h, do
     unsigned long long *keys = (unsigned long long *)buf;
     // 0 is below every key (the key of -NaN): the maxima start empty
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, keys, 2, 0ull);
-    const int block = 256;
-    const int grid = (int)((N + block - 1) / block);
+    const int block = kMatBlock;
+    const int grid = (int)std::min<size_t>((N + block - 1) / block, kMatGrid);
     const int px = fd23_ctx_px(c);
     if (psv && f64) {
         const double *in = (const double *)(buf + kIn);
