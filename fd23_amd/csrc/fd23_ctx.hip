@@ -128,6 +128,7 @@ struct fd23_ctx {
     // first-arrival window per 32 x 16 block (fd23_set_punch)
     bool punch = false;
     int *d_pfirst = nullptr, *d_plast = nullptr;
+    size_t pfirst_cap = 0, plast_cap = 0;
     std::vector<int> h_pfirst, h_plast; // per block, row-major over nx/32 blocks
     int4 *d_pwin = nullptr;             // per (tile, shot), built with the event lists
     size_t pwin_cap = 0;
@@ -1203,6 +1204,15 @@ static void enqueue_commit(fd23_ctx *c, bool stress) {
     hipLaunchKernelGGL(k_commit_scales, dim3(1), dim3(64), 0, c->stream, c->scl, pass_mask(c, stress));
 }
 
+// a step bound truncated toward zero as the reference's int() does, held
+// inside the int range (where that conversion would be undefined)
+static int step_of(float v) {
+    if (!(v == v)) return 0;
+    if (v >= 2147483520.0f) return INT_MAX;
+    if (v <= -2147483648.0f) return INT_MIN;
+    return (int)v;
+}
+
 // First-arrival window, SH/slide/cu_slide.cu:7-61: per block of 32 x 16
 // cells the step window from the block's earliest and latest arrival time,
 // in float arithmetic and truncated to int as the reference does.
@@ -1231,12 +1241,12 @@ extern "C" int fd23_set_punch(fd23_ctx *c, const float *times, float before, flo
                     lo = std::min(lo, t);
                     hi = std::max(hi, t);
                 }
-            first[(size_t)k * nxb + i] = (int)((lo - before) / dt);
-            last[(size_t)k * nxb + i] = (int)((hi + after) / dt);
+            first[(size_t)k * nxb + i] = step_of((lo - before) / dt);
+            last[(size_t)k * nxb + i] = step_of((hi + after) / dt);
         }
     int r;
-    size_t cap_f = c->d_pfirst ? first.size() : 0, cap_l = c->d_plast ? last.size() : 0;
-    if ((r = upload_vec(c, c->d_pfirst, cap_f, first)) || (r = upload_vec(c, c->d_plast, cap_l, last))) return r;
+    if ((r = upload_vec(c, c->d_pfirst, c->pfirst_cap, first)) || (r = upload_vec(c, c->d_plast, c->plast_cap, last)))
+        return r;
     HIPCHK(hipStreamSynchronize(c->stream));
     c->h_pfirst = std::move(first);
     c->h_plast = std::move(last);
