@@ -495,7 +495,7 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
             ST::st(f.Txx, q, txx[r], wi_xx);
             ST::st(f.Tzz, q, tzz[r], wi_zz);
             ST::st(f.Txz, q, txz[r], wi_xz);
-            if (ST::scaled) {
+            if (ST::scaled && amax) { // the maxima are gathered on the step before a check
                 mx_xx = fmaxf(mx_xx, fabsf((float)txx[r]));
                 mx_zz = fmaxf(mx_zz, fabsf((float)tzz[r]));
                 mx_xz = fmaxf(mx_xz, fabsf((float)txz[r]));
@@ -725,7 +725,7 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
             const long q = base + (long)gz * px + gx;
             ST::st(f.U, q, nv[0][r], wi_u);
             ST::st(f.W, q, nv[1][r], wi_w);
-            if (ST::scaled) {
+            if (ST::scaled && amax) { // the maxima are gathered on the step before a check
                 mx_u = fmaxf(mx_u, fabsf((float)nv[0][r]));
                 mx_w = fmaxf(mx_w, fabsf((float)nv[1][r]));
             }
@@ -901,7 +901,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
             const long q = base + (long)gz * px + gx;
             ST::st(f.Tx, q, vx[r], wi_x);
             ST::st(f.Tz, q, vz[r], wi_z);
-            if (ST::scaled) {
+            if (ST::scaled && amax) { // the maxima are gathered on the step before a check
                 mx_x = fmaxf(mx_x, fabsf((float)vx[r]));
                 mx_z = fmaxf(mx_z, fabsf((float)vz[r]));
             }
@@ -1070,7 +1070,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
             const int gz = z0 + lz0 + r;
             if (gx >= nx || gz >= nz) continue;
             ST::st(f.V, base + (long)gz * px + gx, nv[0][r], wi_v);
-            if (ST::scaled) mx_v = fmaxf(mx_v, fabsf((float)nv[0][r]));
+            if (ST::scaled && amax) mx_v = fmaxf(mx_v, fabsf((float)nv[0][r]));
         }
         if (ev.clock) fused_record<C, RZ, 1>(ev, it, t.tile * g.nshots + shot, sX, sX, nv, lz0);
     }
