@@ -529,10 +529,10 @@ def run_psv(sim, nt, src, M, xs, zs, kind=0, rx=None, rz=None, rxw=None, rzw=Non
     return np.array(su), np.array(sw)
 
 
-def run_sh(sim, nt, src, M, xs, zs, kind=0, rx=None, rz=None, itrecord=1):
+def run_sh(sim, nt, src, M, xs, zs, kind=0, rx=None, rz=None, itrecord=1, it0=0):
     """The per-step order of SH/main.cpp:93-141; returns V seismograms."""
     sv = []
-    for it in range(nt):
+    for it in range(it0, it0 + nt):
         sim.stress(it)
         sim.inject(kind, it, src, M, xs, zs)
         sim.velocity(it)

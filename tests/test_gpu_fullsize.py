@@ -21,7 +21,9 @@ the parity tests pin the same kernels to it on smaller grids):
   antisymmetric about that column, with each field's half-cell staggering,
   bitwise;
 - batch == single: shots of the 8-shot 4096x2048 configuration run in one
-  context equal the same shots run alone, bitwise.
+  context equal the same shots run alone, bitwise;
+- the 400x200 SH configuration's full run and the first and last shot of
+  the 64-shot 1024x512 batch (300 steps) against the FP32 oracle (1e-5).
 """
 import numpy as np
 import pytest
@@ -119,3 +121,91 @@ def test_c4_batch_equals_single_shots():
         _print("shot", k, st1[0] == states[k])
         assert st1[0] == states[k]
         assert np.array_equal(s1, seis[2 * k:2 * k + 2])
+
+
+def _oracle_for(wl, shot, nt):
+    """The oracle's run of one shot of a workload (same material, C-PML,
+    wavelet and moment tensor as the context gets), FP32."""
+    import oracle as O
+    from helpers import unpack_cpml
+
+    cp = unpack_cpml(wl.packed, wl.wave, wl.nx, wl.nz, np.float32)
+    xs, zs = wl.sources[shot]
+    rx, rz = wl.receivers()
+    stf, M = wl.stf.astype(np.float32), wl.M.astype(np.float32)
+    if wl.wave == _lib.WAVE_PSV:
+        sim = O.PSV(wl.nx, wl.nz, wl.npml, dict(zip(("BU", "BW", "MU", "MUA", "LAM"), wl.mat)), cp, wl.topabsorb)
+        kind = {_lib.SRC_LINE: 0, _lib.SRC_P: 1}[wl.kind]
+        u, w = O.run_psv(sim, nt, stf, M, xs, zs, kind=kind, rx=rx, rz=rz)
+        return sim, [u, w]
+    sim = O.SH(wl.nx, wl.nz, wl.npml, dict(zip(("BV", "MUX", "MUZ"), wl.mat)), cp, wl.topabsorb)
+    v = O.run_sh(sim, nt, stf, M, xs, zs, kind=_lib.SRC_FORCE, rx=rx, rz=rz)
+    return sim, [v]
+
+
+def test_c1_sh_full_run_vs_oracle():
+    """The 400x200 SH configuration over its 2000 steps against the FP32
+    oracle: seismograms within 1e-5 relative L2; the fields at step 700
+    (the wave inside the grid) within 1e-5 relative L2, and at the end,
+    when the absorbed wave has left small residuals, within 1e-5 of the
+    step-700 peak.  The receivers at z=2 lie in the frozen rows of the
+    absorbing top (C-PML profiles are zero for the first 4 rows,
+    cpml.cpp:54-64): rows 0-3 only couple along x and stay zero, so both
+    sides record zeros there."""
+    import oracle as O
+    from helpers import unpack_cpml
+
+    _print("case: sh 400x200 full run vs oracle")
+    wl = W.config_sh_400x200().build()
+    names = ("V", "Tx", "Tz")
+    mid = 700
+    ctx = wl.context()
+    try:
+        ctx.run(0, mid, 1)
+        f_mid = {n: ctx.field(n) for n in names}
+        ctx.run(mid, wl.nt - mid, 1)
+        ctx.sync()
+        g = ctx.seismograms(0, 0, wl.nt)
+        f_end = {n: ctx.field(n) for n in names}
+    finally:
+        ctx.close()
+    cp = unpack_cpml(wl.packed, wl.wave, wl.nx, wl.nz, np.float32)
+    sim = O.SH(wl.nx, wl.nz, wl.npml, dict(zip(("BV", "MUX", "MUZ"), wl.mat)), cp, wl.topabsorb)
+    rx, rz = wl.receivers()
+    stf, M = wl.stf.astype(np.float32), wl.M.astype(np.float32)
+    xs, zs = wl.sources[0]
+    v1 = O.run_sh(sim, mid, stf, M, xs, zs, kind=_lib.SRC_FORCE, rx=rx, rz=rz)
+    o_mid = {n: getattr(sim, n).copy() for n in names}
+    v2 = O.run_sh(sim, wl.nt - mid, stf, M, xs, zs, kind=_lib.SRC_FORCE, rx=rx, rz=rz, it0=mid)
+    e = rel(g, np.concatenate([v1, v2]))
+    _print("seis", e)
+    assert e < 1e-5
+    for n in names:
+        ref, peak = o_mid[n], float(np.abs(o_mid[n]).max())
+        e_mid = rel(f_mid[n], ref)
+        e_end = float(np.abs(f_end[n].astype(np.float64) - getattr(sim, n)).max() / peak)
+        _print(n, "step", mid, e_mid, "end, of the step-%d peak" % mid, e_end)
+        assert peak > 0 and e_mid < 1e-5 and e_end < 1e-5
+        assert not ref[:4].any() and not f_mid[n][:4].any()  # the frozen top rows
+
+
+def test_c3_batch_first_and_last_shot_vs_oracle():
+    """The 1024x512 x 64-shot batch (300 steps): the first and the last
+    shot of the one launch against the FP32 oracle run of that shot."""
+    _print("case: 1024x512 x64 shots 0 and 63 vs oracle, 300 steps")
+    nt = 300
+    wl = W.config_psv_batched(64).build()
+    ctx = wl.context(max_frames=nt)
+    try:
+        ctx.run(0, nt, 1)
+        ctx.sync()
+        got = {s: [ctx.seismograms(s, c, nt) for c in range(2)] for s in (0, 63)}
+        fld = {s: ctx.field("Txx", s) for s in (0, 63)}
+    finally:
+        ctx.close()
+    for s in (0, 63):
+        sim, ref = _oracle_for(wl, s, nt)
+        errs = [rel(got[s][c], ref[c]) for c in range(2)] + [rel(fld[s], sim.Txx)]
+        _print("shot", s, errs)
+        assert np.abs(ref[0]).max() > 0
+        assert max(errs) < 1e-5
