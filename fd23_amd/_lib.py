@@ -109,6 +109,12 @@ SIGNATURES = {
     "fd23_psv_moment": (c_int, [c_int, c_int, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_vp]),
     "fd23_sh_moment": (c_int, [c_int, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_vp]),
     "fd23_stf_gaussian": (c_int, [c_int, c_vp, c_int, P_int]),
+    "fd23_stf_gaussian_f64": (c_int, [c_int, c_vp, c_int, P_int]),
+    "fd23_stf_trapezoid_f64": (c_int, [c_dbl, c_dbl, c_dbl, c_dbl, c_vp, c_int, P_int]),
+    "fd23_psv_moment_f64": (c_int, [c_int, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_vp]),
+    "fd23_model_read_f64": (c_int, [ctypes.c_char_p, c_int, c_int, c_dbl, c_vp, c_vp, c_vp]),
+    "fd23_isis_write_f64": (c_int, [ctypes.c_char_p, c_vp, c_int, c_int, c_vp, c_vp, c_dbl, c_dbl, c_dbl, c_dbl,
+                                    c_int, c_int, c_int, c_dbl, c_dbl, c_int]),
     "fd23_stf_trapezoid": (c_int, [c_dbl, c_dbl, c_dbl, c_dbl, c_vp, c_int, P_int]),
     "fd23_stf_ricker": (c_int, [c_int, c_dbl, c_dbl, c_dbl, c_vp]),
     "fd23_model_read": (c_int, [ctypes.c_char_p, c_int, c_int, c_dbl, c_vp, c_vp, c_vp]),

@@ -51,7 +51,8 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
-CLI_NAMES = ("nbpsv2d", "nbpsv2d_topabsorb", "nbpsv2d_exp", "nbpsv2d_exp_topabsorb", "sh2d", "sh2d_topabsorb")
+CLI_NAMES = ("nbpsv2d", "nbpsv2d_topabsorb", "nbpsv2d_exp", "nbpsv2d_exp_topabsorb", "sh2d", "sh2d_topabsorb",
+             "psv2d_db", "psv2d_db_topabsorb")
 
 
 def build_cli(verbose: bool = False) -> str:

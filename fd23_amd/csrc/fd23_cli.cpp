@@ -1,9 +1,11 @@
 // Command-line driver with the reference programs' interface, on top of the
 // C ABI only.  One executable; the name it is started under picks the
-// reference build variant (PSV/clean_nobox/Makefile, SH/Makefile):
+// reference build variant (PSV/clean_nobox/Makefile, SH/Makefile,
+// PSV/psv_clean_double/Makefile):
 //   nbpsv2d, nbpsv2d_topabsorb, nbpsv2d_exp, nbpsv2d_exp_topabsorb, sh2d,
-//   sh2d_topabsorb; under any other name the par keys wave=psv|sh,
-//   topabsorb=0|1 and explosion=0|1 choose.
+//   sh2d_topabsorb, psv2d_db, psv2d_db_topabsorb (FP64: double parameters,
+//   model, wavelet and moment, {output}_U.db.isis); under any other name
+//   the par keys wave=psv|sh, topabsorb=0|1 and explosion=0|1 choose.
 //
 // P-SV follows PSV/clean_nobox/main.cpp:24-330 and param/param.cpp:94-193;
 // SH follows SH/main.cpp:20-200 and SH/param/param.cpp:12-84:
@@ -41,6 +43,7 @@ struct Par {
     }
     bool has_i(const char *k, int &v) { return fd23_par_int(p, k, &v) == 1; }
     bool has_f(const char *k, float &v) { return fd23_par_float(p, k, &v) == 1; }
+    bool has_d(const char *k, double &v) { return fd23_par_double(p, k, &v) == 1; }
     bool has_s(const char *k, std::string &v) {
         char buf[1024];
         if (fd23_par_string(p, k, buf, sizeof buf) != 1) return false;
@@ -56,6 +59,11 @@ struct Par {
     float mf(const char *k) {
         float v;
         if (!has_f(k, v)) missing(k);
+        return v;
+    }
+    double md(const char *k) {
+        double v;
+        if (!has_d(k, v)) missing(k);
         return v;
     }
     std::string ms(const char *k) {
@@ -117,7 +125,13 @@ int main(int argc, char **argv) {
     CHECK(fd23_par_open(argc, (const char *const *)argv, &par.p));
 
     bool psv = true, top = false, explosion = false;
-    if (prog.rfind("nbpsv2d", 0) == 0 || prog.rfind("sh2d", 0) == 0) {
+    // the FP64 tree (PSV/psv_clean_double/Makefile: psv2d_db[_topabsorb]):
+    // its par keys (param.cpp:64-154), double parameters, double host steps
+    // and the .db.isis writer
+    const bool dbl = prog.rfind("psv2d_db", 0) == 0;
+    if (dbl) {
+        top = prog.find("topabsorb") != std::string::npos;
+    } else if (prog.rfind("nbpsv2d", 0) == 0 || prog.rfind("sh2d", 0) == 0) {
         psv = prog.rfind("nbpsv2d", 0) == 0;
         top = prog.find("topabsorb") != std::string::npos;
         explosion = prog.find("_exp") != std::string::npos;
@@ -137,13 +151,24 @@ int main(int argc, char **argv) {
             else if (st == "f64") storage = FD23_STORE_F64;
             else if (st != "f32") Par::missing("storage (f32|f16|bf16|f64)");
         }
+        if (dbl) storage = FD23_STORE_F64;
     }
 
     // ---- parameters (P-SV param.cpp:94-193, SH param.cpp:12-84) --------
     int restart = 0, nt0 = 0, savestate = 0, itprint = 100, itrecord = 1, ntsnap;
     int usepunch = 0;
     int v;
-    if (psv) {
+    // real parameters: float in the FP32 trees, double in the FP64 one
+    auto real = [&](const char *k) { return dbl ? par.md(k) : (double)par.mf(k); };
+    if (dbl) {
+        if (par.mi("usebox") == 1) {
+            fprintf(stderr, "usebox=1 (box source files) is not supported by this driver: the plane-wave box is "
+                            "fd23_box_configure\n");
+            return -1;
+        }
+        usepunch = par.mi("usepunch");
+        par.mi("ngpu");
+    } else if (psv) {
         par.has_i("restart", restart);
         if (restart) nt0 = par.mi("nt0");
         par.has_i("savestate", savestate);
@@ -152,30 +177,35 @@ int main(int argc, char **argv) {
         par.mi("ngpu");
     }
     const int nx = par.mi("nx"), nz = par.mi("nz"), nt = par.mi("nt");
-    const float h = par.mf("h"), dt = par.mf("dt");
+    const double h = real("h"), dt = real("dt");
     char srctype = '1';
     int stype = 3;
+    double src_shift_x = 0.25, src_shift_z = -0.25; // FP64 tree, param.cpp:87-91
     std::string s;
-    if (psv) {
+    if (dbl) {
+        par.has_i("stype", stype);
+        srctype = stype == 13 ? 'p' : '1';
+        par.has_d("src_shift_x", src_shift_x);
+        par.has_d("src_shift_z", src_shift_z);
+    } else if (psv) {
         if (par.has_s("srctype", s) && !s.empty()) srctype = s[0];
     } else {
         par.has_i("stype", stype);
     }
-    const std::string srctime = par.ms(psv ? "srctime" : "sourcetime");
-    float trap1 = 0, trap2 = 0, trap3 = 0, alpha = 0;
+    const std::string srctime = par.ms(psv && !dbl ? "srctime" : "sourcetime");
+    double trap1 = 0, trap2 = 0, trap3 = 0, alpha = 0;
     if (srctime[0] == 'T' || srctime[0] == 't') {
-        trap1 = par.mf("trap1");
-        trap2 = par.mf("trap2");
-        trap3 = par.mf("trap3");
+        trap1 = real("trap1");
+        trap2 = real("trap2");
+        trap3 = real("trap3");
     }
-    if (srctime[0] == 'G' || srctime[0] == 'g') alpha = par.mf("alpha");
-    const float strike = par.mf("strike"), dip = par.mf("dip"), rake = par.mf("rake"), azimuth = par.mf("azimuth");
+    if (srctime[0] == 'G' || srctime[0] == 'g') alpha = real("alpha");
+    const double strike = real("strike"), dip = real("dip"), rake = real("rake"), azimuth = real("azimuth");
     const int xs = par.mi("xs"), zs = par.mi("zs");
     par.has_i("itprint", itprint);
     const int npml = par.mi("npml");
-    const float pml_dt = par.mf("pml_dt"), pml_v = par.mf("pml_v"), pml_r = par.mf("pml_r"),
-                pml_fc = par.mf("pml_fc");
-    if (psv)
+    const double pml_dt = real("pml_dt"), pml_v = real("pml_v"), pml_r = real("pml_r"), pml_fc = real("pml_fc");
+    if (psv && !dbl)
         par.has_i("itrecord", itrecord);
     else
         itrecord = par.mi("itrecord");
@@ -218,11 +248,11 @@ int main(int argc, char **argv) {
         }
     }
     const std::string model = par.ms("model");
-    if (psv) par.has_i("usetable", v);
+    if (psv && !dbl) par.has_i("usetable", v);
     ntsnap = par.mi("ntsnap");
     std::string snapfile;
     if (!psv) par.has_s("ntsnapfile", snapfile);
-    if (psv) par.has_i("usepunch", usepunch);
+    if (psv && !dbl) par.has_i("usepunch", usepunch);
     float before = 0, after = 0;
     std::string timefile;
     if (usepunch == 1) { // P-SV param.cpp:187-191, SH param.cpp:80-84
@@ -245,7 +275,15 @@ int main(int argc, char **argv) {
         }
     }
     std::vector<float> vp(N), vs(N), den(N);
-    CHECK(fd23_model_read(model.c_str(), nx, nz, h, vp.data(), vs.data(), den.data()));
+    std::vector<double> vpd, vsd, dend; // the FP64 tree's model, read in double
+    if (dbl) {
+        vpd.resize(N);
+        vsd.resize(N);
+        dend.resize(N);
+        CHECK(fd23_model_read_f64(model.c_str(), nx, nz, h, vpd.data(), vsd.data(), dend.data()));
+    } else {
+        CHECK(fd23_model_read(model.c_str(), nx, nz, h, vp.data(), vs.data(), den.data()));
+    }
     const bool f64 = storage == FD23_STORE_F64;
     const int nmat = psv ? 5 : 3;
     std::vector<std::vector<float>> matf(nmat, std::vector<float>(N));
@@ -254,9 +292,13 @@ int main(int argc, char **argv) {
     int mr;
     if (psv && f64) {
         matd.assign(nmat, std::vector<double>(N));
-        std::vector<double> a(vp.begin(), vp.end()), b(vs.begin(), vs.end()), c(den.begin(), den.end());
-        mr = fd23_psv_material_f64(nx, nz, h, dt, a.data(), b.data(), c.data(), matd[0].data(), matd[1].data(),
-                                   matd[2].data(), matd[3].data(), matd[4].data(), &stab);
+        if (!dbl) { // storage=f64 on an FP32 tree's inputs: the float model widened
+            vpd.assign(vp.begin(), vp.end());
+            vsd.assign(vs.begin(), vs.end());
+            dend.assign(den.begin(), den.end());
+        }
+        mr = fd23_psv_material_f64(nx, nz, h, dt, vpd.data(), vsd.data(), dend.data(), matd[0].data(),
+                                   matd[1].data(), matd[2].data(), matd[3].data(), matd[4].data(), &stab);
     } else if (psv) {
         mr = fd23_psv_material(nx, nz, h, dt, vp.data(), vs.data(), den.data(), matf[0].data(), matf[1].data(),
                                matf[2].data(), matf[3].data(), matf[4].data(), &stab);
@@ -283,8 +325,12 @@ int main(int argc, char **argv) {
     // ---- source ----------------------------------------------------------
     std::vector<double> stf(100000);
     int lsrc = 0;
-    if (srctime[0] == 'T' || srctime[0] == 't')
+    if ((srctime[0] == 'T' || srctime[0] == 't') && dbl)
+        CHECK(fd23_stf_trapezoid_f64(dt, trap1, trap2, trap3, stf.data(), (int)stf.size(), &lsrc));
+    else if (srctime[0] == 'T' || srctime[0] == 't')
         CHECK(fd23_stf_trapezoid(dt, trap1, trap2, trap3, stf.data(), (int)stf.size(), &lsrc));
+    else if ((srctime[0] == 'G' || srctime[0] == 'g') && dbl)
+        CHECK(fd23_stf_gaussian_f64((int)alpha, stf.data(), (int)stf.size(), &lsrc));
     else if (srctime[0] == 'G' || srctime[0] == 'g')
         CHECK(fd23_stf_gaussian((int)alpha, stf.data(), (int)stf.size(), &lsrc));
     else {
@@ -305,7 +351,13 @@ int main(int argc, char **argv) {
     } else {
         kind = stype == 13 ? FD23_SRC_P : FD23_SRC_LINE; // other stypes fall back to 3 (SH/main.cpp:104-112)
     }
-    if (kind == FD23_SRC_P) { // running sum in float (main.cpp:94-99)
+    if (kind == FD23_SRC_P && dbl) { // running sum in double (PSV/psv_clean_double/main.cpp:105-110)
+        double sum = 0;
+        for (auto &x : stf) {
+            sum += x;
+            x = sum;
+        }
+    } else if (kind == FD23_SRC_P) { // running sum in float (main.cpp:94-99)
         float sum = 0;
         for (auto &x : stf) {
             sum += (float)x;
@@ -313,7 +365,9 @@ int main(int argc, char **argv) {
         }
     }
     double M[3];
-    if (psv)
+    if (dbl)
+        CHECK(fd23_psv_moment_f64(kind, strike, dip, rake, azimuth, h, dt, M));
+    else if (psv)
         CHECK(fd23_psv_moment(kind, explosion ? 1 : 0, strike, dip, rake, azimuth, h, dt, M));
     else
         CHECK(fd23_sh_moment(kind, strike, dip, rake, azimuth, h, dt, M));
@@ -409,6 +463,13 @@ int main(int argc, char **argv) {
         write_file(raw, all.data() + (size_t)first_frame * n * csize, (size_t)nframes * n * csize);
         // toisis reads floor(nt/itrecord) frames of the raw file (isis.cpp:20)
         const int ntrec = std::min(nt / itrecord, nframes);
+        if (dbl) { // double samples behind 136-byte headers, {raw}.db.isis
+            CHECK(fd23_isis_write_f64((raw + ".db.isis").c_str(),
+                                      (const double *)all.data() + (size_t)first_frame * n, n, ntrec, rx[c].data(),
+                                      rz[c].data(), shx[c], shz[c], h, dt, itrecord, xs, zs, src_shift_x,
+                                      src_shift_z, 0));
+            continue;
+        }
         std::vector<float> tr((size_t)ntrec * n);
         for (size_t q = 0; q < tr.size(); q++)
             tr[q] = f64 ? (float)((const double *)all.data())[(size_t)first_frame * n + q]

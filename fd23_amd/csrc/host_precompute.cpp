@@ -241,6 +241,25 @@ extern "C" int fd23_psv_moment(int kind, int explosion, double strike, double di
     return FD23_OK;
 }
 
+// the FP64 tree's moment tensor (PSV/psv_clean_double/point_source/
+// cu_point_source.cu:11-27, the _p factor :90): double throughout
+extern "C" int fd23_psv_moment_f64(int kind, double strike, double dip, double rake, double azimuth, double h,
+                                   double dt, double *M) {
+    if (!M) return herr(FD23_EINVAL, "psv_moment: null output");
+    const double delta = dip * kPI / 180.0, lambda = rake * kPI / 180.0, phi = (strike - azimuth) * kPI / 180.0;
+    double Mxx = -(std::sin(delta) * std::cos(lambda) * std::sin(2.0 * phi) +
+                   std::sin(2.0 * delta) * std::sin(lambda) * std::sin(phi) * std::sin(phi));
+    double Mxz = -(std::cos(delta) * std::cos(lambda) * std::cos(phi) + std::cos(2.0 * delta) * std::sin(lambda) *
+                                                                           std::sin(phi));
+    double Mzz = std::sin(2.0 * delta) * std::sin(lambda);
+    double factor = 1.0 / kPI / 1.4142135623731 / (h * h);
+    if (kind == FD23_SRC_P) factor = factor * dt / h;
+    M[0] = Mxx * factor;
+    M[1] = Mxz * factor;
+    M[2] = Mzz * factor;
+    return FD23_OK;
+}
+
 extern "C" int fd23_sh_moment(int kind, double strike, double dip, double rake, double azimuth, double h, double dt,
                               double *M) {
     if (!M) return herr(FD23_EINVAL, "sh_moment: null output");
@@ -284,6 +303,42 @@ extern "C" int fd23_stf_gaussian(int alpha, double *out, int cap, int *lsrc) {
     return FD23_OK;
 }
 
+// the FP64 tree's Gaussian (PSV/psv_clean_double/source/waveform.cpp:27-47):
+// double arithmetic, alpha an int
+extern "C" int fd23_stf_gaussian_f64(int alpha, double *out, int cap, int *lsrc) {
+    const int aint = std::abs((int)(alpha * 3.0));
+    const int n = 2 * aint + 1;
+    if (lsrc) *lsrc = n;
+    if (!out || cap < n) return herr(FD23_EINVAL, "stf_gaussian: output too small");
+    if (alpha >= 0) {
+        for (int i = -aint; i <= aint; i++) out[i + aint] = (double)i * std::exp((double)(-i) * (double)i / (alpha * alpha));
+    } else {
+        double sum = 0.0;
+        for (int i = -aint; i <= aint; i++) {
+            out[i + aint] = std::exp((double)(-i) * (double)i / (alpha * alpha));
+            sum += out[i + aint];
+        }
+        for (int i = 0; i < n; i++) out[i] /= sum;
+    }
+    return FD23_OK;
+}
+
+// the FP64 tree's trapezoid (PSV/psv_clean_double/source/waveform.cpp:8-25)
+extern "C" int fd23_stf_trapezoid_f64(double dt, double trap1, double trap2, double trap3, double *out, int cap,
+                                      int *lsrc) {
+    int t1 = (int)(trap1 / dt), t2 = (int)(trap2 / dt), t3 = (int)(trap3 / dt);
+    t2 += t1;
+    const int n = t2 + t3;
+    if (lsrc) *lsrc = n;
+    if (!out || cap < n) return herr(FD23_EINVAL, "stf_trapezoid: output too small");
+    const double amp = (trap2 + 0.5 * (trap1 + trap3)) / dt;
+    int i, j = 0;
+    for (i = 0; i < t1; i++) out[j++] = i / (t1 * amp);
+    for (; i < t2; i++) out[j++] = 1.0 / amp;
+    for (i = t3; i > 0; i--) out[j++] = i / (t3 * amp);
+    return FD23_OK;
+}
+
 extern "C" int fd23_stf_trapezoid(double dt, double trap1, double trap2, double trap3, double *out, int cap,
                                   int *lsrc) {
     const float fdt = (float)dt, f1 = (float)trap1, f2 = (float)trap2, f3 = (float)trap3;
@@ -318,9 +373,12 @@ extern "C" int fd23_stf_ricker(int n, double dt, double f0, double t0, double *o
 // x = ix*h, z = iz*h with later polygons painted over earlier ones
 // (:175-237) and the crossing-number test of :437-469.
 // ---------------------------------------------------------------------------
+// values and vertices as parsed (double): the FP32 trees keep them as float
+// (SH/model/model.cpp:12-26), the FP64 tree as double
+// (PSV/psv_clean_double/model/model.cpp:12-26); readers round as theirs do
 struct Poly {
-    float vp, vs, den;
-    std::vector<std::pair<float, float>> v;
+    double vp, vs, den;
+    std::vector<std::pair<double, double>> v;
 };
 
 static bool ends_with(const std::string &s, const std::string &e) {
@@ -353,7 +411,7 @@ static int load_mdl(const std::string &path, std::vector<Poly> &polys) {
     if (!in) return herr(FD23_EIO, "cannot open model file " + path);
     std::vector<std::string> lines;
     for (std::string l; std::getline(in, l);) lines.push_back(l);
-    std::vector<std::pair<std::string, std::pair<float, float>>> defs;
+    std::vector<std::pair<std::string, std::pair<double, double>>> defs;
     bool have_back = false;
     for (size_t i = 0; i < lines.size();) {
         const std::string line = lines[i++];
@@ -364,7 +422,7 @@ static int load_mdl(const std::string &path, std::vector<Poly> &polys) {
         for (auto &ch : key) ch = (char)std::toupper((unsigned char)ch);
         if (key == "DEF") {
             if (a.size() != 4) return herr(FD23_EINVAL, "model: DEF needs 3 arguments");
-            defs.push_back({a[1], {(float)atof(a[2].c_str()), (float)atof(a[3].c_str())}});
+            defs.push_back({a[1], {atof(a[2].c_str()), atof(a[3].c_str())}});
             continue;
         }
         if (key != "BACK" && key != "POLY") return herr(FD23_EINVAL, "bad model command: " + line);
@@ -372,7 +430,7 @@ static int load_mdl(const std::string &path, std::vector<Poly> &polys) {
         double vp = atof(a[1].c_str()), vs = atof(a[2].c_str()), den = atof(a[3].c_str());
         if (vs < 0) vs = -vp * vs;
         if (den < 0) den = -vp * den;
-        Poly p{(float)vp, (float)vs, (float)den, {}};
+        Poly p{vp, vs, den, {}};
         if (key == "BACK") {
             if (polys.empty()) polys.push_back(p);
             else polys[0] = p;
@@ -392,7 +450,7 @@ static int load_mdl(const std::string &path, std::vector<Poly> &polys) {
                     if (!found) return herr(FD23_EINVAL, "undefined model symbol " + t[j]);
                 } else {
                     if (j + 1 >= t.size()) return herr(FD23_EINVAL, "model: odd vertex list");
-                    p.v.push_back({(float)atof(t[j].c_str()), (float)atof(t[j + 1].c_str())});
+                    p.v.push_back({atof(t[j].c_str()), atof(t[j + 1].c_str())});
                     j++;
                 }
             }
@@ -404,7 +462,7 @@ static int load_mdl(const std::string &path, std::vector<Poly> &polys) {
     return FD23_OK;
 }
 
-static int inside(double xp, double yp, const std::vector<std::pair<float, float>> &v) {
+template <class V> static int inside(double xp, double yp, const std::vector<std::pair<V, V>> &v) {
     const int n = (int)v.size();
     if (n == 0) return 0;
     int nleft = 0;
@@ -442,20 +500,21 @@ int fd23_mdl_flatten(const char *path, int nx, int nz, double h, MdlFlat &out) {
         out.count.push_back(p == 0 ? 0 : (int)P.v.size());
         for (auto &v : P.v) {
             if (p == 0) break;
-            out.vx.push_back(v.first);
-            out.vz.push_back(v.second);
-            x1 = std::min(x1, v.first);
-            x2 = std::max(x2, v.first);
-            z1 = std::min(z1, v.second);
-            z2 = std::max(z2, v.second);
+            const float vx = (float)v.first, vz = (float)v.second;
+            out.vx.push_back(vx);
+            out.vz.push_back(vz);
+            x1 = std::min(x1, vx);
+            x2 = std::max(x2, vx);
+            z1 = std::min(z1, vz);
+            z2 = std::max(z2, vz);
         }
         out.ix1.push_back(clampi((int)(x1 / hh - 1), 0, nx - 1));
         out.ix2.push_back(clampi((int)(x2 / hh + 1), 0, nx - 1));
         out.iz1.push_back(clampi((int)(z1 / hh - 1), 0, nz - 1));
         out.iz2.push_back(clampi((int)(z2 / hh + 1), 0, nz - 1));
-        out.vp.push_back(P.vp);
-        out.vs.push_back(P.vs);
-        out.den.push_back(P.den);
+        out.vp.push_back((float)P.vp);
+        out.vs.push_back((float)P.vs);
+        out.den.push_back((float)P.den);
     }
     return FD23_OK;
 }
@@ -497,6 +556,58 @@ extern "C" int fd23_model_read(const char *name, int nx, int nz, double h, float
         if (!in) return herr(FD23_EIO, fn + " does not exist");
         in.read((char *)dst[i], (std::streamsize)(N * sizeof(float)));
         if ((size_t)in.gcount() != N * sizeof(float)) return herr(FD23_EIO, fn + " is shorter than nx*nz floats");
+    }
+    return FD23_OK;
+}
+
+// The FP64 tree's reader (PSV/psv_clean_double/model/model.cpp:175-237,
+// util.cpp:12-27, material.cpp:101-150): .mdl polygons with double
+// vertices, values and cell bounds; raster files are float, widened.
+extern "C" int fd23_model_read_f64(const char *name, int nx, int nz, double h, double *vp, double *vs,
+                                   double *den) {
+    if (!name || nx < 1 || nz < 1 || !vp || !vs || !den) return herr(FD23_EINVAL, "model_read: bad arguments");
+    const std::string nm(name);
+    const size_t N = (size_t)nx * nz;
+    if (!ends_with(nm, "mdl")) {
+        std::vector<float> a(N), b(N), c(N);
+        int r = fd23_model_read(name, nx, nz, h, a.data(), b.data(), c.data());
+        if (r) return r;
+        for (size_t q = 0; q < N; q++) {
+            vp[q] = a[q];
+            vs[q] = b[q];
+            den[q] = c[q];
+        }
+        return FD23_OK;
+    }
+    std::vector<Poly> polys;
+    int r = load_mdl(nm, polys);
+    if (r) return r;
+    for (size_t q = 0; q < N; q++) {
+        vp[q] = polys[0].vp;
+        vs[q] = polys[0].vs;
+        den[q] = polys[0].den;
+    }
+    auto clampi = [](int a, int lo, int hi) { return a < lo ? lo : (a > hi ? hi : a); };
+    for (size_t p = 1; p < polys.size(); p++) {
+        const Poly &P = polys[p];
+        if (P.v.empty()) continue;
+        double x1 = 1E28, x2 = -1E28, z1 = 1E28, z2 = -1E28;
+        for (auto &v : P.v) {
+            x1 = std::min(x1, v.first);
+            x2 = std::max(x2, v.first);
+            z1 = std::min(z1, v.second);
+            z2 = std::max(z2, v.second);
+        }
+        const int ix1 = clampi((int)(x1 / h - 1), 0, nx - 1), ix2 = clampi((int)(x2 / h + 1), 0, nx - 1);
+        const int iz1 = clampi((int)(z1 / h - 1), 0, nz - 1), iz2 = clampi((int)(z2 / h + 1), 0, nz - 1);
+        for (int iz = iz1; iz <= iz2; iz++)
+            for (int ix = ix1; ix <= ix2; ix++)
+                if (inside(h * (double)ix, h * (double)iz, P.v)) {
+                    const size_t q = (size_t)iz * nx + ix;
+                    vp[q] = P.vp;
+                    vs[q] = P.vs;
+                    den[q] = P.den;
+                }
     }
     return FD23_OK;
 }
@@ -546,6 +657,51 @@ extern "C" int fd23_isis_write(const char *path, const float *raw, int nrec, int
         out.write((const char *)&hd, sizeof hd);
         for (int k = 0; k < ntrecord; k++) tr[k] = raw[(size_t)k * nrec + ir];
         out.write((const char *)tr.data(), (std::streamsize)(sizeof(float) * ntrecord));
+    }
+    return out ? FD23_OK : herr(FD23_EIO, std::string("write failed: ") + path);
+}
+
+// the FP64 tree's ISIS writer (PSV/psv_clean_double/record/isis.cpp:8-61,
+// record/isis.h:7-26): a 136-byte header of doubles with natural
+// alignment, the source position shifted by src_shift, user[1] = bw, double
+// samples.  Fields and padding the reference leaves uninitialised are 0.
+struct IsisHeader64 {
+    double xs, ys, zs, xr, yr, zr, t0;
+    int nt;
+    double samplerate;
+    int srcflagnum, recflagnum, status;
+    double gain, dtstatic, ampstatic;
+    unsigned short recorder, chan, scomp, rcomp;
+    int user[4];
+};
+static_assert(sizeof(IsisHeader64) == 136, "FP64 ISIS trace header is 136 bytes");
+
+extern "C" int fd23_isis_write_f64(const char *path, const double *raw, int nrec, int ntrecord, const int *rx,
+                                   const int *rz, double shiftx, double shiftz, double h, double dt, int itrecord,
+                                   int xs, int zs, double src_shift_x, double src_shift_z, int bw) {
+    if (!path || (!raw && nrec * ntrecord > 0) || !rx || !rz || nrec < 0 || ntrecord < 0)
+        return herr(FD23_EINVAL, "isis_write: bad arguments");
+    std::ofstream out(path, std::ios::binary);
+    if (!out) return herr(FD23_EIO, std::string("cannot create ") + path);
+    std::vector<double> tr(ntrecord);
+    for (int ir = 0; ir < nrec; ir++) {
+        IsisHeader64 hd;
+        memset(&hd, 0, sizeof hd);
+        hd.xr = (rx[ir] + shiftx) * h;
+        hd.zr = (rz[ir] + shiftz) * h;
+        hd.xs = (double)(xs + src_shift_x) * h;
+        hd.zs = (double)(zs + src_shift_z) * h;
+        hd.nt = ntrecord;
+        hd.samplerate = dt * (double)itrecord;
+        hd.gain = dt / h;
+        hd.ampstatic = 1.0;
+        hd.srcflagnum = xs;
+        hd.recflagnum = rx[ir];
+        hd.user[1] = bw;
+        hd.user[2] = 123456789;
+        out.write((const char *)&hd, sizeof hd);
+        for (int k = 0; k < ntrecord; k++) tr[k] = raw[(size_t)k * nrec + ir];
+        out.write((const char *)tr.data(), (std::streamsize)(sizeof(double) * ntrecord));
     }
     return out ? FD23_OK : herr(FD23_EIO, std::string("write failed: ") + path);
 }

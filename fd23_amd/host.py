@@ -102,6 +102,46 @@ def isis_write(path, raw, rx, rz, shiftx, shiftz, h, dt, itrecord, xs, zs):
                                       h, dt, itrecord, xs, zs))
 
 
+def psv_moment_f64(kind, strike, dip, rake, azimuth, h, dt):
+    """The FP64 tree's moment tensor (double throughout)."""
+    M = np.zeros(3, np.float64)
+    check(_lib.load().fd23_psv_moment_f64(kind, strike, dip, rake, azimuth, h, dt, _ptr(M)))
+    return M
+
+
+def gaussian_f64(alpha):
+    a = int(alpha)
+    cap = 2 * abs(int(a * 3.0)) + 2
+    out = np.zeros(cap, np.float64)
+    n = ctypes.c_int()
+    check(_lib.load().fd23_stf_gaussian_f64(a, _ptr(out), cap, ctypes.byref(n)))
+    return out[:n.value].copy()
+
+
+def trapezoid_f64(dt, t1, t2, t3):
+    cap = int(t1 / dt) + int(t2 / dt) + int(t3 / dt) + 8
+    out = np.zeros(cap, np.float64)
+    n = ctypes.c_int()
+    check(_lib.load().fd23_stf_trapezoid_f64(dt, t1, t2, t3, _ptr(out), cap, ctypes.byref(n)))
+    return out[:n.value].copy()
+
+
+def model_read_f64(name, nx, nz, h):
+    vp, vs, den = (np.zeros((nz, nx), np.float64) for _ in range(3))
+    check(_lib.load().fd23_model_read_f64(name.encode(), nx, nz, h, _ptr(vp), _ptr(vs), _ptr(den)))
+    return vp, vs, den
+
+
+def isis_write_f64(path, raw, rx, rz, shiftx, shiftz, h, dt, itrecord, xs, zs, src_shift_x=0.25,
+                   src_shift_z=-0.25, bw=0):
+    raw = np.ascontiguousarray(raw, np.float64)
+    rx = np.ascontiguousarray(rx, np.int32)
+    rz = np.ascontiguousarray(rz, np.int32)
+    ntrec, nrec = raw.shape
+    check(_lib.load().fd23_isis_write_f64(path.encode(), _ptr(raw), nrec, ntrec, _ptr(rx), _ptr(rz), shiftx,
+                                          shiftz, h, dt, itrecord, xs, zs, src_shift_x, src_shift_z, bw))
+
+
 def line_receivers(nsta, ix0, iz0, idx, idz):
     rx = np.zeros(2 * nsta, np.int32)
     rz = np.zeros(2 * nsta, np.int32)

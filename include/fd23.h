@@ -265,16 +265,35 @@ int fd23_stf_gaussian(int alpha, double *out, int cap, int *lsrc);
 int fd23_stf_trapezoid(double dt, double trap1, double trap2, double trap3,
                        double *out, int cap, int *lsrc);
 int fd23_stf_ricker(int n, double dt, double f0, double t0, double *out);
+/* the FP64 tree's wavelets and moment tensor, in double throughout
+ * (PSV/psv_clean_double/source/waveform.cpp:8-47,
+ * point_source/cu_point_source.cu:11-27,90; no explosion variant)       */
+int fd23_stf_gaussian_f64(int alpha, double *out, int cap, int *lsrc);
+int fd23_stf_trapezoid_f64(double dt, double trap1, double trap2, double trap3,
+                           double *out, int cap, int *lsrc);
+int fd23_psv_moment_f64(int kind, double strike, double dip, double rake,
+                        double azimuth, double h, double dt, double *M);
 /* model: "{name}.mdl" polygons (SH/model/model.cpp:175-469) or raster
  * "{name}.vp/.vs/.den" float32 files (SH/model/util.cpp:12-29)          */
 int fd23_model_read(const char *name, int nx, int nz, double h, float *vp,
                     float *vs, float *den);
+/* the FP64 tree's reader: .mdl vertices, values and cell bounds in double
+ * (PSV/psv_clean_double/model/model.cpp:175-237); raster files widened  */
+int fd23_model_read_f64(const char *name, int nx, int nz, double h, double *vp,
+                        double *vs, double *den);
 /* ISIS trace file (PSV/clean_nobox/record/isis.cpp:8-61): raw is
  * time-major [ntrecord][nrec]                                           */
 int fd23_isis_write(const char *path, const float *raw, int nrec,
                     int ntrecord, const int *rx, const int *rz, double shiftx,
                     double shiftz, double h, double dt, int itrecord, int xs,
                     int zs);
+/* the FP64 tree's ISIS file ({output}_U.db.isis, PSV/psv_clean_double/
+ * record/isis.cpp:8-61): 136-byte double header (source position shifted
+ * by src_shift_x/z, user[1] = bw), double samples                       */
+int fd23_isis_write_f64(const char *path, const double *raw, int nrec, int ntrecord,
+                        const int *rx, const int *rz, double shiftx, double shiftz,
+                        double h, double dt, int itrecord, int xs, int zs,
+                        double src_shift_x, double src_shift_z, int bw);
 
 /* receiver pairs (x, z), (x-1, z) per station: a line of nsta stations
  * (RECORD::init_line, PSV/clean_nobox/record/cu_record.cu:12-28) or a

@@ -64,3 +64,31 @@ def test_punch_keys(ttdir):
 def test_missing_model_is_an_error(ttdir):
     r = _run("nbpsv2d", ["par=t1", "model=none.mdl"], ttdir)
     assert r.returncode != 0 and "fd23_model_read" in r.stderr
+
+
+@pytest.fixture()
+def fp64dir(tmp_path):
+    d = os.path.join(ROOT, "tests", "golden", "ref", "psv_fp64")
+    for f in ("0.par", "homo.mdl", "station200"):
+        shutil.copy(os.path.join(d, f), tmp_path / f)
+    return tmp_path
+
+
+def test_fp64_tree_par_file(fp64dir):
+    """psv2d_db reads the FP64 tree's par file (PSV/psv_clean_double/test/
+    0.par, its keys: usebox, sourcetime, stype, src_shift_*) through every
+    host step; without a GPU it stops at context creation."""
+    r = _run("psv2d_db", ["par=0.par"], fp64dir)
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present: the GPU test runs this case to the end")
+    except Exception:
+        pass
+    assert r.returncode != 0 and "fd23_ctx_create" in r.stderr, r.stderr
+    r = _run("psv2d_db", ["par=0.par", "usebox=1"], fp64dir)
+    assert r.returncode != 0 and "usebox=1" in r.stderr
+    (fp64dir / "nobox.par").write_text(open(fp64dir / "0.par").read().replace("usebox=0", ""))
+    r = _run("psv2d_db", ["par=nobox.par"], fp64dir)
+    assert r.returncode != 0 and "must specify value for 'usebox'" in r.stderr

@@ -116,3 +116,34 @@ def test_tt_with_first_arrival_window(tmp_path):
           flush=True)
     assert e < 1e-5
     assert rel(want, np.frombuffer(free.state_bytes(), np.float32)) > 1e-3
+
+
+def test_fp64_tree_program_reproduces_the_reference_run(tmp_path):
+    """psv2d_db on the FP64 tree's own test case (PSV/psv_clean_double/test:
+    2048x1024, 2001 steps, the reference's 0_U, 0_W and 0_U.isis): the raw
+    double seismograms within 1e-10 relative L2, the ISIS headers byte for
+    byte (but the bytes the reference leaves uninitialised) and its samples
+    equal to the raw file."""
+    print("case: psv2d_db on psv_clean_double/test", flush=True)
+    d = os.path.join(ROOT, "tests", "golden", "ref", "psv_fp64")
+    for f in ("0.par", "homo.mdl", "station200"):
+        shutil.copy(os.path.join(d, f), tmp_path / f)
+    exe = os.path.join(ROOT, "fd23_amd", "bin", "psv2d_db")
+    r = subprocess.run([exe, "par=0.par"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-300:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0 and "finish" in r.stdout
+    for comp in ("U", "W"):
+        got = np.fromfile(tmp_path / ("0_" + comp), np.float64)
+        ref = np.fromfile(os.path.join(d, "0_" + comp), np.float64)
+        e = rel(got, ref)
+        print(comp, e, flush=True)
+        assert got.size == ref.size == 2001 * 36 and e < 1e-10
+    isis = np.frombuffer((tmp_path / "0_U.db.isis").read_bytes(), np.uint8).reshape(36, -1)
+    ref = np.frombuffer(open(os.path.join(d, "0_U.isis"), "rb").read(), np.uint8).reshape(36, -1)
+    assert isis.shape == ref.shape
+    mask = np.ones(136, bool)
+    mask[60:64] = mask[84:88] = mask[120:124] = False
+    assert np.array_equal(isis[:, :136][:, mask], ref[:, :136][:, mask])
+    raw = np.fromfile(tmp_path / "0_U", np.float64).reshape(2001, 36)
+    for i in range(36):
+        assert np.array_equal(isis[i, 136:].view(np.float64), raw[:, i])

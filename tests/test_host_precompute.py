@@ -210,3 +210,46 @@ def test_station_file_and_line_receivers():
             host.station_read(bad)
     with pytest.raises(_lib.Fd23Error):
         host.station_read("/nonexistent/stations")
+
+
+# ---------------------------------------------------------------------------
+# the FP64 tree's host steps (PSV/psv_clean_double): model, wavelets, moment,
+# and its ISIS file against the reference's own output of that tree
+# ---------------------------------------------------------------------------
+FP64_DIR = os.path.join(os.path.dirname(D), "ref", "psv_fp64")
+
+
+def test_fp64_model_wavelet_moment_match_oracle():
+    for path, nx, nz, h in ((os.path.join(FP64_DIR, "homo.mdl"), 2048, 1024, 0.3),):
+        got = host.model_read_f64(path, nx, nz, h)
+        want = O.mdl_raster(path, nx, nz, h, np.float64)
+        for a, b in zip(got, want):
+            assert a.dtype == np.float64 and np.array_equal(a, b)
+        # FP64 values are the decimal strings parsed as double, not widened floats
+        assert got[1].max() == 4.0 and 6.8 in got[0] and np.float64(np.float32(6.8)) not in got[0]
+    for a in (-5, -30, 3):
+        assert np.array_equal(host.gaussian_f64(a), O.gaussian(a, np.float64)), a
+    assert abs(host.gaussian_f64(-5).sum() - 1.0) < 1e-15
+    for kind in (0, 1):
+        M = host.psv_moment_f64(kind, 0.0, 20.0, 90.0, 45.0, 0.3, 0.01)
+        assert np.array_equal(M, O.psv_moment(kind, 0.0, 20.0, 90.0, 45.0, 0.3, 0.01, np.float64)), kind
+    # trapezoid: unit area in samples (int s dt = 1), the rise, flat and fall lengths
+    s = host.trapezoid_f64(0.01, 0.3, 0.0, 0.3)
+    assert len(s) == 60 and abs(s.sum() - 1.0) < 0.05 and s[0] == 0.0 and s.argmax() in (29, 30)
+
+
+def test_fp64_isis_reproduces_the_reference_file(tmp_path):
+    """PSV/psv_clean_double/test/0_U.isis: 36 traces of 2001 doubles behind
+    136-byte headers.  Written again from the raw file (0_U) with the run's
+    parameters (0.par), every byte agrees except the ones the reference
+    leaves uninitialised (the padding after nt and after status, user[0])."""
+    raw = np.fromfile(os.path.join(FP64_DIR, "0_U"), np.float64).reshape(2001, 36)
+    rx, rz = O.station_receivers(os.path.join(FP64_DIR, "station200"))
+    out = tmp_path / "0_U.db.isis"
+    host.isis_write_f64(str(out), raw, rx, rz, 0.5, 0.0, 0.3, 0.01, 1, 1024, 100, 0.25, -0.25, 0)
+    got = np.frombuffer(out.read_bytes(), np.uint8).reshape(36, -1)
+    ref = np.frombuffer(open(os.path.join(FP64_DIR, "0_U.isis"), "rb").read(), np.uint8).reshape(36, -1)
+    assert got.shape == ref.shape == (36, 136 + 8 * 2001)
+    mask = np.ones(got.shape[1], bool)
+    mask[60:64] = mask[84:88] = mask[120:124] = False
+    assert np.array_equal(got[:, mask], ref[:, mask])
