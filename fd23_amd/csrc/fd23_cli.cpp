@@ -16,6 +16,7 @@
 //   each snapshot and progress-print step) -> optional state%06d ->
 //   raw time-major seismograms {output}_U/_W or _V -> ISIS files.
 // Keys the one-GPU build reads but does not need: gpuid, ngpu, usetable.
+// snapbox (PSV/clean_nobox_new) writes windowed snapshots to ./OUTPUT/.
 // Additional key: storage=f32|f16|bf16|f64 (default f32).  usepunch=1
 // reads before, after and timefile (an nz*nx float32 travel-time grid) and
 // sets the first-arrival window (SH/slide/cu_slide.cu:7-61).
@@ -33,6 +34,8 @@
 #include <set>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
 
 namespace {
 
@@ -261,6 +264,25 @@ int main(int argc, char **argv) {
         timefile = par.ms("timefile");
     }
     const std::set<int> extra_snaps = snap_list(snapfile);
+    // snapbox=x1,x2,z1,z2,it1,it2 (PSV/clean_nobox_new/param/param.cpp:184-198,
+    // record/record.cpp:27-60): snapshots of that window only, for steps in
+    // [it1, it2], written to ./OUTPUT/
+    int snapbox[6] = {0, nx - 1, 0, nz - 1, 0, nt};
+    bool use_snapbox = false;
+    if (psv && !dbl && par.has_s("snapbox", s)) {
+        if (sscanf(s.c_str(), "%d,%d,%d,%d,%d,%d", &snapbox[0], &snapbox[1], &snapbox[2], &snapbox[3], &snapbox[4],
+                   &snapbox[5]) != 6) {
+            fprintf(stderr, "snapbox should be six integer");
+            return -1;
+        }
+        if (snapbox[0] < 0 || snapbox[1] >= nx || snapbox[0] > snapbox[1] || snapbox[2] < 0 || snapbox[3] >= nz ||
+            snapbox[2] > snapbox[3]) {
+            fprintf(stderr, "snapbox %s is outside the %dx%d grid\n", s.c_str(), nx, nz);
+            return -1;
+        }
+        use_snapbox = true;
+        mkdir("OUTPUT", 0777);
+    }
 
     // ---- model, material, C-PML -----------------------------------------
     const size_t N = (size_t)nx * nz;
@@ -425,11 +447,22 @@ int main(int argc, char **argv) {
         if ((e % ntsnap == 0 || extra_snaps.count(e)) && e != 0) {
             CHECK(fd23_sync(ctx));
             std::cout << e << std::endl;
-            for (int k = 0; k < (psv ? 2 : 1); k++) {
+            for (int k = 0; k < (psv ? 2 : 1) && (!use_snapbox || (e >= snapbox[4] && e <= snapbox[5])); k++) {
                 CHECK(fd23_get_field(ctx, 0, sfields[k], snap.data()));
-                char name[32];
-                snprintf(name, sizeof name, "%s%05d", snames[k], e);
-                write_file(name, snap.data(), N * csize);
+                char name[48];
+                if (!use_snapbox) {
+                    snprintf(name, sizeof name, "%s%05d", snames[k], e);
+                    write_file(name, snap.data(), N * csize);
+                    continue;
+                }
+                // rows z1..z2 of columns x1..x2
+                const size_t w = (size_t)(snapbox[1] - snapbox[0] + 1) * csize;
+                std::vector<char> win(w * (snapbox[3] - snapbox[2] + 1));
+                for (int z = snapbox[2]; z <= snapbox[3]; z++)
+                    memcpy(win.data() + (size_t)(z - snapbox[2]) * w,
+                           snap.data() + ((size_t)z * nx + snapbox[0]) * csize, w);
+                snprintf(name, sizeof name, "./OUTPUT/%s%05d", snames[k], e);
+                write_file(name, win.data(), win.size());
             }
         }
         if (e % itprint == 0) {

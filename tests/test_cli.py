@@ -92,3 +92,10 @@ def test_fp64_tree_par_file(fp64dir):
     (fp64dir / "nobox.par").write_text(open(fp64dir / "0.par").read().replace("usebox=0", ""))
     r = _run("psv2d_db", ["par=nobox.par"], fp64dir)
     assert r.returncode != 0 and "must specify value for 'usebox'" in r.stderr
+
+
+def test_snapbox_is_checked(ttdir):
+    r = _run("nbpsv2d", ["par=t1", "snapbox=1,2,3"], ttdir)
+    assert r.returncode != 0 and "snapbox should be six integer" in r.stderr
+    r = _run("nbpsv2d", ["par=t1", "snapbox=0,96,0,10,0,100"], ttdir)
+    assert r.returncode != 0 and "outside" in r.stderr

@@ -147,3 +147,26 @@ def test_fp64_tree_program_reproduces_the_reference_run(tmp_path):
     raw = np.fromfile(tmp_path / "0_U", np.float64).reshape(2001, 36)
     for i in range(36):
         assert np.array_equal(isis[i, 136:].view(np.float64), raw[:, i])
+
+
+def test_snapbox_windows_the_snapshots(tmp_path):
+    """snapbox=x1,x2,z1,z2,it1,it2 (PSV/clean_nobox_new): the snapshot files
+    in ./OUTPUT hold that window of the full snapshots, for steps in
+    [it1, it2] only."""
+    print("case: driver snapbox", flush=True)
+    for f in ("t1", "station200"):
+        shutil.copy(os.path.join(TT, f), tmp_path / f)
+    shutil.copy(os.path.join(TT, "homo_6.8.mdl"), tmp_path / "homo.mdl")
+    par = open(os.path.join(TT, "t1")).read().replace("nt=500", "nt=300")
+    (tmp_path / "tb").write_text(par)
+    r = subprocess.run([EXE, "par=tb"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-300:]
+    full = {n: np.fromfile(tmp_path / n, np.float32).reshape(96, 96) for n in ("U00100", "W00200")}
+    r = subprocess.run([EXE, "par=tb", "snapbox=10,50,20,60,100,200"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-300:]
+    out = tmp_path / "OUTPUT"
+    assert sorted(os.listdir(out)) == ["U00100", "U00200", "W00100", "W00200"]
+    for n, f in full.items():
+        win = np.fromfile(out / n, np.float32).reshape(41, 41)
+        assert np.abs(f).max() > 0 and np.array_equal(win, f[20:61, 10:51]), n
