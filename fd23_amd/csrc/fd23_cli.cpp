@@ -306,6 +306,20 @@ int main(int argc, char **argv) {
     } else {
         CHECK(fd23_model_read(model.c_str(), nx, nz, h, vp.data(), vs.data(), den.data()));
     }
+    // SH output_material=1 (SH/param/param.cpp:12-13, SH/material/
+    // material.cpp:50-67): a .mdl model's vs and den as {model}_VS, _DEN,
+    // two int sizes then nx*nz floats
+    if (!psv && par.has_i("output_material", v) && v == 1 && model.size() > 4 &&
+        model.compare(model.size() - 4, 4, ".mdl") == 0) {
+        const std::pair<const char *, const std::vector<float> *> outs[2] = {{"_VS", &vs}, {"_DEN", &den}};
+        for (auto &o : outs) {
+            std::vector<char> buf(2 * sizeof(int) + N * sizeof(float));
+            memcpy(buf.data(), &nx, sizeof(int));
+            memcpy(buf.data() + sizeof(int), &nz, sizeof(int));
+            memcpy(buf.data() + 2 * sizeof(int), o.second->data(), N * sizeof(float));
+            write_file(model + o.first, buf.data(), buf.size());
+        }
+    }
     const bool f64 = storage == FD23_STORE_F64;
     const int nmat = psv ? 5 : 3;
     std::vector<std::vector<float>> matf(nmat, std::vector<float>(N));

@@ -99,3 +99,24 @@ def test_snapbox_is_checked(ttdir):
     assert r.returncode != 0 and "snapbox should be six integer" in r.stderr
     r = _run("nbpsv2d", ["par=t1", "snapbox=0,96,0,10,0,100"], ttdir)
     assert r.returncode != 0 and "outside" in r.stderr
+
+
+def test_sh_output_material(tmp_path):
+    """sh2d output_material=1 writes the .mdl model's vs and den
+    ({model}_VS, _DEN: two int sizes, then nx*nz floats) before any device
+    work; they equal the library's model reader."""
+    import numpy as np
+
+    from fd23_amd import host
+
+    shutil.copy(os.path.join(TT, "homo_6.8.mdl"), tmp_path / "homo.mdl")
+    par = ("nx=96 nz=96 nt=10 h=0.3 dt=0.01 usepunch=0 ngpu=1 sourcetime=Gaussian alpha=-30 strike=0 dip=20 "
+           "rake=90 azimuth=45 xs=33 zs=33 npml=32 pml_dt=0.005 pml_v=30 pml_r=1e-11 pml_fc=2 itrecord=1 "
+           "output=o nrec=2 ixrec0=10 izrec0_v=10 idxrec=1 idzrec=0 model=homo.mdl ntsnap=100 "
+           "output_material=1").split()
+    _run("sh2d", par, tmp_path)
+    _, vs, den = host.model_read(str(tmp_path / "homo.mdl"), 96, 96, 0.3)
+    for name, ref in (("homo.mdl_VS", vs), ("homo.mdl_DEN", den)):
+        b = (tmp_path / name).read_bytes()
+        assert np.frombuffer(b[:8], np.int32).tolist() == [96, 96]
+        assert np.array_equal(np.frombuffer(b[8:], np.float32).reshape(96, 96), ref)
