@@ -1638,8 +1638,11 @@ extern "C" int fd23_autotune(fd23_ctx *c, int steps, double *best_ms) {
     double bt = 1e30;
     for (int t = 0; t < kNumTiles; t++) {
         if (kTiles[t].spb > 1 && c->d.nshots < kTiles[t].spb) continue;
-        if (kTiles[t].split && !is_psv(c)) continue; // SH runs split shapes unsplit
-        if (tile_fused(t) && (!is_psv(c) || c->box)) continue;
+        // shapes that would run as another one: SH and a first-arrival
+        // window run split shapes unsplit, and the window and the box run
+        // fused shapes as tile 0
+        if (kTiles[t].split && (!is_psv(c) || c->punch)) continue;
+        if (tile_fused(t) && (!is_psv(c) || c->box || c->punch)) continue;
         c->tile = t;
         double a, b;
         fd23_time_passes(c, 1, &a, &b); // warm-up
