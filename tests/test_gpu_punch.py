@@ -222,3 +222,16 @@ def test_box_with_window_vs_oracle():
     assert np.abs(u).max() > 0
     bad = {k: v for k, v in errs.items() if not v < 1e-5}
     assert not bad, bad
+
+
+def test_autotune_under_a_window_picks_a_shape_that_runs():
+    """Under a window split shapes run unsplit and fused shapes run as tile
+    0, so the tuner must not choose them (it would time one shape and run
+    another)."""
+    _print("case: autotune with a window")
+    p = _psv_problem()
+    with p.context() as ctx:
+        ctx.set_punch(*_window(p), p.dt)
+        tile, _ = ctx.autotune(2)
+        assert ctx.tile == tile
+        assert ctx.tile_kind() == _lib.TILE_TILED and not ctx.tile_split()
