@@ -80,6 +80,9 @@ SIGNATURES = {
     "fd23_run": (c_int, [c_vp, c_int, c_int, c_int]),
     "fd23_sync": (c_int, [c_vp]),
     "fd23_set_punch": (c_int, [c_vp, c_vp, c_flt, c_flt, c_flt]),
+    "fd23_set_punch_f64": (c_int, [c_vp, c_vp, c_dbl, c_dbl, c_dbl]),
+    "fd23_punch_blocks": (c_int, [c_vp, c_int, c_int, c_flt, c_flt, c_flt, c_vp, c_vp]),
+    "fd23_punch_blocks_f64": (c_int, [c_vp, c_int, c_int, c_dbl, c_dbl, c_dbl, c_vp, c_vp]),
     "fd23_get_seismograms": (c_int, [c_vp, c_int, c_int, c_vp, c_int]),
     "fd23_get_field": (c_int, [c_vp, c_int, c_int, c_vp]),
     "fd23_set_field": (c_int, [c_vp, c_int, c_int, c_vp]),

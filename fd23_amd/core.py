@@ -137,16 +137,18 @@ class Context:
     def record(self, it, itrecord=1):
         check(self.lib.fd23_record(self.h, it, itrecord))
 
-    def set_punch(self, times, before=0.0, after=0.0, dt=1.0):
+    def set_punch(self, times, before=0.0, after=0.0, dt=1.0, f64=False):
         """First-arrival window from an nz x nx travel-time grid (None
-        removes it); see fd23_set_punch."""
+        removes it); see fd23_set_punch.  f64=True forms the windows in
+        double from a double grid, as the FP64 tree (fd23_set_punch_f64)."""
         if times is None:
             check(self.lib.fd23_set_punch(self.h, None, 0.0, 0.0, 1.0))
             return
-        t = np.ascontiguousarray(times, np.float32)
+        t = np.ascontiguousarray(times, np.float64 if f64 else np.float32)
         if t.shape != (self.nz, self.nx):
             raise ValueError(f"travel times {t.shape} != (nz, nx) = {(self.nz, self.nx)}")
-        check(self.lib.fd23_set_punch(self.h, _ptr(t), before, after, dt))
+        fn = self.lib.fd23_set_punch_f64 if f64 else self.lib.fd23_set_punch
+        check(fn(self.h, _ptr(t), before, after, dt))
 
     def run(self, it0, nt, itrecord=1):
         check(self.lib.fd23_run(self.h, it0, nt, itrecord))

@@ -18,7 +18,8 @@
 // Keys the one-GPU build reads but does not need: gpuid, ngpu, usetable.
 // snapbox (PSV/clean_nobox_new) writes windowed snapshots to ./OUTPUT/.
 // Additional key: storage=f32|f16|bf16|f64 (default f32).  usepunch=1
-// reads before, after and timefile (an nz*nx float32 travel-time grid) and
+// reads before, after and timefile (an nz*nx travel-time grid, float32, or
+// double for psv2d_db as in the FP64 tree) and
 // sets the first-arrival window (SH/slide/cu_slide.cu:7-61).
 #include "fd23.h"
 
@@ -38,6 +39,8 @@
 #include <sys/stat.h>
 
 namespace {
+
+const int kGetparError = 100;
 
 struct Par {
     fd23_par *p = nullptr;
@@ -74,9 +77,11 @@ struct Par {
         if (!has_s(k, v)) missing(k);
         return v;
     }
+    // getpar reports a missing key through gp_getpar_err, which exits with
+    // GETPAR_ERROR = 100 (PSV/clean_nobox/getpar/getpar.c:82,442,724-734)
     [[noreturn]] static void missing(const char *k) {
         fprintf(stderr, "mstpar: must specify value for '%s'\n", k);
-        exit(-1);
+        exit(kGetparError);
     }
 };
 
@@ -125,7 +130,15 @@ std::set<int> snap_list(const std::string &file) {
 int main(int argc, char **argv) {
     const std::string prog = basename_of(argv[0]);
     Par par;
-    CHECK(fd23_par_open(argc, (const char *const *)argv, &par.p));
+    {
+        // a par file that cannot be opened is a getpar error too (getpar.c:646)
+        const int rc = fd23_par_open(argc, (const char *const *)argv, &par.p);
+        if (rc != FD23_OK) {
+            const char *e = fd23_last_error();
+            fprintf(stderr, "getpar: %s\n", e ? e : "cannot parse the command line");
+            exit(kGetparError);
+        }
+    }
 
     bool psv = true, top = false, explosion = false;
     // the FP64 tree (PSV/psv_clean_double/Makefile: psv2d_db[_topabsorb]):
@@ -256,11 +269,11 @@ int main(int argc, char **argv) {
     std::string snapfile;
     if (!psv) par.has_s("ntsnapfile", snapfile);
     if (psv && !dbl) par.has_i("usepunch", usepunch);
-    float before = 0, after = 0;
+    double before = 0, after = 0; // float in the FP32 trees, double in the FP64 one
     std::string timefile;
-    if (usepunch == 1) { // P-SV param.cpp:187-191, SH param.cpp:80-84
-        before = par.mf("before");
-        after = par.mf("after");
+    if (usepunch == 1) { // P-SV param.cpp:187-191, SH param.cpp:80-84, FP64 param.cpp:149-153
+        before = real("before");
+        after = real("after");
         timefile = par.ms("timefile");
     }
     const std::set<int> extra_snaps = snap_list(snapfile);
@@ -286,11 +299,19 @@ int main(int argc, char **argv) {
 
     // ---- model, material, C-PML -----------------------------------------
     const size_t N = (size_t)nx * nz;
+    // the travel-time grid, nz rows of nx values: float32 in the FP32 trees
+    // (SH/slide/cu_slide.cu:24-39), double in the FP64 tree
+    // (PSV/psv_clean_double/slide/cu_slide.cu:24-38)
     std::vector<float> times;
-    if (usepunch == 1) { // the travel-time grid, nz rows of nx float32 (cu_slide.cu:24-39)
-        times.resize(N);
+    std::vector<double> times_d;
+    if (usepunch == 1) {
+        const size_t es = dbl ? sizeof(double) : sizeof(float);
+        if (dbl)
+            times_d.resize(N);
+        else
+            times.resize(N);
         std::ifstream f(timefile, std::ios::binary);
-        f.read((char *)times.data(), (std::streamsize)(N * sizeof(float)));
+        f.read(dbl ? (char *)times_d.data() : (char *)times.data(), (std::streamsize)(N * es));
         if (!f) {
             fprintf(stderr, "cannot read %zu travel times from %s\n", N, timefile.c_str());
             return -1;
@@ -430,7 +451,8 @@ int main(int argc, char **argv) {
     CHECK(fd23_set_source(ctx, 0, kind, xs, zs, stf.data(), lsrc, M));
     for (size_t c = 0; c < rx.size(); c++)
         CHECK(fd23_set_receivers(ctx, 0, (int)c, rx[c].data(), rz[c].data(), (int)rx[c].size()));
-    if (usepunch == 1) CHECK(fd23_set_punch(ctx, times.data(), before, after, dt));
+    if (usepunch == 1 && dbl) CHECK(fd23_set_punch_f64(ctx, times_d.data(), before, after, dt));
+    if (usepunch == 1 && !dbl) CHECK(fd23_set_punch(ctx, times.data(), (float)before, (float)after, (float)dt));
 
     const size_t sbytes = fd23_state_bytes(ctx);
     if (restart) {

@@ -1206,44 +1206,10 @@ static void enqueue_commit(fd23_ctx *c, bool stress) {
 
 // a step bound truncated toward zero as the reference's int() does, held
 // inside the int range (where that conversion would be undefined)
-static int step_of(float v) {
-    if (!(v == v)) return 0;
-    if (v >= 2147483520.0f) return INT_MAX;
-    if (v <= -2147483648.0f) return INT_MIN;
-    return (int)v;
-}
-
-// First-arrival window, SH/slide/cu_slide.cu:7-61: per block of 32 x 16
-// cells the step window from the block's earliest and latest arrival time,
-// in float arithmetic and truncated to int as the reference does.
-extern "C" int fd23_set_punch(fd23_ctx *c, const float *times, float before, float after, float dt) {
-    if (!c) return fail(FD23_EINVAL, "null context");
-    if (!times) {
-        c->punch = false;
-        fd23_ctx_invalidate_graphs(c);
-        return FD23_OK;
-    }
-    // FP16/BF16 storage rewrites every cell each step in the next step's
-    // scale; a cell left alone would be read back in the wrong one
-    if (scaled_storage(c)) return fail(FD23_EINVAL, "a first-arrival window needs FP32 or FP64 storage");
-    const int nx = c->d.nx, nz = c->d.nz;
-    if (nx % kPunchBX || nz % kPunchBZ)
-        return fail(FD23_EINVAL, "a first-arrival window needs nx a multiple of %d and nz of %d", kPunchBX, kPunchBZ);
-    if (!(dt > 0)) return fail(FD23_EINVAL, "bad dt for the first-arrival window");
-    const int nxb = nx / kPunchBX, nzb = nz / kPunchBZ;
-    std::vector<int> first((size_t)nxb * nzb), last((size_t)nxb * nzb);
-    for (int k = 0; k < nzb; k++)
-        for (int i = 0; i < nxb; i++) {
-            float lo = 1e28f, hi = -1e28f;
-            for (int kk = 0; kk < kPunchBZ; kk++)
-                for (int ii = 0; ii < kPunchBX; ii++) {
-                    const float t = times[(size_t)(k * kPunchBZ + kk) * nx + i * kPunchBX + ii];
-                    lo = std::min(lo, t);
-                    hi = std::max(hi, t);
-                }
-            first[(size_t)k * nxb + i] = step_of((lo - before) / dt);
-            last[(size_t)k * nxb + i] = step_of((hi + after) / dt);
-        }
+// First-arrival window, SH/slide/cu_slide.cu:7-61 (float) and
+// PSV/psv_clean_double/slide/cu_slide.cu:7-52 (double): the per-block step
+// tables come from the host functions fd23_punch_blocks[_f64].
+static int set_punch_tables(fd23_ctx *c, std::vector<int> &first, std::vector<int> &last) {
     int r;
     if ((r = upload_vec(c, c->d_pfirst, c->pfirst_cap, first)) || (r = upload_vec(c, c->d_plast, c->plast_cap, last)))
         return r;
@@ -1254,6 +1220,39 @@ extern "C" int fd23_set_punch(fd23_ctx *c, const float *times, float before, flo
     c->events_dirty = true; // the per-tile window table is built with them
     fd23_ctx_invalidate_graphs(c);
     return FD23_OK;
+}
+
+// shared checks; returns 1 when times is null (the window was removed)
+static int punch_prologue(fd23_ctx *c, const void *times) {
+    if (!c) return fail(FD23_EINVAL, "null context");
+    if (!times) {
+        c->punch = false;
+        fd23_ctx_invalidate_graphs(c);
+        return 1;
+    }
+    // FP16/BF16 storage rewrites every cell each step in the next step's
+    // scale; a cell left alone would be read back in the wrong one
+    if (scaled_storage(c)) return fail(FD23_EINVAL, "a first-arrival window needs FP32 or FP64 storage");
+    return FD23_OK;
+}
+
+extern "C" int fd23_set_punch(fd23_ctx *c, const float *times, float before, float after, float dt) {
+    int r = punch_prologue(c, times);
+    if (r) return r > 0 ? FD23_OK : r;
+    const size_t nb = (size_t)(c->d.nx / kPunchBX) * (c->d.nz / kPunchBZ);
+    std::vector<int> first(nb), last(nb);
+    if ((r = fd23_punch_blocks(times, c->d.nx, c->d.nz, before, after, dt, first.data(), last.data()))) return r;
+    return set_punch_tables(c, first, last);
+}
+
+extern "C" int fd23_set_punch_f64(fd23_ctx *c, const double *times, double before, double after, double dt) {
+    int r = punch_prologue(c, times);
+    if (r) return r > 0 ? FD23_OK : r;
+    const size_t nb = (size_t)(c->d.nx / kPunchBX) * (c->d.nz / kPunchBZ);
+    std::vector<int> first(nb), last(nb);
+    if ((r = fd23_punch_blocks_f64(times, c->d.nx, c->d.nz, before, after, dt, first.data(), last.data())))
+        return r;
+    return set_punch_tables(c, first, last);
 }
 
 // the first-arrival window needs the step index, which only the graph

@@ -17,6 +17,7 @@
 #include "fd23.h"
 #include "fd_device.hpp"
 #include "mdl.hpp"
+#include "keys.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -224,37 +225,44 @@ const double kCoefSum = 1225. / 1024. + 245. / 3072. + 49. / 5120. + 5. / 7168.;
 
 } // namespace
 
-// the material kernels on device inputs buf + kIn (vp vs den, or vs den for
-// SH), reduction keys at buf; frees buf
-static int material_from_device(fd23_ctx *c, char *buf, size_t kIn, double h, double dt, double *stab) {
+// The material kernels on device inputs `in` (vp vs den, or vs den for
+// SH, nz*nx each) with the two 64-bit reduction keys at `keys`.  The keys
+// are updated by 64-bit atomics, which fault the device on an address
+// that is not 8-byte aligned (HSA_STATUS_ERROR_MEMORY_APERTURE_VIOLATION
+// in k_psv_material): a staging layout that placed them after three
+// arrays of an odd cell count did that, so the address is checked here
+// before any launch.  Waits for the stream; the caller frees the buffers.
+static int material_from_device(fd23_ctx *c, const char *in, unsigned long long *keys, double h, double dt,
+                                double *stab) {
+    if (!fd23_keys_aligned(keys))
+        return merr(FD23_EINVAL, "set_model: reduction keys not 8-byte aligned (64-bit atomics need it)");
     const fd23_desc &d = fd23_ctx_desc(c);
     const bool psv = d.wave == FD23_WAVE_PSV;
     const bool f64 = psv && d.storage == FD23_STORE_F64;
     const size_t N = (size_t)d.nx * d.nz;
     hipStream_t st = fd23_ctx_stream(c);
-    unsigned long long *keys = (unsigned long long *)buf;
     // 0 is below every key (the key of -NaN): the maxima start empty
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, keys, 2, 0ull);
     const int block = kMatBlock;
     const int grid = (int)std::min<size_t>((N + block - 1) / block, kMatGrid);
     const int px = fd23_ctx_px(c);
     if (psv && f64) {
-        const double *in = (const double *)(buf + kIn);
-        hipLaunchKernelGGL(k_psv_material<double>, dim3(grid), dim3(block), 0, st, in, in + N, in + 2 * N, d.nx,
+        const double *v = (const double *)in;
+        hipLaunchKernelGGL(k_psv_material<double>, dim3(grid), dim3(block), 0, st, v, v + N, v + 2 * N, d.nx,
                            d.nz, px, dt / h, (double *)fd23_ctx_material(c, 0), (double *)fd23_ctx_material(c, 1),
                            (double *)fd23_ctx_material(c, 2), (double *)fd23_ctx_material(c, 3),
                            (double *)fd23_ctx_material(c, 4), keys);
     } else if (psv) {
-        const float *in = (const float *)(buf + kIn);
+        const float *v = (const float *)in;
         const float ff = (float)dt / (float)h;
-        hipLaunchKernelGGL(k_psv_material<float>, dim3(grid), dim3(block), 0, st, in, in + N, in + 2 * N, d.nx, d.nz,
+        hipLaunchKernelGGL(k_psv_material<float>, dim3(grid), dim3(block), 0, st, v, v + N, v + 2 * N, d.nx, d.nz,
                            px, ff, (float *)fd23_ctx_material(c, 0), (float *)fd23_ctx_material(c, 1),
                            (float *)fd23_ctx_material(c, 2), (float *)fd23_ctx_material(c, 3),
                            (float *)fd23_ctx_material(c, 4), keys);
     } else {
-        const float *in = (const float *)(buf + kIn);
+        const float *v = (const float *)in;
         const float ff = (float)dt / (float)h;
-        hipLaunchKernelGGL(k_sh_material, dim3(grid), dim3(block), 0, st, in, in + N, d.nx, d.nz, px, ff,
+        hipLaunchKernelGGL(k_sh_material, dim3(grid), dim3(block), 0, st, v, v + N, d.nx, d.nz, px, ff,
                            (float *)fd23_ctx_material(c, 0), (float *)fd23_ctx_material(c, 1),
                            (float *)fd23_ctx_material(c, 2), keys);
     }
@@ -263,7 +271,6 @@ static int material_from_device(fd23_ctx *c, char *buf, size_t kIn, double h, do
     if (e == hipSuccess) e = hipMemcpyAsync(hk, keys, sizeof hk, hipMemcpyDeviceToHost, st);
     const hipError_t es_ = hipStreamSynchronize(st);
     if (e == hipSuccess) e = es_;
-    hipFree(buf);
     if (e != hipSuccess) return merr(FD23_EHIP, std::string("set_model: ") + hipGetErrorString(e));
     const double vmax = std::max(order_value(hk[0]), -99999.0), bmax = order_value(hk[1]);
     fd23_ctx_material_ready(c, bmax);
@@ -273,6 +280,14 @@ static int material_from_device(fd23_ctx *c, char *buf, size_t kIn, double h, do
     if (stab) *stab = s;
     if (s > 1) return merr(FD23_ESTAB, "model stability number " + std::to_string(s) + " > 1");
     return FD23_OK;
+}
+
+// material_from_device on a staging allocation (keys at buf, inputs at
+// buf + kIn), which it frees
+static int material_from_staging(fd23_ctx *c, char *buf, size_t kIn, double h, double dt, double *stab) {
+    const int r = material_from_device(c, buf + kIn, (unsigned long long *)buf, h, dt, stab);
+    hipFree(buf);
+    return r;
 }
 
 static int check_model_ctx(fd23_ctx *c) {
@@ -312,7 +327,7 @@ extern "C" int fd23_set_model(fd23_ctx *c, const void *vp, const void *vs, const
             return merr(FD23_EHIP, std::string("set_model copy: ") + hipGetErrorString(e));
         }
     }
-    return material_from_device(c, buf, kIn, h, dt, stab);
+    return material_from_staging(c, buf, kIn, h, dt, stab);
 }
 
 // a .mdl model rasterised on the device, then its material; raster files
@@ -377,5 +392,5 @@ extern "C" int fd23_set_model_file(fd23_ctx *c, const char *model, double h, dou
     else
         hipLaunchKernelGGL(k_mdl_raster<float>, dim3(grid), dim3(block), 0, st, md, d.nx, d.nz, hh, (float *)ovp,
                            (float *)ovs, (float *)oden);
-    return material_from_device(c, buf, kIn, h, dt, stab);
+    return material_from_staging(c, buf, kIn, h, dt, stab);
 }

@@ -126,7 +126,9 @@ struct fd23_server {
     std::map<long long, std::shared_ptr<Request>> all;
     std::vector<std::unique_ptr<Group>> groups; // engine thread only
     size_t used = 0;
-    std::map<std::tuple<int, int, int, int, int>, int> tiles; // tuned tile per shape
+    // tuned tile per shape: wave, storage, nx, nz, shots, npml, topabsorb and
+    // box, since a box or a first-arrival window runs some shapes as others
+    std::map<std::tuple<int, int, int, int, int, int, int, int>, int> tiles;
     fd23_server_stats st{};
 
     void run();
@@ -169,7 +171,7 @@ int fd23_server::start_group(Group &g) {
                                          b.box_rho, b.dx, b.dt)))
         return r;
     if (this->d.autotune) {
-        const auto key = std::make_tuple(b.wave, q0.storage, b.nx, b.nz, g.nshots);
+        const auto key = std::make_tuple(b.wave, q0.storage, b.nx, b.nz, g.nshots, b.npml, b.topabsorb, b.box ? 1 : 0);
         auto it = tiles.find(key);
         if (it == tiles.end()) {
             double ms = 0;

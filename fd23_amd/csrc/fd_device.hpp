@@ -157,7 +157,7 @@ struct Grid {
     int pnxb;           // blocks per row, nx / 32
 };
 
-constexpr int kPunchBX = 32, kPunchBZ = 16; // block of the window, SH/param/const.h:20-21
+constexpr int kPunchBX = FD23_PUNCH_BX, kPunchBZ = FD23_PUNCH_BZ; // block of the window, SH/param/const.h:20-21
 // ix < nx and iz < nz (nx and nz are multiples of the block)
 __device__ __forceinline__ bool punch_open(const Grid &g, int ix, int iz, int it) {
     const int b = (int)((unsigned)iz / kPunchBZ) * g.pnxb + (int)((unsigned)ix / kPunchBX);

@@ -707,6 +707,55 @@ extern "C" int fd23_isis_write_f64(const char *path, const double *raw, int nrec
 }
 
 // ---------------------------------------------------------------------------
+// First-arrival window per block of 32 x 16 cells (SLIDE::SLIDE): the
+// block's earliest and latest travel time, (min - before)/dt and
+// (max + after)/dt truncated toward zero, in float for the FP32 trees
+// (SH/slide/cu_slide.cu:7-61, PSV/clean_nobox/slide/cu_slide.cu) and in
+// double for the FP64 tree (PSV/psv_clean_double/slide/cu_slide.cu:7-52,
+// whose timefile holds doubles).  Quotients beyond the int range are
+// clamped to it (the reference's int() is undefined there), NaN gives 0.
+// ---------------------------------------------------------------------------
+template <class T>
+static int step_trunc(T v) {
+    if (!(v == v)) return 0;
+    if (v >= (T)2147483520.0f) return 2147483647;
+    if (v <= (T)-2147483648.0) return (-2147483647 - 1);
+    return (int)v;
+}
+
+template <class T>
+static int punch_blocks(const T *times, int nx, int nz, T before, T after, T dt, int *first, int *last) {
+    if (!times || !first || !last) return herr(FD23_EINVAL, "punch_blocks: null argument");
+    if (nx <= 0 || nz <= 0 || nx % FD23_PUNCH_BX || nz % FD23_PUNCH_BZ)
+        return herr(FD23_EINVAL, "a first-arrival window needs nx a multiple of 32 and nz of 16");
+    if (!(dt > 0)) return herr(FD23_EINVAL, "bad dt for the first-arrival window");
+    const int nxb = nx / FD23_PUNCH_BX, nzb = nz / FD23_PUNCH_BZ;
+    for (int k = 0; k < nzb; k++)
+        for (int i = 0; i < nxb; i++) {
+            T lo = (T)1e28, hi = (T)-1e28;
+            for (int kk = 0; kk < FD23_PUNCH_BZ; kk++)
+                for (int ii = 0; ii < FD23_PUNCH_BX; ii++) {
+                    const T t = times[(size_t)(k * FD23_PUNCH_BZ + kk) * nx + i * FD23_PUNCH_BX + ii];
+                    lo = std::min(lo, t);
+                    hi = std::max(hi, t);
+                }
+            first[(size_t)k * nxb + i] = step_trunc<T>((lo - before) / dt);
+            last[(size_t)k * nxb + i] = step_trunc<T>((hi + after) / dt);
+        }
+    return FD23_OK;
+}
+
+extern "C" int fd23_punch_blocks(const float *times, int nx, int nz, float before, float after, float dt, int *first,
+                                 int *last) {
+    return punch_blocks<float>(times, nx, nz, before, after, dt, first, last);
+}
+
+extern "C" int fd23_punch_blocks_f64(const double *times, int nx, int nz, double before, double after, double dt,
+                                     int *first, int *last) {
+    return punch_blocks<double>(times, nx, nz, before, after, dt, first, last);
+}
+
+// ---------------------------------------------------------------------------
 // receivers: every station gives the pair (x, z) and (x-1, z)
 // (RECORD::init_line and init_list, PSV/clean_nobox/record/cu_record.cu:12-76)
 // ---------------------------------------------------------------------------

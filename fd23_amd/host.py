@@ -211,3 +211,15 @@ class Par:
         if v is None:
             raise KeyError("must specify value for '%s'" % name)
         return v
+
+
+def punch_blocks(times, before, after, dt, f64=False):
+    """Per-block first/last step tables of the first-arrival window
+    (fd23_punch_blocks, or fd23_punch_blocks_f64 for the FP64 tree)."""
+    t = np.ascontiguousarray(times, np.float64 if f64 else np.float32)
+    nz, nx = t.shape
+    first = np.zeros((nz // 16, nx // 32), np.int32)
+    last = np.zeros_like(first)
+    fn = _lib.load().fd23_punch_blocks_f64 if f64 else _lib.load().fd23_punch_blocks
+    check(fn(_ptr(t), nx, nz, before, after, dt, _ptr(first), _ptr(last)))
+    return first, last

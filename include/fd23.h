@@ -59,6 +59,10 @@ extern "C" {
 #define FD23_F_TX 1
 #define FD23_F_TZ 2
 
+/* block of the first-arrival window (SH/param/const.h:20-21 BDIMX, BDIMY) */
+#define FD23_PUNCH_BX 32
+#define FD23_PUNCH_BZ 16
+
 typedef struct fd23_ctx fd23_ctx;
 
 typedef struct fd23_desc {
@@ -161,6 +165,22 @@ int fd23_sync(fd23_ctx *ctx);
  * window and hold no source tap or receiver return at once.           */
 int fd23_set_punch(fd23_ctx *ctx, const float *times, float before,
                    float after, float dt);
+/* The FP64 tree's window: times is the nz*nx double travel-time grid of
+ * its timefile and the quotients are formed in double
+ * (PSV/psv_clean_double/slide/cu_slide.cu:7-52); otherwise as
+ * fd23_set_punch.                                                       */
+int fd23_set_punch_f64(fd23_ctx *ctx, const double *times, double before,
+                       double after, double dt);
+/* The per-block window tables alone, on the host: first[k*nx/32 + i] and
+ * last[...] for block row k and block column i, as SLIDE::SLIDE builds
+ * them (float: SH/slide/cu_slide.cu:27-53; double: PSV/psv_clean_double/
+ * slide/cu_slide.cu:27-44).  Quotients beyond the int range are clamped
+ * to it.  FD23_EINVAL unless nx % 32 == 0, nz % 16 == 0 and dt > 0.     */
+int fd23_punch_blocks(const float *times, int nx, int nz, float before,
+                      float after, float dt, int *first, int *last);
+int fd23_punch_blocks_f64(const double *times, int nx, int nz,
+                          double before, double after, double dt,
+                          int *first, int *last);
 
 /* ---- outputs ---------------------------------------------------------
  * Seismograms of one shot/component, time-major [nframes][n] floats

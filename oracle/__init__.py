@@ -313,20 +313,22 @@ def mdl_raster(path, nx, nz, h, dtype=np.float32):
 PUNCH_BX, PUNCH_BZ = 32, 16
 
 
-def punch_window(times, before, after, dt):
+def punch_window(times, before, after, dt, dtype=np.float32):
     """Per 32 x 16 block, the first and last step of the window,
-    SH/slide/cu_slide.cu:27-53: the block's min and max of the float32
-    travel-time grid (nz x nx), (min - before) / dt and (max + after) / dt
-    in float32, truncated toward zero."""
-    t = np.ascontiguousarray(times, np.float32)
+    SH/slide/cu_slide.cu:27-53: the block's min and max of the travel-time
+    grid (nz x nx), (min - before) / dt and (max + after) / dt, truncated
+    toward zero.  dtype float32 follows the FP32 trees; float64 the FP64
+    tree, whose timefile, before, after and dt are double
+    (PSV/psv_clean_double/slide/cu_slide.cu:10-44)."""
+    t = np.ascontiguousarray(times, dtype)
     nz, nx = t.shape
     nzb, nxb = nz // PUNCH_BZ, nx // PUNCH_BX
     blk = t[:nzb * PUNCH_BZ, :nxb * PUNCH_BX].reshape(nzb, PUNCH_BZ, nxb, PUNCH_BX)
     lo = blk.min(axis=(1, 3))
     hi = blk.max(axis=(1, 3))
-    f32 = np.float32
-    first = ((lo - f32(before)) / f32(dt)).astype(np.int32)
-    last = ((hi + f32(after)) / f32(dt)).astype(np.int32)
+    f = np.dtype(dtype).type
+    first = ((lo - f(before)) / f(dt)).astype(np.int32)
+    last = ((hi + f(after)) / f(dt)).astype(np.int32)
     return first, last
 
 

@@ -55,3 +55,34 @@ def test_device_bytes_estimate():
     assert 5 * 1024 * 512 * 4 < one < 20 * 1024 * 512 * 4
     # the material is shared by the shots of a context
     assert 30 * one < many < 64 * one and half < 0.6 * many
+
+
+def test_reduction_keys_alignment_check(tmp_path):
+    """The material kernels' 64-bit atomic keys must sit on an 8-byte
+    boundary (a misaligned 64-bit atomic faults the device).  The host check
+    accepts the layouts in use (keys at the start of an allocation) and
+    rejects the one that faulted: keys placed after three FP32 arrays of an
+    odd cell count (333 x 17 cells -> byte offset 67932)."""
+    import shutil
+    import subprocess
+
+    if not shutil.which("g++"):
+        import pytest
+
+        pytest.skip("g++ not available")
+    src = tmp_path / "keys_probe.cpp"
+    src.write_text(
+        '#include "keys.hpp"\n#include <cstdio>\n#include <cstdlib>\n'
+        "int main() {\n"
+        "  alignas(256) static unsigned char buf[1 << 18];\n"
+        "  const long offs[] = {0, 8, 256, 4, 67932, 1, 3 * 333 * 17 * 8};\n"
+        "  for (long o : offs) std::printf(\"%ld %d\\n\", o, (int)fd23_keys_aligned(buf + o));\n"
+        "  std::printf(\"null %d\\n\", (int)fd23_keys_aligned(nullptr));\n"
+        "  return 0;\n}\n")
+    exe = tmp_path / "keys_probe"
+    subprocess.check_call(["g++", "-std=c++17", "-I", os.path.join(ROOT, "fd23_amd", "csrc"), str(src), "-o",
+                           str(exe)])
+    out = subprocess.check_output([str(exe)], text=True).split("\n")
+    got = dict(line.split() for line in out if line)
+    assert got == {"0": "1", "8": "1", "256": "1", "4": "0", "67932": "0", "1": "0", str(3 * 333 * 17 * 8): "1",
+                   "null": "0"}

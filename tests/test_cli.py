@@ -45,8 +45,12 @@ def test_reference_par_file_reaches_the_device(ttdir):
 
 
 def test_missing_key_is_reported(ttdir):
+    # getpar exits with GETPAR_ERROR = 100 on a missing mstpar key and on a
+    # par file it cannot open (PSV/clean_nobox/getpar/getpar.c:82,442,646)
     r = _run("sh2d", ["par=t1"], ttdir)
-    assert r.returncode != 0 and "must specify value for 'nrec'" in r.stderr
+    assert r.returncode == 100 and "must specify value for 'nrec'" in r.stderr
+    r = _run("nbpsv2d", ["par=no_such_file"], ttdir)
+    assert r.returncode == 100 and "no_such_file" in r.stderr
 
 
 def test_punch_keys(ttdir):

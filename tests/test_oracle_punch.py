@@ -187,3 +187,57 @@ def test_bad_window_is_refused():
         sim.set_window(np.zeros((32, 32), np.float32), 0.0, 0.0, 1e-3)
     with pytest.raises(ValueError):
         _sh(64, 30, 6, False, 1).set_window(np.zeros((30, 64), np.float32), 0.0, 0.0, 1e-3)
+
+
+def _window_loop_f64(times, before, after, dt):
+    # the FP64 tree's SLIDE::SLIDE in Python floats (IEEE double)
+    nz, nx = times.shape
+    first = np.zeros((nz // 16, nx // 32), np.int32)
+    last = np.zeros_like(first)
+    for k in range(nz // 16):
+        for i in range(nx // 32):
+            blk = [float(v) for v in times[k * 16:(k + 1) * 16, i * 32:(i + 1) * 32].ravel()]
+            first[k, i] = int((min(blk) - before) / dt)
+            last[k, i] = int((max(blk) + after) / dt)
+    return first, last
+
+
+def test_window_tables_float_and_double_trees():
+    """The library's host window tables (fd23_punch_blocks[_f64]) equal the
+    oracle and a direct loop, in float for the FP32 trees and in double for
+    the FP64 tree, on travel times chosen so that the two precisions
+    truncate differently: 0.3/0.1 is 2.9999999999999996 in double (2) and
+    exactly 3 in float32 (3)."""
+    from fd23_amd import host
+
+    rng = np.random.default_rng(11)
+    t = rng.uniform(0.0, 0.6, (32, 64))
+    # block (0,0): minimum exactly 0.3 -> first differs between precisions
+    t[0:16, 0:32] = np.maximum(t[0:16, 0:32], 0.3)
+    t[3, 5] = 0.3
+    # block (1,1): maximum 0.7 with after 0 -> 0.7/0.1 = 6.999999999999999 (6) in double, 7 in float32
+    t[16:32, 32:64] = np.minimum(t[16:32, 32:64], 0.7)
+    t[20, 40] = 0.7
+    before, after, dt = 0.0, 0.0, 0.1
+    f32f, f32l = host.punch_blocks(t, before, after, dt)
+    f64f, f64l = host.punch_blocks(t, before, after, dt, f64=True)
+    of, ol = O.punch_window(t, before, after, dt)
+    df, dl = O.punch_window(t, before, after, dt, dtype=np.float64)
+    lf, ll = _window_loop_f64(t, before, after, dt)
+    assert np.array_equal(f32f, of) and np.array_equal(f32l, ol)
+    assert np.array_equal(f64f, df) and np.array_equal(f64l, dl)
+    assert np.array_equal(f64f, lf) and np.array_equal(f64l, ll)
+    assert f64f[0, 0] == 2 and f32f[0, 0] == 3
+    assert (f64l != f32l).any() or (f64f != f32f).sum() >= 1
+    # float32 tables of the float32 loop too
+    g, h = _window_loop(t.astype(np.float32), before, after, dt)
+    assert np.array_equal(f32f, g) and np.array_equal(f32l, h)
+
+
+def test_window_tables_reject_bad_grids():
+    from fd23_amd import _lib, host
+
+    with pytest.raises(_lib.Fd23Error):
+        host.punch_blocks(np.zeros((16, 48)), 0.0, 0.0, 0.1)
+    with pytest.raises(_lib.Fd23Error):
+        host.punch_blocks(np.zeros((16, 32)), 0.0, 0.0, 0.0, f64=True)
