@@ -68,6 +68,8 @@ SIGNATURES = {
     "fd23_desc_device_bytes": (c_sz, [ctypes.POINTER(Desc)]),
     "fd23_set_material": (c_int, [c_vp, ctypes.POINTER(c_vp)]),
     "fd23_set_model": (c_int, [c_vp, c_vp, c_vp, c_vp, c_dbl, c_dbl, ctypes.POINTER(c_dbl)]),
+    "fd23_set_model_noalloc": (c_int, [c_vp, c_vp, c_vp, c_vp, c_dbl, c_dbl, ctypes.POINTER(c_dbl)]),
+    "fd23_set_model_noalloc_capacity": (ctypes.c_size_t, []),
     "fd23_get_material": (c_int, [c_vp, c_int, c_vp]),
     "fd23_set_model_file": (c_int, [c_vp, ctypes.c_char_p, c_dbl, c_dbl, ctypes.POINTER(c_dbl)]),
     "fd23_set_cpml": (c_int, [c_vp, c_vp]),

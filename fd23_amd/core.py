@@ -73,15 +73,18 @@ class Context:
         ptrs = (ctypes.c_void_p * len(keep))(*[a.ctypes.data for a in keep])
         check(self.lib.fd23_set_material(self.h, ptrs))
 
-    def set_model(self, vp, vs, den, h, dt):
+    def set_model(self, vp, vs, den, h, dt, noalloc=False):
         """Material computed on the device from vp, vs, den (nz x nx); returns
-        the stability number (raises Fd23Error above 1).  SH ignores vp."""
+        the stability number (raises Fd23Error above 1).  SH ignores vp.
+        noalloc=True goes through fd23_set_model_noalloc (no device
+        allocation; the library's fixed scratch)."""
         arrs = [None if a is None else np.ascontiguousarray(a, self.cdtype) for a in (vp, vs, den)]
         for a in arrs:
             assert a is None or a.shape == (self.nz, self.nx), a.shape
         stab = ctypes.c_double(0.0)
         p = [None if a is None else a.ctypes.data for a in arrs]
-        check(self.lib.fd23_set_model(self.h, p[0], p[1], p[2], h, dt, ctypes.byref(stab)))
+        fn = self.lib.fd23_set_model_noalloc if noalloc else self.lib.fd23_set_model
+        check(fn(self.h, p[0], p[1], p[2], h, dt, ctypes.byref(stab)))
         return stab.value
 
     def set_model_file(self, path, h, dt):

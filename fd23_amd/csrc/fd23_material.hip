@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -328,6 +329,70 @@ extern "C" int fd23_set_model(fd23_ctx *c, const void *vp, const void *vs, const
         }
     }
     return material_from_staging(c, buf, kIn, h, dt, stab);
+}
+
+// ---------------------------------------------------------------------------
+// No-allocation variant of fd23_set_model, for callers that cannot allocate
+// device memory.  Its scratch is one file-scope device array per GPU: the
+// two 64-bit reduction keys in the first 256 bytes, then the staged model
+// arrays (vp vs den, or vs den for SH), at most kNoAllocStage bytes.  The
+// host name `scratch` is a stand-in for the device array, never a device
+// address: each device's address comes from hipGetSymbolAddress with that
+// device current, once, and only that pointer is passed to copies and
+// kernels.  Every call on a GPU shares that GPU's array, so calls that
+// overlap on one GPU race; the keys are cleared by a kernel at each call.
+// ---------------------------------------------------------------------------
+static constexpr size_t kNoAllocKeys = 256;
+static constexpr size_t kNoAllocStage = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocKeys + kNoAllocStage];
+
+static std::mutex g_scratch_mu;
+static std::vector<char *> g_scratch_dev; // per device ordinal
+
+static int scratch_on_device(int dev, char **out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_dev.size() <= dev) g_scratch_dev.resize(dev + 1, nullptr);
+    if (!g_scratch_dev[dev]) {
+        int cur = -1;
+        MCHK(hipGetDevice(&cur));
+        if (cur != dev) MCHK(hipSetDevice(dev));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) MCHK(hipSetDevice(cur));
+        if (e != hipSuccess || !p)
+            return merr(FD23_EHIP, std::string("set_model_noalloc: scratch address: ") + hipGetErrorString(e));
+        g_scratch_dev[dev] = (char *)p;
+    }
+    *out = g_scratch_dev[dev];
+    return FD23_OK;
+}
+
+extern "C" size_t fd23_set_model_noalloc_capacity(void) { return kNoAllocStage; }
+
+extern "C" int fd23_set_model_noalloc(fd23_ctx *c, const void *vp, const void *vs, const void *den, double h,
+                                      double dt, double *stab) {
+    if (!c || !vs || !den) return merr(FD23_EINVAL, "set_model_noalloc: null argument");
+    const fd23_desc &d = fd23_ctx_desc(c);
+    const bool psv = d.wave == FD23_WAVE_PSV;
+    if (psv && !vp) return merr(FD23_EINVAL, "set_model_noalloc: null vp");
+    int r = check_model_ctx(c);
+    if (r) return r;
+    const bool f64 = psv && d.storage == FD23_STORE_F64;
+    const size_t N = (size_t)d.nx * d.nz, es = f64 ? 8 : 4;
+    const int nin = psv ? 3 : 2;
+    if (N * es * nin > kNoAllocStage)
+        return merr(FD23_EINVAL, "set_model_noalloc: the model needs " + std::to_string(N * es * nin) +
+                                     " bytes of staging, above the fixed scratch of " +
+                                     std::to_string(kNoAllocStage) + " bytes (use fd23_set_model)");
+    hipStream_t st = fd23_ctx_stream(c);
+    int dev = -1;
+    MCHK(hipStreamGetDevice(st, &dev));
+    char *buf = nullptr;
+    if ((r = scratch_on_device(dev, &buf))) return r;
+    const void *src[3] = {psv ? vp : vs, psv ? vs : den, den};
+    for (int i = 0; i < nin; i++)
+        MCHK(hipMemcpyAsync(buf + kNoAllocKeys + i * N * es, src[i], N * es, hipMemcpyHostToDevice, st));
+    return material_from_device(c, buf + kNoAllocKeys, (unsigned long long *)buf, h, dt, stab);
 }
 
 // a .mdl model rasterised on the device, then its material; raster files

@@ -105,6 +105,20 @@ int fd23_set_material(fd23_ctx *ctx, const void *const *arrays);
  * 31-135).                                                              */
 int fd23_set_model(fd23_ctx *ctx, const void *vp, const void *vs, const void *den, double h, double dt,
                    double *stab);
+/* As fd23_set_model, but it allocates no device memory: no counterpart in
+ * the reference; for callers that cannot call hipMalloc (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * The staging of the model arrays and the two reduction keys live in a
+ * fixed per-GPU scratch array of the library (256 bytes of keys plus
+ * fd23_set_model_noalloc_capacity() = 32 MiB of staging, held by each GPU
+ * the library runs on until the process ends); a model needing more
+ * staging (nin * nx * nz * element bytes, nin = 3 for P-SV, 2 for SH) is
+ * rejected with FD23_EINVAL.  Calls that overlap on one GPU share that
+ * scratch and race; serialise them.  Same results as fd23_set_model,
+ * bitwise.                                                              */
+int fd23_set_model_noalloc(fd23_ctx *ctx, const void *vp, const void *vs, const void *den, double h,
+                           double dt, double *stab);
+size_t fd23_set_model_noalloc_capacity(void);
 /* The model read from a file and its material computed on the device: a
  * .mdl polygon model (SH/model/model.cpp:175-237) is rasterised by a
  * kernel (each cell takes the last polygon containing it), raster files
