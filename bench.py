@@ -3,17 +3,23 @@
 
 A step is one time step of the loop (stress pass with fused C-PML, source
 injection, velocity pass with fused C-PML, receiver gather) over every shot
-of the workload, replayed from HIP graphs.  Prints ONE JSON line.
+of the workload, replayed from HIP graphs (captured before the timed
+region).  Prints ONE JSON line.
 
-Default workload: plain P-SV FP32, 2000x1000, dx=5 m, dt=0.3 ms, npml 32,
-seeded 12-layer + GRF model, explosive Ricker 15 Hz at (1000,10), 1000
-receiver pairs at z=5 (vx, vz).  Inputs are resident in HBM before timing.
+Default workload (the largest single-GPU configuration): P-SV 4096x2048,
+dx=4 m, dt=0.25 ms, npml 32, seeded 12-layer + GRF model, 8 shots of an
+explosive Ricker 20 Hz at z=8 spread in x, 2048 receiver pairs per shot at
+z=4 (vx, vz), FP32 storage.  Inputs are resident in HBM before timing.  The
+same line carries the FP16 and BF16 storage rates of that workload with
+their seismogram error against FP64 over a stated number of steps, and the
+64-shot 1024x512 batch's and one 1024x512 shot's shots/s.
 
 roofline: algorithmic HBM bytes of the dominant pass (fields read once,
 updated arrays read and written once, material once, psi read+written in
 the C-PML strips) divided by that pass's average duration from HIP events
 on the context stream.  cpu_baseline: the oracle's C restatement of the
-same passes on this host's cores over a bounded number of steps.
+same passes on this host's cores over a bounded number of steps of one
+shot of the workload.
 """
 from __future__ import annotations
 
@@ -74,7 +80,7 @@ def sync():
         pass
 
 
-def run_gpu(wl, steps, warmup, autotune=True, tile=None):
+def run_gpu(wl, steps, warmup, autotune=True, tile=None, passes=True):
     from fd23_amd import _lib
 
     ctx = wl.context(max_frames=warmup + steps)
@@ -84,7 +90,9 @@ def run_gpu(wl, steps, warmup, autotune=True, tile=None):
     elif autotune:
         _, tune_ms = ctx.autotune(4)
     tile = ctx.tile
-    # warm-up: also builds the HIP graphs
+    # the graphs of both runs are captured before any step runs
+    ctx.prepare(warmup)
+    ctx.prepare(steps)
     ctx.run(0, warmup, 1)
     ctx.sync()
     sync()
@@ -98,7 +106,7 @@ def run_gpu(wl, steps, warmup, autotune=True, tile=None):
     kind = ctx.tile_kind()
     fused = kind == _lib.TILE_FUSED and wl.wave == _lib.WAVE_PSV and wl.punch_after is None
     names = kernel_names(ctx, wl)
-    s_ms, v_ms = ctx.time_passes(20)
+    s_ms, v_ms = ctx.time_passes(20) if passes else (None, None)
     ctx.close()
     return dict(elapsed=el, stress_ms=s_ms, vel_ms=v_ms, tile=tile, tune_ms=tune_ms, finite=finite, fused=fused,
                 kernels=names)
@@ -194,9 +202,10 @@ def cpu_baseline(wl, budget_s=12.0, budget_1t=4.0):
     lib.or_set_threads(1)
     k1, el1 = timed(budget_1t, 200)
     lib.or_set_threads(cores)
+    shot = "" if wl.nshots == 1 else "shot 0 of the %d-shot workload, " % wl.nshots
     return dict(value=wl.nx * wl.nz * k / el / 1e9, unit="Gcell/s", cores=cores, kind="port",
-                sample="%d steps of the %dx%d P-SV FP32 loop (oracle C restatement, OpenMP, %d threads)"
-                       % (k, wl.nx, wl.nz, cores),
+                sample="%d steps of %sthe %dx%d P-SV FP32 loop (oracle C restatement, OpenMP, %d threads)"
+                       % (k, shot, wl.nx, wl.nz, cores),
                 value_1thread=wl.nx * wl.nz * k1 / el1 / 1e9, sample_1thread="%d steps, 1 thread" % k1,
                 cpu=_cpu_model())
 
@@ -212,6 +221,14 @@ def _cpu_model():
 
 
 STORAGE = {"f32": 0, "f16": 1, "bf16": 2, "f64": 3}
+STORAGE_TAG = {0: "fp32", 1: "fp16", 2: "bf16", 3: "fp64"}
+
+
+def workload_id(wl):
+    """Workload name with the storage type it runs at, e.g.
+    psv_4096x2048_x8_fp16 (the key of the profiles' traffic table)."""
+    base = wl.name[:-5] if wl.name.endswith("_fp32") else wl.name
+    return base + "_" + STORAGE_TAG[wl.storage]
 
 
 def make_workload(name, storage=None):
@@ -236,23 +253,34 @@ def make_workload(name, storage=None):
     return wl
 
 
-def accuracy_vs_f64(name, storage, nt):
-    """Seismograms of the workload at its storage type against an FP64
+def seismograms(wl, nt):
+    """All shots' and components' seismograms of nt steps, as float64."""
+    ctx = wl.context(max_frames=nt)
+    ctx.run(0, nt, 1)
+    ctx.sync()
+    out = np.stack([ctx.seismograms(s, c, nt).astype(np.float64) for s in range(wl.nshots)
+                    for c in range(ctx.ncomp)])
+    ctx.close()
+    return out
+
+
+def accuracy_vs_f64(name, storages, nt, built=None):
+    """Seismograms of the workload at each storage type against an FP64
     storage run of the same inputs (FP64 on the GPU, pinned to the FP64
     oracle by the parity tests): relative L2 and peak error over all shots
-    and components."""
+    and components.  built: an already built FP32-material workload whose
+    storage is switched (FP32, FP16 and BF16 share the FP32 material)."""
+    ref = seismograms(make_workload(name, "f64").build(), nt)
     out = {}
-    for st in (storage, "f64"):
-        wl = make_workload(name, st).build()
-        ctx = wl.context(max_frames=nt)
-        ctx.run(0, nt, 1)
-        ctx.sync()
-        out[st] = np.stack([ctx.seismograms(s, c, nt).astype(np.float64) for s in range(wl.nshots)
-                            for c in range(ctx.ncomp)])
-        ctx.close()
-    a, b = out[storage], out["f64"]
-    return {"steps": nt, "rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
-            "peak": float(np.abs(a - b).max() / np.abs(b).max()), "finite": bool(np.isfinite(a).all())}
+    for st in storages:
+        wl = built if built is not None else make_workload(name, st).build()
+        keep = wl.storage
+        wl.storage = STORAGE[st]
+        a = seismograms(wl, nt)
+        wl.storage = keep
+        out[st] = {"steps": nt, "rel_l2": float(np.linalg.norm(a - ref) / np.linalg.norm(ref)),
+                   "peak": float(np.abs(a - ref).max() / np.abs(ref).max()), "finite": bool(np.isfinite(a).all())}
+    return out
 
 
 def calibrate_copy(nbytes=1 << 30, reps=10):
@@ -378,9 +406,11 @@ def bench_material(reps=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--workload", default="psv2000",
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--quick", action="store_true",
+                    help="c4: the FP32 line only (no FP16/BF16, accuracy or batching figures)")
+    ap.add_argument("--workload", default="c4",
                     choices=["psv2000", "batched", "single1024", "c4", "f16", "bf16", "sh400", "serve", "punch",
                              "material"])
     ap.add_argument("--requests", type=int, default=40, help="serve: requests in the stream")
@@ -388,7 +418,8 @@ def main():
     ap.add_argument("--storage", choices=sorted(STORAGE), default=None)
     ap.add_argument("--accuracy", action="store_true",
                     help="also report seismogram error against an FP64 run of the same inputs")
-    ap.add_argument("--accuracy-steps", type=int, default=0, help="steps of the accuracy runs (default nt)")
+    ap.add_argument("--accuracy-steps", type=int, default=0,
+                    help="steps of the accuracy runs (default: 1000 for c4's line, else nt)")
     ap.add_argument("--calibrate-copy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
@@ -411,10 +442,42 @@ def main():
     from fd23_amd import _lib
 
     wl = make_workload(args.workload, args.storage).build()
-    r = run_gpu(wl, args.steps, args.warmup, autotune=not args.no_autotune, tile=args.tile)
-    cells = float(wl.nshots) * wl.nx * wl.nz * args.steps
+    out = measure_line(wl, args.steps, args.warmup, autotune=not args.no_autotune, tile=args.tile)
+    headline = args.workload == "c4" and args.storage in (None, "f32") and not args.quick
+    if headline:
+        # FP16 / BF16 storage of the same workload, timed the same way
+        variants = {}
+        for st in ("f16", "bf16"):
+            wl.storage = STORAGE[st]
+            v = measure_line(wl, args.steps, args.warmup)
+            variants[st] = {"value": v["value"], "unit": "Gcell/s", "ms_per_step": v["ms_per_step"],
+                            "speedup_vs_f32": round(v["value"] / out["value"], 3), "tile": v["config"]["tile"],
+                            "roofline_frac": v["roofline"]["frac"], "kernel_ms": v["roofline"]["kernel_ms"],
+                            "kernel_name": v["roofline"]["kernel_name"], "finite": v["finite"]}
+        wl.storage = STORAGE["f32"]
+        acc = accuracy_vs_f64("c4", ("f32", "f16", "bf16"), args.accuracy_steps or 1000, built=wl)
+        out["error_vs_fp64"] = acc["f32"]
+        for st in ("f16", "bf16"):
+            variants[st]["error_vs_fp64"] = acc[st]
+        out["storage_variants"] = variants
+        out["batching"] = bench_batching(max(args.steps, 64), max(args.warmup, 8))
+    elif args.accuracy:
+        sname = out["storage"]
+        out["error_vs_fp64"] = accuracy_vs_f64(args.workload, (sname,), args.accuracy_steps or wl.nt)[sname]
+    if not args.no_cpu and wl.wave == _lib.WAVE_PSV:
+        out["cpu_baseline"] = cpu_baseline(wl)
+    print(json.dumps(out), flush=True)
+
+
+def measure_line(wl, steps, warmup, autotune=True, tile=None):
+    """One workload timed over `steps` graph-replayed steps after `warmup`
+    untimed ones, with the dominant pass's roofline."""
+    from fd23_amd import _lib
+
+    r = run_gpu(wl, steps, warmup, autotune=autotune, tile=tile)
+    cells = float(wl.nshots) * wl.nx * wl.nz * steps
     gcell = cells / r["elapsed"] / 1e9
-    ms = r["elapsed"] / args.steps * 1e3
+    ms = r["elapsed"] / steps * 1e3
     sb, vb = pass_bytes(wl)
     if r["fused"]:
         name, nbytes, kms, kname = "fused", fused_bytes(wl), r["stress_ms"], r["kernels"][0]
@@ -432,8 +495,8 @@ def main():
         "value": round(gcell, 3),
         "unit": "Gcell/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": round(ms, 5),
         "higher_is_better": True,
         "scaling": "weak",
@@ -441,14 +504,14 @@ def main():
         "dtype": "f32" if sname in ("f32", "f16", "bf16") else "f64",
         "storage": sname,
         "data": "synthetic (seeded layered + GRF model, Ricker source)",
-        "config": {"workload": wl.name, "nx": wl.nx, "nz": wl.nz, "nshots": wl.nshots, "npml": wl.npml,
-                   "dx": wl.dx, "dt": wl.dt, "nt": wl.nt, "tile": r["tile"]},
+        "config": {"workload": workload_id(wl), "nx": wl.nx, "nz": wl.nz, "nshots": wl.nshots,
+                   "npml": wl.npml, "dx": wl.dx, "dt": wl.dt, "nt": wl.nt, "tile": r["tile"]},
         "shots_per_s": round(wl.nshots / (ms * 1e-3 * wl.nt), 4),
         "hbm_fraction_step": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "bytes_per_step": step_bytes,
         "roofline": {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(wl.name, kname), "kernel_name": list(kname),
+                     "traffic": measured_traffic(workload_id(wl), kname), "kernel_name": list(kname),
                      "kernel_ms": round(kms, 5), "algorithmic_bytes": nbytes,
                      "stress_ms": None if r["fused"] else round(r["stress_ms"], 5),
                      "velocity_ms": None if r["fused"] else round(r["vel_ms"], 5)},
@@ -458,18 +521,29 @@ def main():
         # the same steps without the window, for the speed-up it gives
         before, after = wl.punch_before, wl.punch_after
         wl.punch_before = wl.punch_after = None
-        r0 = run_gpu(wl, args.steps, args.warmup, autotune=False, tile=r["tile"])
+        r0 = run_gpu(wl, steps, warmup, autotune=False, tile=r["tile"])
         wl.punch_before, wl.punch_after = before, after
         out["first_arrival_window"] = {
             "before_s": before, "after_s": after,
-            "active_fraction": round(wl.punch_active_fraction(args.warmup, args.steps), 4),
-            "ms_per_step_without": round(r0["elapsed"] / args.steps * 1e3, 5),
-            "speedup": round(r0["elapsed"] / r["elapsed"], 3)}
-    if args.accuracy:
-        out["error_vs_fp64"] = accuracy_vs_f64(args.workload, sname, args.accuracy_steps or wl.nt)
-    if not args.no_cpu and wl.wave == _lib.WAVE_PSV and wl.nshots == 1:
-        out["cpu_baseline"] = cpu_baseline(wl)
-    print(json.dumps(out), flush=True)
+            "active_fraction": round(wl.punch_active_fraction(warmup, steps), 4),
+            "ms_per_step_without": round(r0["elapsed"] / steps * 1e3, 5),
+            "speedup": round(r0["elapsed"] / r["elapsed"], 3),
+            "note": "value counts the cells of the whole grid: an effective rate, not throughput"}
+    return out
+
+
+def bench_batching(steps, warmup, single_steps=512):
+    """The 64-shot 1024x512 batch (one context, one launch per pass for all
+    shots) against one 1024x512 shot, in shots/s over the configuration's
+    4000 steps."""
+    b = measure_line(make_workload("batched").build(), steps, warmup)
+    one = measure_line(make_workload("single1024").build(), single_steps, warmup)
+    return {"workload": "psv_1024x512_x64_fp32", "shots_per_s": b["shots_per_s"], "value": b["value"],
+            "ms_per_step": b["ms_per_step"], "steps": steps, "roofline_frac": b["roofline"]["frac"],
+            "kernel_ms": b["roofline"]["kernel_ms"], "kernel_name": b["roofline"]["kernel_name"],
+            "single_shot": {"shots_per_s": one["shots_per_s"], "value": one["value"],
+                            "ms_per_step": one["ms_per_step"], "steps": single_steps},
+            "batch_over_single": round(b["shots_per_s"] / one["shots_per_s"], 3)}
 
 
 if __name__ == "__main__":

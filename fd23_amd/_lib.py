@@ -80,6 +80,7 @@ SIGNATURES = {
     "fd23_step_velocity": (c_int, [c_vp]),
     "fd23_record": (c_int, [c_vp, c_int, c_int]),
     "fd23_run": (c_int, [c_vp, c_int, c_int, c_int]),
+    "fd23_prepare": (c_int, [c_vp, c_int, c_int]),
     "fd23_sync": (c_int, [c_vp]),
     "fd23_set_punch": (c_int, [c_vp, c_vp, c_flt, c_flt, c_flt]),
     "fd23_set_punch_f64": (c_int, [c_vp, c_vp, c_dbl, c_dbl, c_dbl]),

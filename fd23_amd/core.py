@@ -153,6 +153,10 @@ class Context:
         fn = self.lib.fd23_set_punch_f64 if f64 else self.lib.fd23_set_punch
         check(fn(self.h, _ptr(t), before, after, dt))
 
+    def prepare(self, nt, itrecord=1):
+        """Capture the graphs run(*, nt, itrecord) replays, without running."""
+        check(self.lib.fd23_prepare(self.h, nt, itrecord))
+
     def run(self, it0, nt, itrecord=1):
         check(self.lib.fd23_run(self.h, it0, nt, itrecord))
 

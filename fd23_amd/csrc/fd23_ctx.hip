@@ -1349,13 +1349,25 @@ static int get_graph(fd23_ctx *c, int itrecord, int steps, hipGraphExec_t *out) 
     return FD23_OK;
 }
 
-extern "C" int fd23_run(fd23_ctx *c, int it0, int nt, int itrecord) {
+// what fd23_run needs before it captures or replays
+static int run_prologue(fd23_ctx *c, int it0, int nt, int itrecord) {
     int r = ready(c);
     if (r) return r;
     if (nt < 0 || it0 < 0 || itrecord < 1) return fail(FD23_EINVAL, "bad run range");
     if ((r = build_events(c))) return r;
     if (use_fused(c) && ((r = ensure_alt(c)) || (r = build_fused_events(c)))) return r;
-    if ((r = init_scales(c))) return r;
+    return init_scales(c);
+}
+
+// A fused-step chunk of odd length ends on the other buffer set, which a
+// cached graph cannot track: it is enqueued directly.
+static bool chunk_as_graph(const fd23_ctx *c, int steps) { return !(use_fused(c) && (steps & 1)); }
+
+// nt steps run as full kGraphSteps-step graphs plus one graph of the
+// remaining steps, so every step of a run is replayed from a graph
+extern "C" int fd23_run(fd23_ctx *c, int it0, int nt, int itrecord) {
+    int r = run_prologue(c, it0, nt, itrecord);
+    if (r) return r;
     hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(64), 0, c->stream, c->clock, it0);
     const int full = nt / kGraphSteps, rest = nt % kGraphSteps;
     if (full > 0) {
@@ -1363,13 +1375,29 @@ extern "C" int fd23_run(fd23_ctx *c, int it0, int nt, int itrecord) {
         if ((r = get_graph(c, itrecord, kGraphSteps, &g))) return r;
         for (int i = 0; i < full; i++) HIPCHK(hipGraphLaunch(g, c->stream));
     }
-    if (rest > 0) {
+    if (rest > 0 && chunk_as_graph(c, rest)) {
+        hipGraphExec_t g;
+        if ((r = get_graph(c, itrecord, rest, &g))) return r;
+        HIPCHK(hipGraphLaunch(g, c->stream));
+    } else if (rest > 0) {
         for (int i = 0; i < rest; i++)
             if ((r = enqueue_step(c, i, itrecord))) return r;
         hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c->stream, c->clock, rest);
     }
     if (c->box) fd23_box_set_step(c, it0 + nt);
     return check_launch();
+}
+
+// the graphs fd23_run(c, it0, nt, itrecord) replays, captured and
+// instantiated now without running, so that run starts with no capture
+extern "C" int fd23_prepare(fd23_ctx *c, int nt, int itrecord) {
+    int r = run_prologue(c, 0, nt, itrecord);
+    if (r) return r;
+    hipGraphExec_t g;
+    if (nt >= kGraphSteps && (r = get_graph(c, itrecord, kGraphSteps, &g))) return r;
+    const int rest = nt % kGraphSteps;
+    if (rest > 0 && chunk_as_graph(c, rest) && (r = get_graph(c, itrecord, rest, &g))) return r;
+    return FD23_OK;
 }
 
 extern "C" int fd23_sync(fd23_ctx *c) {
