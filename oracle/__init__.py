@@ -160,6 +160,16 @@ def ricker(n, dt, f0, t0=None):
     return out
 
 
+def round_storage(a, fmt):
+    """Round the float64 array a in place as stored in fmt ("f32", "f16",
+    "bf16") with one power-of-two scale putting its largest magnitude in
+    [2^8, 2^9) (or_round_storage; the storage floor's emulation)."""
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    f = lib().or_round_storage
+    f.restype = None
+    f(_p(a), ctypes.c_long(a.size), ctypes.c_int({"f32": 0, "f16": 1, "bf16": 2}[fmt]))
+
+
 def psv_moment(kind, strike, dip, rake, azimuth, h, dt, dtype=np.float32):
     M = np.zeros(3, dtype)
     f = getattr(lib(), "or_psv_moment" + _sfx(dtype))

@@ -36,6 +36,60 @@ void or_ricker(int n, double dt, double f0, double t0, double *out)
     }
 }
 
+/* Reduced-precision storage emulation (the FP16/BF16 error floor): round
+ * the n doubles of a in place as they would be stored with one
+ * power-of-two scale s = 2^(floor(log2 max|a|) - 8), which puts the largest
+ * magnitude in [2^8, 2^9): fmt 1 = FP16 (11 significant bits, round to
+ * nearest even, subnormals below 2^-14 spaced 2^-24), fmt 2 = BF16 (the
+ * value first rounded to float, then to 8 significant bits, nearest even,
+ * as a float -> bfloat16 conversion), fmt 0 = float.  Not in the
+ * reference; the same rounding as tests/helpers.py round_storage, checked
+ * against it. */
+static double or_round_f16(double y)
+{
+    if (y == 0.0 || !isfinite(y)) return y;
+    int e;
+    frexp(y, &e);            /* |y| in [2^(e-1), 2^e) */
+    int ee = e - 1;
+    if (ee < -14) ee = -14;  /* subnormal range: fixed spacing */
+    const double q = ldexp(1.0, ee - 10);
+    return nearbyint(y / q) * q;
+}
+
+static double or_round_bf16(double y)
+{
+    float f = (float)y;
+    unsigned int b;
+    memcpy(&b, &f, 4);
+    b = (unsigned int)(((unsigned long long)b + 0x7FFFu + ((b >> 16) & 1u)) & 0xFFFF0000u);
+    memcpy(&f, &b, 4);
+    return (double)f;
+}
+
+void or_round_storage(double *a, long n, int fmt)
+{
+    double m = 0.0;
+#pragma omp parallel for reduction(max : m)
+    for (long i = 0; i < n; i++) {
+        const double v = fabs(a[i]);
+        if (v > m) m = v;
+    }
+    if (m == 0.0) return;
+    if (fmt == 0) {
+#pragma omp parallel for
+        for (long i = 0; i < n; i++) a[i] = (double)(float)a[i];
+        return;
+    }
+    int e;
+    frexp(m, &e);
+    const double s = ldexp(1.0, e - 1 - 8), inv = ldexp(1.0, -(e - 1 - 8));
+#pragma omp parallel for
+    for (long i = 0; i < n; i++) {
+        const double y = a[i] * inv;
+        a[i] = (fmt == 1 ? or_round_f16(y) : or_round_bf16(y)) * s;
+    }
+}
+
 /* number of OpenMP threads the passes use (for the CPU baseline report) */
 #ifdef _OPENMP
 #include <omp.h>

@@ -154,7 +154,9 @@ def round_storage(a, fmt):
 
 
 def emulated_storage(p: Problem, fmt):
-    """Seismograms of p's shot 0 with FP64 arithmetic and fmt storage."""
+    """Seismograms of p's shot 0 with FP64 arithmetic and fmt storage (every
+    array a pass updates rounded after the pass; the oracle's C rounding,
+    bitwise equal to round_storage)."""
     cp = unpack_cpml(p.packed, p.wave, p.nx, p.nz, np.float64)
     xs, zs = p.sources[0]
     stf, M = p.stf.astype(np.float64), p.M.astype(np.float64)
@@ -174,10 +176,10 @@ def emulated_storage(p: Problem, fmt):
         sim.stress()
         sim.inject(kind, it, stf, M, xs, zs)
         for a in after_stress():
-            round_storage(a, fmt)
+            O.round_storage(a, fmt)
         sim.velocity()
         for a in after_vel():
-            round_storage(a, fmt)
+            O.round_storage(a, fmt)
         for c, name in enumerate(comps):
             seis[c].append(sim.record(name, p.rx, p.rz))
     return [np.array(s) for s in seis]

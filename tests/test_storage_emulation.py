@@ -40,3 +40,20 @@ def test_emulated_storage_floors():
     # one rounding per array and step: FP16 well above FP32, BF16 about 8x FP16
     assert 1e-4 < err["f16"] < 2e-2
     assert 4 * err["f16"] < err["bf16"] < 0.2
+
+
+def test_c_rounding_equals_python_rounding():
+    """The oracle's C storage rounding (used for large grids) equals the
+    numpy rounding above bitwise, FP16 subnormal range included."""
+    import oracle as O
+
+    rng = np.random.default_rng(5)
+    for fmt in ("f32", "f16", "bf16"):
+        for scale in (1e-9, 3.7, 2.0 ** 20):
+            a = rng.standard_normal(20000) * scale * 10.0 ** rng.integers(-12, 1, 20000)
+            a[:5] = [0.0, scale, -scale, scale * 2.0 ** -30, scale * (1 + 2.0 ** -11)]
+            b = a.copy()
+            c = a.copy()
+            round_storage(b, fmt)
+            O.round_storage(c, fmt)
+            assert np.array_equal(b, c), (fmt, scale, np.flatnonzero(b != c)[:5])
