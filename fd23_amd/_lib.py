@@ -94,6 +94,9 @@ SIGNATURES = {
     "fd23_set_state": (c_int, [c_vp, c_int, c_vp, c_sz]),
     "fd23_box_configure": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_int, c_int, c_dbl, c_dbl, c_dbl,
                                    c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl]),
+    "fd23_box_configure_frames": (c_int, [c_vp, c_int, c_int, c_int, c_vp, c_int, c_dbl, c_dbl, c_dbl, c_dbl,
+                                          c_dbl]),
+    "fd23_box_forced_cells": (c_int, [c_vp]),
     "fd23_box_get_residual": (c_int, [c_vp, c_int, c_vp]),
     "fd23_set_tile": (c_int, [c_vp, c_int]),
     "fd23_get_tile": (c_int, [c_vp]),

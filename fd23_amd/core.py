@@ -202,6 +202,16 @@ class Context:
                                           vp, vs, rho, h, dt))
         self._box_shape = (bz, bx)
 
+    def box_configure_frames(self, xs, zs, bw, frames, srcpvel, srcsvel, srcden, h, dt):
+        """The reference's file-driven box (fd23_box_configure_frames):
+        frames is (nframes, 2, bw+8, bw+8) float64 displacement (U, W)."""
+        f = np.ascontiguousarray(frames, np.float64)
+        assert f.ndim == 4 and f.shape[1:] == (2, bw + 8, bw + 8), f.shape
+        check(self.lib.fd23_box_configure_frames(self.h, xs, zs, bw, _ptr(f), f.shape[0], srcpvel, srcsvel,
+                                                 srcden, h, dt))
+        self._box_shape = (bw, bw)
+        return self.lib.fd23_box_forced_cells(self.h)
+
     def box_residual(self, shot=0):
         """Residual grid of the box: dict of U W Txx Tzz Txz (bz x bx)."""
         bz, bx = self._box_shape

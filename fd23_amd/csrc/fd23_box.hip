@@ -24,11 +24,25 @@
 // each field's staggered position (U at x+1/2, W at z-1/2, Txz at both).
 // The reference's half-size copy of the drive (cu_box.cu:39-43, which
 // copies sizeof(int)*bw*bw bytes of double arrays) is not reproduced.
+//
+// File drive (fd23_box_configure_frames, the reference's own box): a
+// bw x bw square around (xs, zs) whose drive comes from per-step
+// displacement frames of (bw+8)^2 cells (U then W), as BOX::setboxsource
+// (box.cpp:120-197) converts them: velocities (u[it+1] - u[it])/dt and
+// stresses C grad u[it+1] / dt with the box material, the derivative order
+// reduced toward the box centre.  All frames are uploaded once and a
+// one-workgroup kernel converts frames it and it+1 at the start of each
+// step, so the graph loop needs no host work per step.  The box material
+// follows MATERIAL::init_for_box_from_full (material.cpp:39-99): a cell of
+// the global material is forced to the homogeneous source medium only
+// when one of its five values differs from it by more than 1e-4
+// relative; the residual grid reads the global material at its cells.
 #include "fd23.h"
 #include "fd_device.hpp"
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -98,7 +112,7 @@ __device__ __forceinline__ int ring_depth(int i, int j, int bx, int bz) {
 // add (a = r + s) or sub (r = a - s) over one ring for fields [f0, f1)
 template <class C>
 __global__ void k_box_ring(BoxPar p, C *const *glob, C *resid, int f0, int f1, bool add, const int *clock, int k,
-                           int it_arg, double toff) {
+                           int it_arg, double toff, const double *stab) {
     const int it = clock ? clock[0] + k : it_arg;
     const double t = (it + toff) * p.dt;
     const long nb = (long)p.bx * p.bz;
@@ -111,9 +125,14 @@ __global__ void k_box_ring(BoxPar p, C *const *glob, C *resid, int f0, int f1, b
         const int i = (int)(cell % p.bx), j = (int)(cell / p.bx);
         const int d = ring_depth(i, j, p.bx, p.bz);
         if (d < dlo || d > dhi) continue;
-        double ox, oz;
-        offsets(fi, ox, oz);
-        const C s = C(drive(p, fi, p.x0 + i + ox, p.z0 + j + oz, t));
+        C s;
+        if (stab) { // file drive: this step's converted frames
+            s = C(stab[(long)fi * nb + cell]);
+        } else {
+            double ox, oz;
+            offsets(fi, ox, oz);
+            s = C(drive(p, fi, p.x0 + i + ox, p.z0 + j + oz, t));
+        }
         C *a = glob[fi] + (long)shot * p.sstride + (long)(p.z0 + j) * p.px + (p.x0 + i);
         C *r = resid + ((long)shot * 5 + fi) * nb + cell;
         if (add)
@@ -130,7 +149,7 @@ template <class C> __device__ __forceinline__ C rat(const C *f, int bx, int bz, 
 // residual grid passes (PSV/psv_clean_double/fd/cu_fd.cu:93-207): full
 // order, zero outside the box, homogeneous (lam, mu, b) already scaled
 template <class C>
-__global__ void k_box_stress_r(BoxPar p, C *resid, C lam, C mu, C mua) {
+__global__ void k_box_stress_r(BoxPar p, C *resid, const C *LAMg, const C *MUg, const C *MUAg) {
     const long nb = (long)p.bx * p.bz;
     const C c0 = C(1225. / 1024.), c1 = C(245. / 3072.), c2 = C(49. / 5120.), c3 = C(5. / 7168.);
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nb * p.nshots; e += (long)gridDim.x * blockDim.x) {
@@ -166,6 +185,8 @@ __global__ void k_box_stress_r(BoxPar p, C *resid, C lam, C mu, C mua) {
             return mad(-c3, rat(f, bx, bz, i, j + 4) - rat(f, bx, bz, i, j - 3), r);
         };
         const C ux = dmx(U), uz = dmz(U), wx = dpx(W), wz = dpz(W);
+        const long qg = (long)(p.z0 + j) * p.px + p.x0 + i;
+        const C lam = LAMg[qg], mu = MUg[qg], mua = MUAg[qg];
         const C gam = lam + C(2.0) * mu;
         const long q = (long)j * bx + i;
         Txx[q] = Txx[q] + (gam * ux + lam * wz);
@@ -174,7 +195,7 @@ __global__ void k_box_stress_r(BoxPar p, C *resid, C lam, C mu, C mua) {
     }
 }
 
-template <class C> __global__ void k_box_velocity_r(BoxPar p, C *resid, C bu, C bw) {
+template <class C> __global__ void k_box_velocity_r(BoxPar p, C *resid, const C *BUg, const C *BWg) {
     const long nb = (long)p.bx * p.bz;
     const C c0 = C(1225. / 1024.), c1 = C(245. / 3072.), c2 = C(49. / 5120.), c3 = C(5. / 7168.);
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nb * p.nshots; e += (long)gridDim.x * blockDim.x) {
@@ -210,6 +231,8 @@ template <class C> __global__ void k_box_velocity_r(BoxPar p, C *resid, C bu, C 
             return mad(-c3, rat(f, bx, bz, i, j + 4) - rat(f, bx, bz, i, j - 3), r);
         };
         const C dxTxx = dpx(Txx), dzTzz = dmz(Tzz), dxTxz = dmx(Txz), dzTxz = dpz(Txz);
+        const long qg = (long)(p.z0 + j) * p.px + p.x0 + i;
+        const C bu = BUg[qg], bw = BWg[qg];
         const long q = (long)j * bx + i;
         U[q] = U[q] + bu * (dxTxx + dzTxz);
         W[q] = W[q] + bw * (dxTxz + dzTzz);
@@ -232,6 +255,85 @@ __global__ void k_box_material(BoxPar p, C *BU, C *BW, C *MU, C *MUA, C *LAM, C 
     }
 }
 
+// MATERIAL::init_for_box_from_full (material.cpp:58-97): a box cell keeps
+// the global material unless one of its five values differs from the
+// homogeneous source medium by more than eps relative; then all five are
+// set to it.  Comparisons in double, as the FP64 tree.
+template <class C>
+__global__ void k_box_material_force(BoxPar p, C *BU, C *BW, C *MU, C *MUA, C *LAM, double b, double mu,
+                                     double lam, int *nforced) {
+    const long nb = (long)p.bx * p.bz;
+    const double eps = 1e-4;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nb; e += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(e % p.bx), j = (int)(e / p.bx);
+        const long q = (long)(p.z0 + j) * p.px + p.x0 + i;
+        if (fabs(b - (double)BU[q]) > b * eps || fabs(b - (double)BW[q]) > b * eps ||
+            fabs(mu - (double)MU[q]) > mu * eps || fabs(mu - (double)MUA[q]) > mu * eps ||
+            fabs(lam - (double)LAM[q]) > lam * eps) {
+            BU[q] = C(b);
+            BW[q] = C(b);
+            MU[q] = C(mu);
+            MUA[q] = C(mu);
+            LAM[q] = C(lam);
+            atomicAdd(nforced, 1);
+        }
+    }
+}
+
+// BOX::setboxsource (box.cpp:142-196) for step it on the device: frames
+// it and it+1 (each U then W, (bw+8)^2, x fastest) to the drive of the
+// five fields, s[field][j*bw + i] in the internal order U W Txx Tzz Txz,
+// with the box material at the box cells.  Same double arithmetic and
+// order of operations as the host code, no contraction.
+template <class C>
+__global__ void k_box_frames_drive(BoxPar p, const double *frames, int nframes, const int *clock, int k, int it_arg,
+                                   const C *LAM, const C *MU, const C *MUA, double *s) {
+    const int it = clock ? clock[0] + k : it_arg;
+    const int bw = p.bx, fw = bw + 8;
+    const long fsz = (long)fw * fw;
+    // the host checks every run's frame range; the clamp only keeps a
+    // step beyond it inside the table
+    const int f1 = min(max(it + 1, 1), nframes - 1), f0 = f1 - 1;
+    const double *uo = frames + (long)f0 * 2 * fsz, *wo = uo + fsz;
+    const double *un = frames + (long)f1 * 2 * fsz, *wn = un + fsz;
+    const long nb = (long)bw * bw;
+    const int mid = (bw + 1) / 2;
+    for (long e = threadIdx.x; e < nb; e += blockDim.x) {
+        const int tx = (int)(e % bw) + 4, ty = (int)(e / bw) + 4;
+        int ord = min(4, max(abs(ty - 3 - mid) - 1, abs(tx - 3 - mid) - 1));
+        ord = max(ord, 0);
+        double c0, c1, c2, c3;
+        Coef<double>::get(ord, c0, c1, c2, c3);
+        auto U = [&](int x, int z) { return un[(long)z * fw + x]; };
+        auto W = [&](int x, int z) { return wn[(long)z * fw + x]; };
+        double ux = c0 * (U(tx, ty) - U(tx - 1, ty));
+        ux = ux - c1 * (U(tx + 1, ty) - U(tx - 2, ty));
+        ux = ux + c2 * (U(tx + 2, ty) - U(tx - 3, ty));
+        ux = ux - c3 * (U(tx + 3, ty) - U(tx - 4, ty));
+        double uz = c0 * (U(tx, ty) - U(tx, ty - 1));
+        uz = uz - c1 * (U(tx, ty + 1) - U(tx, ty - 2));
+        uz = uz + c2 * (U(tx, ty + 2) - U(tx, ty - 3));
+        uz = uz - c3 * (U(tx, ty + 3) - U(tx, ty - 4));
+        double wx = c0 * (W(tx + 1, ty) - W(tx, ty));
+        wx = wx - c1 * (W(tx + 2, ty) - W(tx - 1, ty));
+        wx = wx + c2 * (W(tx + 3, ty) - W(tx - 2, ty));
+        wx = wx - c3 * (W(tx + 4, ty) - W(tx - 3, ty));
+        double wz = c0 * (W(tx, ty + 1) - W(tx, ty));
+        wz = wz - c1 * (W(tx, ty + 2) - W(tx, ty - 1));
+        wz = wz + c2 * (W(tx, ty + 3) - W(tx, ty - 2));
+        wz = wz - c3 * (W(tx, ty + 4) - W(tx, ty - 3));
+        const long qf = (long)ty * fw + tx;
+        const long qg = (long)(p.z0 + ty - 4) * p.px + p.x0 + tx - 4;
+        const double lam = (double)LAM[qg], mu = (double)MU[qg], mua = (double)MUA[qg];
+        const double gam = lam + 2.0 * mu;
+        s[e] = (un[qf] - uo[qf]) / p.dt;
+        s[nb + e] = (wn[qf] - wo[qf]) / p.dt;
+        s[2 * nb + e] = (gam * ux + lam * wz) / p.dt;
+        s[3 * nb + e] = (lam * ux + gam * wz) / p.dt;
+        s[4 * nb + e] = (mua * (wx + uz)) / p.dt;
+    }
+}
+
 struct Box {
     BoxPar p;
     void *resid = nullptr; // [shot][5][bz][bx], compute type
@@ -239,6 +341,11 @@ struct Box {
     double lam, mu, b;     // scaled box medium
     bool f64 = false;
     int staged_it = 0;     // step index of the staged entry points
+    // file drive: all frames on the device and this step's drive
+    double *frames = nullptr;
+    int nframes = 0;
+    double *sdrive = nullptr; // [5][bw*bw]
+    int nforced = 0;          // box cells whose material was forced
 };
 
 Box *get(fd23_ctx *c) { return (Box *)fd23_ctx_box(c); }
@@ -248,8 +355,16 @@ template <class C> int launch_ring(fd23_ctx *c, Box *bx, int f0, int f1, bool ad
     const int nblk = (int)std::min<long>((total + 255) / 256, 2048);
     hipLaunchKernelGGL((k_box_ring<C>), dim3(nblk), dim3(256), 0, fd23_ctx_stream(c), bx->p, (C *const *)bx->gtab,
                        (C *)bx->resid, f0, f1, add, k >= 0 ? fd23_ctx_clock(c) : nullptr, k < 0 ? 0 : k,
-                       bx->staged_it, toff);
+                       bx->staged_it, toff, bx->sdrive);
     return FD23_OK;
+}
+
+// the file drive of this step, ahead of its first ring
+template <class C> void launch_frames_drive(fd23_ctx *c, Box *bx, int k) {
+    hipLaunchKernelGGL((k_box_frames_drive<C>), dim3(1), dim3(1024), 0, fd23_ctx_stream(c), bx->p, bx->frames,
+                       bx->nframes, k >= 0 ? fd23_ctx_clock(c) : nullptr, k < 0 ? 0 : k, bx->staged_it,
+                       (const C *)fd23_ctx_material(c, 4), (const C *)fd23_ctx_material(c, 2),
+                       (const C *)fd23_ctx_material(c, 3), bx->sdrive);
 }
 
 } // namespace
@@ -264,7 +379,31 @@ void fd23_box_set_step(fd23_ctx *c, int it) {
 int fd23_box_launch_pre_stress(fd23_ctx *c, int k) {
     Box *b = get(c);
     if (!b) return FD23_OK;
+    if (b->frames) {
+        if (k < 0 && b->staged_it + 1 >= b->nframes) {
+            fd23_set_error_("the box source frames end before this step");
+            return FD23_EINVAL;
+        }
+        if (b->f64)
+            launch_frames_drive<double>(c, b, k);
+        else
+            launch_frames_drive<float>(c, b, k);
+    }
     return b->f64 ? launch_ring<double>(c, b, 0, 2, true, k, 0.5) : launch_ring<float>(c, b, 0, 2, true, k, 0.5);
+}
+
+// frames a run of steps [it0, it0+nt) needs: it0+nt+1
+int fd23_box_check_run(fd23_ctx *c, int it0, int nt) {
+    Box *b = get(c);
+    if (!b || !b->frames || nt == 0) return FD23_OK;
+    if ((long)it0 + nt + 1 > b->nframes) {
+        char msg[160];
+        snprintf(msg, sizeof msg, "the box source has %d frames; steps %d..%d need %ld", b->nframes, it0,
+                 it0 + nt - 1, (long)it0 + nt + 1);
+        fd23_set_error_(msg);
+        return FD23_EINVAL;
+    }
+    return FD23_OK;
 }
 
 int fd23_box_launch_post_stress(fd23_ctx *c, int k) {
@@ -275,12 +414,14 @@ int fd23_box_launch_post_stress(fd23_ctx *c, int k) {
     if (b->f64) {
         launch_ring<double>(c, b, 0, 2, false, k, 0.5);
         hipLaunchKernelGGL((k_box_stress_r<double>), dim3(nblk), dim3(256), 0, fd23_ctx_stream(c), b->p,
-                           (double *)b->resid, b->lam, b->mu, b->mu);
+                           (double *)b->resid, (const double *)fd23_ctx_material(c, 4),
+                           (const double *)fd23_ctx_material(c, 2), (const double *)fd23_ctx_material(c, 3));
         launch_ring<double>(c, b, 2, 5, true, k, 1.0);
     } else {
         launch_ring<float>(c, b, 0, 2, false, k, 0.5);
         hipLaunchKernelGGL((k_box_stress_r<float>), dim3(nblk), dim3(256), 0, fd23_ctx_stream(c), b->p,
-                           (float *)b->resid, (float)b->lam, (float)b->mu, (float)b->mu);
+                           (float *)b->resid, (const float *)fd23_ctx_material(c, 4),
+                           (const float *)fd23_ctx_material(c, 2), (const float *)fd23_ctx_material(c, 3));
         launch_ring<float>(c, b, 2, 5, true, k, 1.0);
     }
     return FD23_OK;
@@ -294,11 +435,13 @@ int fd23_box_launch_post_velocity(fd23_ctx *c, int k) {
     if (b->f64) {
         launch_ring<double>(c, b, 2, 5, false, k, 1.0);
         hipLaunchKernelGGL((k_box_velocity_r<double>), dim3(nblk), dim3(256), 0, fd23_ctx_stream(c), b->p,
-                           (double *)b->resid, b->b, b->b);
+                           (double *)b->resid, (const double *)fd23_ctx_material(c, 0),
+                           (const double *)fd23_ctx_material(c, 1));
     } else {
         launch_ring<float>(c, b, 2, 5, false, k, 1.0);
         hipLaunchKernelGGL((k_box_velocity_r<float>), dim3(nblk), dim3(256), 0, fd23_ctx_stream(c), b->p,
-                           (float *)b->resid, (float)b->b, (float)b->b);
+                           (float *)b->resid, (const float *)fd23_ctx_material(c, 0),
+                           (const float *)fd23_ctx_material(c, 1));
     }
     if (k < 0) b->staged_it++;
     return FD23_OK;
@@ -417,6 +560,122 @@ extern "C" int fd23_box_configure(fd23_ctx *c, int shot, int x0, int z0, int bx,
     return FD23_OK;
 }
 
+// The reference's box (BOX::init_box, box.cpp:10-119): a bw x bw square
+// whose cell (i, j) is global (xs + idxstart + i, zs + idxstart + j),
+// idxstart = (bw+1)/2 + 1 - bw, driven by nframes displacement frames
+// (frame f: U then W, (bw+8)^2 doubles, x fastest: the sourcefile layout
+// read at box.cpp:103-105 and 124-137); step it uses frames it and it+1.
+// The source medium (srcpvel, srcsvel, srcden) is forced into the global
+// material where it differs (material.cpp:39-99).
+extern "C" int fd23_box_configure_frames(fd23_ctx *c, int xs, int zs, int bw, const double *frames, int nframes,
+                                         double srcpvel, double srcsvel, double srcden, double h, double dt) {
+    char msg[256];
+    if (!c || !frames) {
+        fd23_set_error_("null argument");
+        return FD23_EINVAL;
+    }
+    const fd23_desc &d = fd23_ctx_desc(c);
+    if (d.wave != FD23_WAVE_PSV || (d.storage != FD23_STORE_F32 && d.storage != FD23_STORE_F64)) {
+        fd23_set_error_("box coupling needs a P-SV context with FP32 or FP64 storage");
+        return FD23_EINVAL;
+    }
+    const int idxstart = (bw + 1) / 2 + 1 - bw;
+    const int x0 = xs + idxstart, z0 = zs + idxstart;
+    if (bw < 16 || bw > 1024 || nframes < 2 || x0 < 4 || z0 < 4 || x0 + bw > d.nx - 4 || z0 + bw > d.nz - 4 ||
+        !(srcden > 0) || !(srcpvel > 0) || srcsvel < 0 || !(h > 0) || !(dt > 0)) {
+        snprintf(msg, sizeof msg, "bad box: xs=%d zs=%d bw=%d nframes=%d in %dx%d", xs, zs, bw, nframes, d.nx,
+                 d.nz);
+        fd23_set_error_(msg);
+        return FD23_EINVAL;
+    }
+    fd23_box_free(c);
+    Box *b = new Box();
+    BoxPar &p = b->p;
+    p.x0 = x0;
+    p.z0 = z0;
+    p.bx = bw;
+    p.bz = bw;
+    p.px = fd23_ctx_px(c);
+    p.sstride = fd23_ctx_sstride(c);
+    p.nshots = d.nshots;
+    p.h = h;
+    p.dt = dt;
+    b->f64 = d.storage == FD23_STORE_F64;
+    b->nframes = nframes;
+    // the source medium scaled as material.cpp:52-58 forms it
+    double mu = srcsvel * srcsvel * srcden;
+    double lam = (srcpvel * srcpvel * srcden - 2.0 * mu);
+    double bb = 1.0 / srcden;
+    mu = mu * dt / h;
+    lam = lam * dt / h;
+    bb = bb * dt / h;
+    b->lam = lam;
+    b->mu = mu;
+    b->b = bb;
+    const size_t cs = b->f64 ? 8 : 4;
+    const size_t rbytes = (size_t)d.nshots * 5 * bw * bw * cs;
+    const size_t fbytes = (size_t)nframes * 2 * (bw + 8) * (bw + 8) * sizeof(double);
+    const size_t sbytes = (size_t)5 * bw * bw * sizeof(double);
+    int *d_nforced = nullptr;
+    if (hipMalloc(&b->resid, rbytes) != hipSuccess || hipMalloc((void **)&b->gtab, 5 * sizeof(void *)) != hipSuccess ||
+        hipMalloc((void **)&b->frames, fbytes) != hipSuccess ||
+        hipMalloc((void **)&b->sdrive, sbytes) != hipSuccess ||
+        hipMalloc((void **)&d_nforced, sizeof(int)) != hipSuccess) {
+        fd23_set_error_("box allocation failed");
+        if (b->resid) (void)hipFree(b->resid);
+        if (b->gtab) (void)hipFree(b->gtab);
+        if (b->frames) (void)hipFree(b->frames);
+        if (b->sdrive) (void)hipFree(b->sdrive);
+        if (d_nforced) (void)hipFree(d_nforced);
+        delete b;
+        return FD23_ENOMEM;
+    }
+    hipStream_t st = fd23_ctx_stream(c);
+    void *tab[5];
+    for (int f = 0; f < 5; f++) tab[f] = fd23_ctx_field(c, f);
+    (void)hipMemsetAsync(b->resid, 0, rbytes, st);
+    (void)hipMemsetAsync(b->sdrive, 0, sbytes, st);
+    (void)hipMemsetAsync(d_nforced, 0, sizeof(int), st);
+    (void)hipMemcpyAsync(b->gtab, tab, sizeof tab, hipMemcpyHostToDevice, st);
+    hipError_t e = hipMemcpyAsync(b->frames, frames, fbytes, hipMemcpyHostToDevice, st);
+    const int nblk = (int)std::min<long>(((long)bw * bw + 255) / 256, 2048);
+    if (b->f64)
+        hipLaunchKernelGGL((k_box_material_force<double>), dim3(nblk), dim3(256), 0, st, p,
+                           (double *)fd23_ctx_material(c, 0), (double *)fd23_ctx_material(c, 1),
+                           (double *)fd23_ctx_material(c, 2), (double *)fd23_ctx_material(c, 3),
+                           (double *)fd23_ctx_material(c, 4), bb, mu, lam, d_nforced);
+    else
+        hipLaunchKernelGGL((k_box_material_force<float>), dim3(nblk), dim3(256), 0, st, p,
+                           (float *)fd23_ctx_material(c, 0), (float *)fd23_ctx_material(c, 1),
+                           (float *)fd23_ctx_material(c, 2), (float *)fd23_ctx_material(c, 3),
+                           (float *)fd23_ctx_material(c, 4), bb, mu, lam, d_nforced);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&b->nforced, d_nforced, sizeof(int), hipMemcpyDeviceToHost, st);
+    const hipError_t es = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = es;
+    (void)hipFree(d_nforced);
+    if (e != hipSuccess) {
+        snprintf(msg, sizeof msg, "box set-up failed: %s", hipGetErrorString(e));
+        fd23_set_error_(msg);
+        (void)hipFree(b->resid);
+        (void)hipFree(b->gtab);
+        (void)hipFree(b->frames);
+        (void)hipFree(b->sdrive);
+        delete b;
+        return FD23_EHIP;
+    }
+    fd23_ctx_box(c) = b;
+    fd23_ctx_invalidate_graphs(c);
+    return FD23_OK;
+}
+
+// box cells whose global material the last fd23_box_configure_frames set
+// to the source medium (the reference prints each, material.cpp:77-85)
+extern "C" int fd23_box_forced_cells(fd23_ctx *c) {
+    Box *b = get(c);
+    return b ? b->nforced : 0;
+}
+
 // residual-grid fields of one shot (U W Txx Tzz Txz, bz*bx each, compute type)
 extern "C" int fd23_box_get_residual(fd23_ctx *c, int shot, void *out) {
     Box *b = get(c);
@@ -440,6 +699,8 @@ void fd23_box_free(fd23_ctx *c) {
     if (!b) return;
     if (b->resid) (void)hipFree(b->resid);
     if (b->gtab) (void)hipFree(b->gtab);
+    if (b->frames) (void)hipFree(b->frames);
+    if (b->sdrive) (void)hipFree(b->sdrive);
     delete b;
     fd23_ctx_box(c) = nullptr;
 }

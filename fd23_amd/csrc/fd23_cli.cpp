@@ -16,6 +16,8 @@
 //   each snapshot and progress-print step) -> optional state%06d ->
 //   raw time-major seismograms {output}_U/_W or _V -> ISIS files.
 // Keys the one-GPU build reads but does not need: gpuid, ngpu, usetable.
+// psv2d_db usebox=1 (srcden, srcsvel, srcpvel, sourcefile, bw) drives the
+// reference's box from its sourcefile frames (fd23_box_configure_frames).
 // snapbox (PSV/clean_nobox_new) writes windowed snapshots to ./OUTPUT/.
 // Additional key: storage=f32|f16|bf16|f64 (default f32).  usepunch=1
 // reads before, after and timefile (an nz*nx travel-time grid, float32, or
@@ -176,12 +178,9 @@ int main(int argc, char **argv) {
     int v;
     // real parameters: float in the FP32 trees, double in the FP64 one
     auto real = [&](const char *k) { return dbl ? par.md(k) : (double)par.mf(k); };
+    int usebox = 0;
     if (dbl) {
-        if (par.mi("usebox") == 1) {
-            fprintf(stderr, "usebox=1 (box source files) is not supported by this driver: the plane-wave box is "
-                            "fd23_box_configure\n");
-            return -1;
-        }
+        usebox = par.mi("usebox");
         usepunch = par.mi("usepunch");
         par.mi("ngpu");
     } else if (psv) {
@@ -194,6 +193,17 @@ int main(int argc, char **argv) {
     }
     const int nx = par.mi("nx"), nz = par.mi("nz"), nt = par.mi("nt");
     const double h = real("h"), dt = real("dt");
+    // the FP64 tree's box keys (PSV/psv_clean_double/param/param.cpp:79-85)
+    double srcden = 0, srcsvel = 0, srcpvel = 0;
+    std::string sourcefile;
+    int bw = 0;
+    if (usebox == 1) {
+        srcden = real("srcden");
+        srcsvel = real("srcsvel");
+        srcpvel = real("srcpvel");
+        sourcefile = par.ms("sourcefile");
+        bw = par.mi("bw");
+    }
     char srctype = '1';
     int stype = 3;
     double src_shift_x = 0.25, src_shift_z = -0.25; // FP64 tree, param.cpp:87-91
@@ -218,6 +228,10 @@ int main(int argc, char **argv) {
     if (srctime[0] == 'G' || srctime[0] == 'g') alpha = real("alpha");
     const double strike = real("strike"), dip = real("dip"), rake = real("rake"), azimuth = real("azimuth");
     const int xs = par.mi("xs"), zs = par.mi("zs");
+    if (usebox == 1 && (zs < bw / 2 + 4 || nz - zs < bw / 2 + 4)) { // PARAM::make_plan, param.cpp:179-182
+        fprintf(stderr, "zs %d is too near to the boundary\n", zs);
+        return -1;
+    }
     par.has_i("itprint", itprint);
     const int npml = par.mi("npml");
     const double pml_dt = real("pml_dt"), pml_v = real("pml_v"), pml_r = real("pml_r"), pml_fc = real("pml_fc");
@@ -429,6 +443,23 @@ int main(int argc, char **argv) {
     else
         CHECK(fd23_sh_moment(kind, strike, dip, rake, azimuth, h, dt, M));
 
+    // box source: nt+1 displacement frames of U then W on (bw+8)^2 cells,
+    // read in full before any device work (the reference reads one per step
+    // and would reuse a stale frame past the end of the file; a short file
+    // is an error here)
+    std::vector<double> frames;
+    if (usebox == 1) {
+        const size_t fsz = (size_t)2 * (bw + 8) * (bw + 8);
+        frames.resize(fsz * (size_t)(nt + 1));
+        std::ifstream f(sourcefile, std::ios::binary);
+        f.read((char *)frames.data(), (std::streamsize)(frames.size() * sizeof(double)));
+        if (!f) {
+            fprintf(stderr, "cannot read %d box source frames of %zu doubles from %s\n", nt + 1, fsz,
+                    sourcefile.c_str());
+            return -1;
+        }
+    }
+
     // ---- context ----------------------------------------------------------
     const int first_frame = (nt0 + itrecord - 1) / itrecord;
     const int end_frame = (nt0 + nt + itrecord - 1) / itrecord;
@@ -448,7 +479,14 @@ int main(int argc, char **argv) {
     CHECK(fd23_ctx_create(&d, &ctx));
     CHECK(fd23_set_material(ctx, matp.data()));
     CHECK(fd23_set_cpml(ctx, cpml.data()));
-    CHECK(fd23_set_source(ctx, 0, kind, xs, zs, stf.data(), lsrc, M));
+    if (usebox == 1) {
+        // the box replaces the point source (PSV/psv_clean_double/main.cpp:153-200)
+        CHECK(fd23_box_configure_frames(ctx, xs, zs, bw, frames.data(), nt + 1, srcpvel, srcsvel, srcden, h, dt));
+        const int nf = fd23_box_forced_cells(ctx);
+        if (nf > 0) fprintf(stderr, "No homogenous region: %d box cells set to the source medium\n", nf);
+    } else {
+        CHECK(fd23_set_source(ctx, 0, kind, xs, zs, stf.data(), lsrc, M));
+    }
     for (size_t c = 0; c < rx.size(); c++)
         CHECK(fd23_set_receivers(ctx, 0, (int)c, rx[c].data(), rz[c].data(), (int)rx[c].size()));
     if (usepunch == 1 && dbl) CHECK(fd23_set_punch_f64(ctx, times_d.data(), before, after, dt));

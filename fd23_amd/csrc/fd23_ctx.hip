@@ -185,6 +185,7 @@ int fd23_box_launch_post_stress(fd23_ctx *c, int k);
 int fd23_box_launch_post_velocity(fd23_ctx *c, int k);
 void fd23_box_free(fd23_ctx *c);
 void fd23_box_set_step(fd23_ctx *c, int it);
+int fd23_box_check_run(fd23_ctx *c, int it0, int nt);
 
 // accessors used by the box and material modules
 hipStream_t fd23_ctx_stream(fd23_ctx *c) { return c->stream; }
@@ -1354,6 +1355,7 @@ static int run_prologue(fd23_ctx *c, int it0, int nt, int itrecord) {
     int r = ready(c);
     if (r) return r;
     if (nt < 0 || it0 < 0 || itrecord < 1) return fail(FD23_EINVAL, "bad run range");
+    if (c->box && (r = fd23_box_check_run(c, it0, nt))) return r;
     if ((r = build_events(c))) return r;
     if (use_fused(c) && ((r = ensure_alt(c)) || (r = build_fused_events(c)))) return r;
     return init_scales(c);

@@ -227,6 +227,26 @@ int fd23_box_configure(fd23_ctx *ctx, int shot, int x0, int z0, int bx, int bz,
                        int incident, double angle_deg, double f0, double t0,
                        double amp, double vp, double vs, double rho,
                        double h, double dt);
+/* The reference's own box, driven by its sourcefile: a bw x bw square
+ * around (xs, zs) (cell (i,j) at (xs+s0+i, zs+s0+j), s0 = (bw+1)/2+1-bw),
+ * frames = nframes displacement frames, each U then W on (bw+8)^2 cells
+ * (x fastest, the sourcefile layout), step it driven by frames it and
+ * it+1: velocities (u[it+1]-u[it])/dt and stresses C grad u[it+1] / dt
+ * with the box material and the derivative order reduced toward the box
+ * centre.  The source medium (srcpvel, srcsvel, srcden) replaces the
+ * global material at box cells differing from it by more than 1e-4
+ * relative.  fd23_run returns FD23_EINVAL for steps whose frame it+1 is
+ * missing.  Replaces BOX::init_box, BOX::setboxsource and
+ * BOX::cu_copy_source (PSV/psv_clean_double/box/box.cpp:10-197,
+ * box/cu_box.cu:35-44) and MATERIAL::init_for_box_from_full
+ * (material/material.cpp:39-99); cu_copy_source's half-size copy (it
+ * copies sizeof(int)*bw*bw bytes of each double array) is not
+ * reproduced.                                                           */
+int fd23_box_configure_frames(fd23_ctx *ctx, int xs, int zs, int bw, const double *frames, int nframes,
+                              double srcpvel, double srcsvel, double srcden, double h, double dt);
+/* box cells whose material fd23_box_configure_frames set to the source
+ * medium (the reference prints one line per such cell)                  */
+int fd23_box_forced_cells(fd23_ctx *ctx);
 /* residual-grid fields of one shot, U W Txx Tzz Txz (bz*bx each; float, or
  * double for F64).  Replaces reading BOX's residual FIELD back.          */
 int fd23_box_get_residual(fd23_ctx *ctx, int shot, void *out);

@@ -643,6 +643,124 @@ class PlaneWaveBox:
                   _p(r["U"]), _p(r["W"]), _p(full), _p(full))
 
 
+# the FP64 tree's coefficient rows by order (PSV/psv_clean_double/param/const.h:13-18)
+G_COEF = np.array([[1.0, 0.0, 0.0, 0.0], [1.0, 0.0, 0.0, 0.0], [9. / 8., 1. / 24., 0.0, 0.0],
+                   [75. / 64., 25. / 384., 3. / 640., 0.0], [1225. / 1024., 245. / 3072., 49. / 5120., 5. / 7168.]])
+
+
+class FileBox:
+    """The reference's own box: a bw x bw square around (xs, zs) driven by
+    the displacement frames of its sourcefile.
+
+    Geometry and rings: BOX::init_box (PSV/psv_clean_double/box/box.cpp:
+    10-95), box cell (i, j) at global (xs + s0 + i, zs + s0 + j) with
+    s0 = (bw+1)/2 + 1 - bw.  Drive: BOX::setboxsource (:120-197), step it
+    from frames it (old) and it+1 (new), each (U, W) on (bw+8)^2 cells.
+    Box medium: MATERIAL::init_for_box_from_full (material/material.cpp:
+    39-99).  frames: (nframes, 2, bw+8, bw+8) float64."""
+
+    NAMES = ("U", "W", "Txx", "Tzz", "Txz")
+
+    def __init__(self, sim, xs, zs, bw, frames, srcpvel, srcsvel, srcden, h, dt):
+        s0 = (bw + 1) // 2 + 1 - bw
+        self.sim, self.bw, self.dt = sim, bw, dt
+        self.x0, self.z0, self.bx, self.bz = xs + s0, zs + s0, bw, bw
+        self.frames = np.ascontiguousarray(frames, np.float64)
+        assert self.frames.shape[1:] == (2, bw + 8, bw + 8)
+        mu = srcsvel * srcsvel * srcden
+        lam = (srcpvel * srcpvel * srcden - 2.0 * mu)
+        b = 1.0 / srcden
+        mu = mu * dt / h
+        lam = lam * dt / h
+        b = b * dt / h
+        sl = (slice(self.z0, self.z0 + bw), slice(self.x0, self.x0 + bw))
+        m = {k: sim.mat[k][sl] for k in ("BU", "BW", "MU", "MUA", "LAM")}
+        g = {k: v.astype(np.float64) for k, v in m.items()}
+        eps = 1e-4
+        bad = ((np.abs(b - g["BU"]) > b * eps) | (np.abs(b - g["BW"]) > b * eps) |
+               (np.abs(mu - g["MU"]) > mu * eps) | (np.abs(mu - g["MUA"]) > mu * eps) |
+               (np.abs(lam - g["LAM"]) > lam * eps))
+        for k, v in (("BU", b), ("BW", b), ("MU", mu), ("MUA", mu), ("LAM", lam)):
+            m[k][bad] = v
+        self.nforced = int(bad.sum())
+        self.bmat = {k: np.ascontiguousarray(sim.mat[k][sl]) for k in m}
+        dt_ = sim.dtype
+        self.r = {k: np.zeros((bw, bw), dt_) for k in self.NAMES}
+        j, i = np.mgrid[0:bw, 0:bw]
+        d = np.minimum(np.minimum(i, j), np.minimum(bw - 1 - i, bw - 1 - j))
+        self.sub, self.add = (d <= 3), (d >= 4) & (d <= 7)
+        # derivative order of each cell (box.cpp:151-152), tx = i + 4, ty = j + 4
+        mid = (bw + 1) // 2
+        ordr = np.minimum(4, np.maximum(np.abs(j + 4 - 3 - mid) - 1, np.abs(i + 4 - 3 - mid) - 1))
+        self.coef = G_COEF[np.maximum(ordr, 0)]
+        self.rstress = getattr(lib(), "or_box_stress" + _sfx(dt_))
+        self.rvel = getattr(lib(), "or_box_velocity" + _sfx(dt_))
+        self.rstress.restype = self.rvel.restype = None
+        self.s = None
+
+    def source(self, it):
+        """The five drive arrays of step it (bw x bw, float64), box.cpp:142-196."""
+        bw, dt = self.bw, self.dt
+        un, wn = self.frames[it + 1]
+        uo, wo = self.frames[it]
+        c0, c1, c2, c3 = (self.coef[..., k] for k in range(4))
+
+        def sh(f, dx, dz):  # f(tx + dx, ty + dz) over the box cells
+            return f[4 + dz:4 + dz + bw, 4 + dx:4 + dx + bw]
+
+        ux = c0 * (sh(un, 0, 0) - sh(un, -1, 0))
+        ux = ux - c1 * (sh(un, 1, 0) - sh(un, -2, 0))
+        ux = ux + c2 * (sh(un, 2, 0) - sh(un, -3, 0))
+        ux = ux - c3 * (sh(un, 3, 0) - sh(un, -4, 0))
+        uz = c0 * (sh(un, 0, 0) - sh(un, 0, -1))
+        uz = uz - c1 * (sh(un, 0, 1) - sh(un, 0, -2))
+        uz = uz + c2 * (sh(un, 0, 2) - sh(un, 0, -3))
+        uz = uz - c3 * (sh(un, 0, 3) - sh(un, 0, -4))
+        wx = c0 * (sh(wn, 1, 0) - sh(wn, 0, 0))
+        wx = wx - c1 * (sh(wn, 2, 0) - sh(wn, -1, 0))
+        wx = wx + c2 * (sh(wn, 3, 0) - sh(wn, -2, 0))
+        wx = wx - c3 * (sh(wn, 4, 0) - sh(wn, -3, 0))
+        wz = c0 * (sh(wn, 0, 1) - sh(wn, 0, 0))
+        wz = wz - c1 * (sh(wn, 0, 2) - sh(wn, 0, -1))
+        wz = wz + c2 * (sh(wn, 0, 3) - sh(wn, 0, -2))
+        wz = wz - c3 * (sh(wn, 0, 4) - sh(wn, 0, -3))
+        lam = self.bmat["LAM"].astype(np.float64)
+        mu = self.bmat["MU"].astype(np.float64)
+        mua = self.bmat["MUA"].astype(np.float64)
+        gam = lam + 2.0 * mu
+        return {"U": (sh(un, 0, 0) - sh(uo, 0, 0)) / dt, "W": (sh(wn, 0, 0) - sh(wo, 0, 0)) / dt,
+                "Txx": (gam * ux + lam * wz) / dt, "Tzz": (lam * ux + gam * wz) / dt,
+                "Txz": (mua * (wx + uz)) / dt}
+
+    def _ring(self, names, add):
+        sl = (slice(self.z0, self.z0 + self.bz), slice(self.x0, self.x0 + self.bx))
+        m = self.add if add else self.sub
+        for nm in names:
+            s = self.s[nm].astype(self.sim.dtype)
+            a = getattr(self.sim, nm)[sl]
+            if add:
+                a[m] = self.r[nm][m] + s[m]
+            else:
+                self.r[nm][m] = a[m] - s[m]
+
+    def pre_stress(self, it):
+        self.s = self.source(it)
+        self._ring(("U", "W"), True)
+
+    def post_stress(self, it):
+        self._ring(("U", "W"), False)
+        r, bm = self.r, self.bmat
+        self.rstress(ctypes.c_int(self.bx), ctypes.c_int(self.bz), _p(r["U"]), _p(r["W"]), _p(r["Txx"]),
+                     _p(r["Tzz"]), _p(r["Txz"]), _p(bm["LAM"]), _p(bm["MU"]), _p(bm["MUA"]))
+        self._ring(("Txx", "Tzz", "Txz"), True)
+
+    def post_velocity(self, it):
+        self._ring(("Txx", "Tzz", "Txz"), False)
+        r, bm = self.r, self.bmat
+        self.rvel(ctypes.c_int(self.bx), ctypes.c_int(self.bz), _p(r["Txx"]), _p(r["Tzz"]), _p(r["Txz"]),
+                  _p(r["U"]), _p(r["W"]), _p(bm["BU"]), _p(bm["BW"]))
+
+
 def run_psv_box(sim, box, nt, rx=None, rz=None, it0=0):
     """Box loop of PSV/psv_clean_double/main.cpp:153-200 (no point source)."""
     su, sw = [], []

@@ -183,3 +183,27 @@ def emulated_storage(p: Problem, fmt):
         for c, name in enumerate(comps):
             seis[c].append(sim.record(name, p.rx, p.rz))
     return [np.array(s) for s in seis]
+
+
+def point_source_frames(sim, nt, xs, zs, bw, stf, M):
+    """Displacement frames for the file-driven box (the sourcefile layout of
+    BOX::setboxsource: per frame U then W on the (bw+8)^2 window around the
+    bw x bw box at (xs, zs)) from a run of sim with a point source at
+    (xs, zs) and no box: frame k = dt * (sum of U, W after steps 0..k-2),
+    frames 0..nt.  Step it's drive is then the velocity before step it and
+    C grad(frame it+1)/dt the stress after its stress pass, wherever no
+    source acts.  sim is stepped nt times (its fields end as the run's)."""
+    s0 = (bw + 1) // 2 + 1 - bw
+    wx = slice(xs + s0 - 4, xs + s0 + bw + 4)
+    wz = slice(zs + s0 - 4, zs + s0 + bw + 4)
+    dt = sim.dt_frames
+    frames = np.zeros((nt + 1, 2, bw + 8, bw + 8))
+    acc = np.zeros((2, bw + 8, bw + 8))
+    for it in range(nt):
+        sim.stress(it)
+        sim.inject(0, it, stf, M, xs, zs)
+        sim.velocity(it)
+        if it + 2 <= nt:
+            acc = acc + dt * np.stack([sim.U[wz, wx], sim.W[wz, wx]]).astype(np.float64)
+            frames[it + 2] = acc
+    return frames

@@ -91,8 +91,16 @@ def test_fp64_tree_par_file(fp64dir):
     except Exception:
         pass
     assert r.returncode != 0 and "fd23_ctx_create" in r.stderr, r.stderr
+    # usebox=1 makes the box keys mandatory (param.cpp:79-85)
     r = _run("psv2d_db", ["par=0.par", "usebox=1"], fp64dir)
-    assert r.returncode != 0 and "usebox=1" in r.stderr
+    assert r.returncode == 100 and "must specify value for 'srcden'" in r.stderr
+    box = ["usebox=1", "srcden=3.0", "srcsvel=4.0", "srcpvel=6.8", "bw=24"]
+    r = _run("psv2d_db", ["par=0.par", "sourcefile=none.bin"] + box + ["zs=14"], fp64dir)
+    assert r.returncode != 0 and "zs 14 is too near to the boundary" in r.stderr
+    # a source file shorter than nt+1 frames (here 3 frames for nt=2001)
+    (fp64dir / "short.bin").write_bytes(b"\0" * (8 * 2 * 32 * 32 * 3))
+    r = _run("psv2d_db", ["par=0.par", "sourcefile=short.bin"] + box, fp64dir)
+    assert r.returncode != 0 and "cannot read 2002 box source frames" in r.stderr
     (fp64dir / "nobox.par").write_text(open(fp64dir / "0.par").read().replace("usebox=0", ""))
     r = _run("psv2d_db", ["par=nobox.par"], fp64dir)
     assert r.returncode != 0 and "must specify value for 'usebox'" in r.stderr

@@ -170,3 +170,62 @@ def test_snapbox_windows_the_snapshots(tmp_path):
     for n, f in full.items():
         win = np.fromfile(out / n, np.float32).reshape(41, 41)
         assert np.abs(f).max() > 0 and np.array_equal(win, f[20:61, 10:51]), n
+
+
+def test_fp64_tree_program_with_the_file_box(tmp_path):
+    """psv2d_db usebox=1 (the FP64 tree's box, PSV/psv_clean_double/
+    main.cpp:153-200, keys param.cpp:79-85): a sourcefile of displacement
+    frames written here from a point-source run of the oracle in the box
+    medium; the program's raw seismograms against the oracle's FileBox run
+    driven by the same file (1e-10), and against the point-source run
+    itself, which the box reproduces outside it (1e-9)."""
+    import oracle as O
+    from fd23_amd import host
+    from helpers import point_source_frames, unpack_cpml
+
+    print("case: psv2d_db usebox=1", flush=True)
+    nx, nz, npml, h, dt, nt = 256, 192, 20, 10.0, 8e-4, 300
+    vp, vs, rho = 3000.0, 1732.0, 2300.0
+    xs, zs, bw = 128, 100, 32
+    nrec, ix0, iz0, idx = 40, 20, 60, 5
+    mat, _ = host.psv_material(np.full((nz, nx), vp), np.full((nz, nx), vs), np.full((nz, nx), rho), h, dt,
+                               f64=True)
+    packed = host.cpml_pack(0, nx, nz, npml, dt, 1e-6, vp / h, 12.0)
+
+    def sim():
+        s = O.PSV(nx, nz, npml, dict(zip(("BU", "BW", "MU", "MUA", "LAM"), [m.copy() for m in mat])),
+                  unpack_cpml(packed, 0, nx, nz, np.float64), dtype=np.float64)
+        s.dt_frames = dt
+        return s
+
+    rx, rz = host.line_receivers(nrec, ix0, iz0, idx, 0)
+    # frames from a point-source run; the same run's seismograms
+    free = sim()
+    stf = host.ricker(nt, dt, 12.0)
+    M = host.psv_moment(0, 0, 0, 0, 0, h, dt, explosion=True).astype(np.float64)
+    frames = point_source_frames(free, nt, xs, zs, bw, stf, M)
+    free2 = sim()
+    fu, fw = O.run_psv(free2, nt, stf, M, xs, zs, rx=rx, rz=rz)
+    frames.tofile(tmp_path / "box.bin")
+    # the oracle's file box on the same inputs
+    osim = sim()
+    box = O.FileBox(osim, xs, zs, bw, frames, vp, vs, rho, h, dt)
+    ou, ow = O.run_psv_box(osim, box, nt, rx, rz)
+    (tmp_path / "box.mdl").write_text("BACK %g %g %g\n" % (vp, vs, rho))
+    par = "\n".join([
+        "usebox=1", "usepunch=0", "ngpu=1", "nx=%d" % nx, "nz=%d" % nz, "nt=%d" % nt, "h=%g" % h, "dt=%g" % dt,
+        "srcden=%g" % rho, "srcsvel=%g" % vs, "srcpvel=%g" % vp, "sourcefile=box.bin", "bw=%d" % bw,
+        "sourcetime=Gaussian", "alpha=-5", "strike=0", "dip=0", "rake=0", "azimuth=0",
+        "xs=%d" % xs, "zs=%d" % zs, "npml=%d" % npml, "pml_dt=%g" % dt, "pml_v=%g" % (vp / h), "pml_r=1e-6",
+        "pml_fc=12", "itrecord=1", "output=bx", "nrec=%d" % nrec, "ixrec0=%d" % ix0, "izrec0_u=%d" % iz0,
+        "izrec0_w=%d" % iz0, "idxrec=%d" % idx, "idzrec=0", "model=box.mdl", "ntsnap=100000", ""])
+    (tmp_path / "box.par").write_text(par)
+    exe = os.path.join(ROOT, "fd23_amd", "bin", "psv2d_db")
+    r = subprocess.run([exe, "par=box.par"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-300:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0 and "finish" in r.stdout
+    for comp, o, f in (("U", ou, fu), ("W", ow, fw)):
+        got = np.fromfile(tmp_path / ("bx_" + comp), np.float64).reshape(nt, -1)
+        e_box, e_free = rel(got, o), rel(got, f)
+        print(comp, "vs oracle file box", e_box, "vs point-source run", e_free, flush=True)
+        assert np.abs(o).max() > 0 and e_box < 1e-10 and e_free < 1e-9
