@@ -299,12 +299,14 @@ def calibrate_copy(nbytes=1 << 30, reps=10):
                       "GBps_read_plus_write": 2 * nbytes / el / 1e9}), flush=True)
 
 
-def bench_serve(nreq, max_active):
-    """Replay the mixed request stream through the serving engine."""
+def bench_serve(nreq, max_active, rate=40.0, models=0):
+    """Replay the mixed request stream through the serving engine (Poisson
+    arrivals at `rate` per second; models > 0: shot gathers on that many
+    shared problems, see serving_stream)."""
     from fd23_amd import _lib
     from fd23_amd.serve import Server, replay, serving_stream
 
-    stream = serving_stream(nreq)
+    stream = serving_stream(nreq, rate=rate, models=models)
     with Server(max_active=max_active, chunk_steps=256, max_batch_shots=64) as srv:
         recs, wall = replay(srv, stream)
         st = srv.stats()
@@ -325,14 +327,16 @@ def bench_serve(nreq, max_active):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic request stream (Poisson 40/s, seeded layered + GRF models)",
-        "config": {"workload": "serving_mixed_sh_psv", "requests": nreq, "max_active": max_active,
-                   "chunk_steps": 256},
+        "data": "synthetic request stream (Poisson %g/s, seeded layered + GRF models)" % rate,
+        "config": {"workload": "serving_mixed_sh_psv" if not models else "serving_shared_%d_models" % models,
+                   "requests": nreq, "max_active": max_active, "chunk_steps": 256, "rate_per_s": rate},
         "requests_per_s": round(len(ok) / wall, 3),
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                       "max": round(float(lat.max()), 4)},
         "queue_s": {"p50": round(float(np.percentile(que, 50)), 4), "p99": round(float(np.percentile(que, 99)), 4)},
         "offered_gcell_per_s": round(offered / 1e9, 2),
+        "service_s": {"p50": round(float(np.percentile(lat - que, 50)), 4),
+                      "p99": round(float(np.percentile(lat - que, 99)), 4)},
         "wall_s": round(wall, 3),
         "completed": len(ok),
         "failed": len(recs) - len(ok),
@@ -415,6 +419,8 @@ def main():
                              "material"])
     ap.add_argument("--requests", type=int, default=40, help="serve: requests in the stream")
     ap.add_argument("--max-active", type=int, default=8, help="serve: resident contexts")
+    ap.add_argument("--rate", type=float, default=40.0, help="serve: Poisson arrivals per second")
+    ap.add_argument("--models", type=int, default=0, help="serve: shot gathers on this many shared problems")
     ap.add_argument("--storage", choices=sorted(STORAGE), default=None)
     ap.add_argument("--accuracy", action="store_true",
                     help="also report seismogram error against an FP64 run of the same inputs")
@@ -433,7 +439,7 @@ def main():
         calibrate_copy()
         return
     if args.workload == "serve":
-        print(json.dumps(bench_serve(args.requests, args.max_active)), flush=True)
+        print(json.dumps(bench_serve(args.requests, args.max_active, args.rate, args.models)), flush=True)
         return
     if args.workload == "material":
         print(json.dumps(bench_material()), flush=True)

@@ -202,17 +202,27 @@ def make_solve(wave, storage, nx, nz, dx, nt, nshots, seed, npml=20, f0=10.0, bo
                  M, rx, rz, kind, box=box, model_key=seed if model_key is None else model_key, tag=tag, model=model)
 
 
-def serving_stream(n, seed=5005, rate=40.0, storages=(_lib.STORE_F32, _lib.STORE_F16)):
+def serving_stream(n, seed=5005, rate=40.0, storages=(_lib.STORE_F32, _lib.STORE_F16), models=0):
     """n requests: Poisson arrivals (rate per second), grids uniform over
     GRIDS, dx U(5,10) m, nt U{1000..8000}, 1-16 shots, SH or P-SV with
     p=0.5, FP32 or FP16 target with p=0.5, and a 200x100-cell plane-wave
     box (P or SV, Ricker 10 Hz, incidence U(0,30) degrees) on half of the
-    P-SV requests (a quarter of all).  Model seed 5005 + request id."""
+    P-SV requests (a quarter of all).  Model seed 5005 + request id.
+
+    models > 0: a survey-like stream instead, where every request is a shot
+    gather on one of `models` fixed problems (model, grid, physics, step
+    count and target drawn as above, seed 5005 + problem id) with its own
+    number of shots and shot positions, so requests on the same problem
+    may share a launch."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, n))
     out = []
     for i in range(n):
-        r = np.random.default_rng(seed + 1 + i)
+        if models > 0:
+            k = int(rng.integers(models))
+            r = np.random.default_rng(seed + 1 + k)
+        else:
+            r = np.random.default_rng(seed + 1 + i)
         nx, nz = GRIDS[r.integers(len(GRIDS))]
         wave = _lib.WAVE_PSV if r.random() < 0.5 else _lib.WAVE_SH
         storage = storages[r.integers(len(storages))]
@@ -224,8 +234,16 @@ def serving_stream(n, seed=5005, rate=40.0, storages=(_lib.STORE_F32, _lib.STORE
             bx, bz = 200, 100
             box = dict(x0=(nx - bx) // 2, z0=max(4, (nz - 20 - bz) // 2), bx=bx, bz=bz,
                        incident=int(r.integers(2)), angle=float(r.uniform(0.0, 30.0)), f0=10.0)
-        out.append((float(t[i]), make_solve(wave, storage, nx, nz, dx, nt, nshots, seed + 1 + i, box=box,
-                                            tag="req%d" % i)))
+        if models > 0:
+            nshots = int(rng.integers(1, 17))
+            s = make_solve(wave, storage, nx, nz, dx, nt, nshots, seed + 1 + k, box=box, tag="req%d" % i)
+            # this gather's shot positions: shifted by up to +-8 cells, kept
+            # between the C-PML strips
+            shift = int(rng.integers(-8, 9))
+            s.xs = np.clip(s.xs + shift, s.npml + 1, s.nx - s.npml - 2).astype(np.int32)
+        else:
+            s = make_solve(wave, storage, nx, nz, dx, nt, nshots, seed + 1 + i, box=box, tag="req%d" % i)
+        out.append((float(t[i]), s))
     return out
 
 

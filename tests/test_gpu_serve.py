@@ -162,3 +162,135 @@ def test_requests_with_model_equal_requests_with_material():
                 got.append(srv.result(rid)[0])
             assert np.abs(got[0]).max() > 0
             assert np.array_equal(got[0], got[1]), spec
+
+
+# ---------------------------------------------------------------------------
+# a slice of the serving configuration's stream, served and pinned to the
+# oracle
+# ---------------------------------------------------------------------------
+C5_NT_CAP, C5_SHOT_CAP = 1500, 3
+
+
+def _c5_slice(n=12):
+    """The first n requests of the serving stream (seed 5005: every grid
+    size of the mix, plane-wave boxes, FP16 targets), with the step count
+    capped at 1500 and the shots at 3 to bound the test; every request
+    keeps at least 1000 steps."""
+    from fd23_amd.serve import serving_stream
+
+    out = []
+    for at, s in serving_stream(n):
+        s.nt = min(s.nt, C5_NT_CAP)
+        s.xs, s.zs = s.xs[:C5_SHOT_CAP], s.zs[:C5_SHOT_CAP]
+        out.append((at, s))
+    return out
+
+
+def _oracle_request(s, shot, dtype, fmt=None):
+    """Seismograms [comp][frame][rec] of one shot of a served request on the
+    oracle: point source, plane-wave box when the request has one, FP32 or
+    FP64 arithmetic; fmt: FP16/BF16 storage emulated after every pass."""
+    import oracle as O
+    from helpers import unpack_cpml
+
+    cp = unpack_cpml(s.cpml, s.wave, s.nx, s.nz, dtype)
+    xs, zs = int(s.xs[shot]), int(s.zs[shot])
+    stf, M = s.stf.astype(dtype), s.M.astype(dtype)
+    if s.wave == _lib.WAVE_PSV:
+        sim = O.PSV(s.nx, s.nz, s.npml, dict(zip(("BU", "BW", "MU", "MUA", "LAM"), [m.copy() for m in s.material])),
+                    cp, s.topabsorb, dtype)
+        kind, comps = 0, ("U", "W")
+        after_s = lambda: [sim.Txx, sim.Tzz, sim.Txz] + [sim.psi[n] for n in ("U_x", "W_x", "U_z", "W_z")]
+        after_v = lambda: [sim.U, sim.W] + [sim.psi[n] for n in ("Txx_x", "Txz_x", "Txz_z", "Tzz_z")]
+    else:
+        sim = O.SH(s.nx, s.nz, s.npml, dict(zip(("BV", "MUX", "MUZ"), s.material)), cp, s.topabsorb, dtype)
+        kind, comps = s.src_kind, ("V",)
+        after_s = lambda: [sim.Tx, sim.Tz, sim.psi["V_x"], sim.psi["V_z"]]
+        after_v = lambda: [sim.V, sim.psi["Tx_x"], sim.psi["Tz_z"]]
+    box = None
+    if s.box:
+        b = s.box
+        box = O.PlaneWaveBox(sim, b["x0"], b["z0"], b["bx"], b["bz"], b["incident"], b["angle"], b["f0"], b["t0"],
+                             b["amp"], b["vp"], b["vs"], b["rho"], s.dx, s.dt)
+    seis = [[] for _ in comps]
+    for it in range(s.nt):
+        if box:
+            box.pre_stress(it)
+        sim.stress(it)
+        sim.inject(kind, it, stf, M, xs, zs)
+        if fmt:
+            for a in after_s():
+                O.round_storage(a, fmt)
+        if box:
+            box.post_stress(it)
+        sim.velocity(it)
+        if fmt:
+            for a in after_v():
+                O.round_storage(a, fmt)
+        for c, nm in enumerate(comps):
+            seis[c].append(sim.record(nm, s.rx, s.rz))
+        if box:
+            box.post_velocity(it)
+    return np.array(seis)
+
+
+def test_serving_stream_slice_against_the_oracle():
+    """Twelve requests of the serving stream submitted on their arrival
+    schedule (compressed) to one engine: FP32 requests up to 512x256 match
+    the FP32 oracle within 1e-5 (every shot); requests with a plane-wave box
+    (FP16 ones run at FP32 storage) match the FP64 oracle within 1.5x the
+    FP32 oracle's own error against it (shot 0); FP16 requests up to 512x256 match the
+    FP64 oracle within 1.3x the FP16 storage floor emulated on it (shot 0);
+    larger requests equal their alone runs."""
+    from fd23_amd.serve import Server
+
+    stream = _c5_slice()
+    sizes = {(s.nx, s.nz) for _, s in stream}
+    assert len(sizes) == 4 and any(s.box for _, s in stream)
+    assert any(s.storage == _lib.STORE_F16 for _, s in stream) and min(s.nt for _, s in stream) >= 1000
+    print("case: serving stream slice, %d requests" % len(stream), flush=True)
+    got = {}
+    with Server(max_active=4, chunk_steps=256) as srv:
+        ids = []
+        t0 = time.perf_counter()
+        for at, s in stream:
+            delay = at / 10.0 - (time.perf_counter() - t0)  # arrivals at 10x the stream's rate
+            if delay > 0:
+                time.sleep(delay)
+            ids.append((s, srv.submit(s)))
+        for s, rid in ids:
+            assert srv.wait(rid, 300.0) == _lib.REQ_DONE, s.tag
+            got[s.tag] = srv.result(rid)[0]
+        st = srv.stats()
+    print("stats", st, flush=True)
+    assert st["completed"] == len(stream) and st["peak_active"] >= 2
+    big = [s for _, s in stream if s.nx * s.nz > 512 * 256]
+    alone = _alone(big)
+    for _, s in stream:
+        g = got[s.tag]
+        assert np.isfinite(g).all() and np.abs(g).max() > 0, s.tag
+        eff = _lib.STORE_F32 if (s.box and s.storage == _lib.STORE_F16) else s.storage
+        if s.nx * s.nz > 512 * 256:
+            e = rel(g, alone[s.tag])
+            print(s.tag, s.nx, s.nz, "served vs alone", e, flush=True)
+            assert e <= TOL[eff], s.tag
+        elif eff == _lib.STORE_F32 and s.box:
+            # outside the box the global grid holds the scattered field, a
+            # small difference of amplitude-1 incident values: FP32 rounding
+            # of the incident wave sets the error floor there, so shot 0 is
+            # measured against the FP64 oracle beside the FP32 oracle's own
+            # error (the drive is numpy in the oracle: shot 0 only)
+            ref = _oracle_request(s, 0, np.float64)
+            e, floor = rel(g[0].astype(np.float64), ref), rel(_oracle_request(s, 0, np.float32), ref)
+            print(s.tag, s.nx, s.nz, "box: served vs FP64 oracle", e, "FP32 oracle vs FP64", floor, flush=True)
+            assert e <= max(1e-5, 1.5 * floor), s.tag
+        elif eff == _lib.STORE_F32:
+            errs = [rel(g[k], _oracle_request(s, k, np.float32)) for k in range(s.nshots)]
+            print(s.tag, s.nx, s.nz, "served vs FP32 oracle", errs, flush=True)
+            assert max(errs) < 1e-5, s.tag
+        else:
+            ref = _oracle_request(s, 0, np.float64)
+            floor = rel(_oracle_request(s, 0, np.float64, fmt="f16"), ref)
+            e = rel(g[0].astype(np.float64), ref)
+            print(s.tag, s.nx, s.nz, "served FP16 vs FP64 oracle", e, "floor", floor, flush=True)
+            assert e <= 1.3 * floor, s.tag
