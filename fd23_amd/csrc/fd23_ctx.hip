@@ -61,7 +61,7 @@ static const TileCfg kTiles[] = {
     {2, 1, 0, 0, 0},   {4, 1, 0, 0, 0},   {8, 1, 0, 0, 0},    {4, 2, 0, 0, 0},   {4, 4, 0, 0, 0},
     {4, 8, 0, 0, 0},   {8, 2, 0, 0, 0},   {8, 4, 0, 0, 0},    {2, 4, 0, 0, 0},   {2, 8, 0, 0, 0},
     {0, 1, 32, 4, 0},  {0, 1, 64, 4, 0},  {0, 1, 128, 4, 0},  {0, 1, 64, 2, 0},  {2, 1, 0, 0, 1},
-    {4, 1, 0, 0, 1}};
+    {4, 1, 0, 0, 1},   {2, 2, 0, 0, 1},   {4, 2, 0, 0, 1}};
 static const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 extern "C" int fd23_num_tiles(void) { return kNumTiles; }
@@ -992,6 +992,7 @@ template <class S> static void launch_pass(fd23_ctx *c, bool stress, const Event
     TILE_CASE(4, 1, false) TILE_CASE(2, 1, false) TILE_CASE(8, 1, false) TILE_CASE(4, 2, false)
     TILE_CASE(4, 4, false) TILE_CASE(4, 8, false) TILE_CASE(8, 2, false) TILE_CASE(8, 4, false)
     TILE_CASE(2, 4, false) TILE_CASE(2, 8, false) TILE_CASE(2, 1, true) TILE_CASE(4, 1, true)
+    TILE_CASE(2, 2, true) TILE_CASE(4, 2, true)
 #undef TILE_CASE
 }
 
@@ -1697,4 +1698,30 @@ extern "C" int fd23_autotune(fd23_ctx *c, int steps, double *best_ms) {
     fd23_set_tile(c, best);
     if (best_ms) *best_ms = bt;
     return FD23_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device copy with 16-byte lanes and non-temporal stores, grid-stride over
+// the buffer: the practical HBM read+write bandwidth the stencil passes are
+// measured against, and the plain copy whose counters scale FETCH_SIZE and
+// WRITE_SIZE
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_stream_copy(float4 *__restrict__ dst, const float4 *__restrict__ src,
+                                                     size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float4 v = src[i];
+        __builtin_nontemporal_store(v.x, &dst[i].x);
+        __builtin_nontemporal_store(v.y, &dst[i].y);
+        __builtin_nontemporal_store(v.z, &dst[i].z);
+        __builtin_nontemporal_store(v.w, &dst[i].w);
+    }
+}
+
+extern "C" int fd23_device_copy(void *dst, const void *src, size_t bytes, int grid, void *stream) {
+    if (!dst || !src || bytes % 16 != 0 || grid < 1 || grid > (1 << 20))
+        return fail(FD23_EINVAL, "device copy needs device buffers, a multiple of 16 bytes and 1..2^20 workgroups");
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(256), 0, (hipStream_t)stream, (float4 *)dst,
+                       (const float4 *)src, bytes / 16);
+    return check_launch();
 }

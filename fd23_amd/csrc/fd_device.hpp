@@ -26,18 +26,43 @@ constexpr int HALO = 4; // 8th-order staggered stencil reach
 // storage traits: S is the element type in HBM, C the arithmetic type.
 // Scaled types hold value/scale; scale is a power of two per array.
 // ---------------------------------------------------------------------------
+// Raw buffer resource over one array, for loads and stores addressed by a
+// per-lane byte offset plus a wave-uniform byte offset that stays in a
+// scalar register: an access then needs no vector address arithmetic.
+// Word 3 is gfx9's (32-bit data format, no swizzle, no thread-id add); the
+// range is 2 GiB, far above one shot's array, so no access is clipped.
+using BufRsrc = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ BufRsrc buf_rsrc(const void *origin) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(origin), (short)0, 0x7fffffff, 0x00020000);
+}
+
+// ld/st: element i of an array; ldb/stb: the same through a buffer
+// resource at byte offsets vo (per lane) + so (wave-uniform)
 template <class S> struct Store;
 template <> struct Store<float> {
     using C = float;
     static constexpr bool scaled = false;
     __device__ __forceinline__ static float ld(const float *p, long i, float) { return p[i]; }
     __device__ __forceinline__ static void st(float *p, long i, float v, float) { p[i] = v; }
+    __device__ __forceinline__ static float ldb(BufRsrc r, unsigned vo, unsigned so, float) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+    }
+    __device__ __forceinline__ static void stb(BufRsrc r, unsigned vo, unsigned so, float v, float) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, so, 0);
+    }
 };
 template <> struct Store<double> {
     using C = double;
     static constexpr bool scaled = false;
     __device__ __forceinline__ static double ld(const double *p, long i, float) { return p[i]; }
     __device__ __forceinline__ static void st(double *p, long i, double v, float) { p[i] = v; }
+    __device__ __forceinline__ static double ldb(BufRsrc r, unsigned vo, unsigned so, float) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+    }
+    __device__ __forceinline__ static void stb(BufRsrc r, unsigned vo, unsigned so, double v, float) {
+        using U2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, vo, so, 0);
+    }
 };
 template <> struct Store<__half> {
     using C = float;
@@ -48,6 +73,14 @@ template <> struct Store<__half> {
     __device__ __forceinline__ static void st(__half *p, long i, float v, float inv) {
         p[i] = __float2half_rn(v * inv);
     }
+    __device__ __forceinline__ static float ldb(BufRsrc r, unsigned vo, unsigned so, float s) {
+        const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0);
+        return __half2float(__builtin_bit_cast(__half, b)) * s;
+    }
+    __device__ __forceinline__ static void stb(BufRsrc r, unsigned vo, unsigned so, float v, float inv) {
+        const __half h = __float2half_rn(v * inv);
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), r, vo, so, 0);
+    }
 };
 template <> struct Store<__hip_bfloat16> {
     using C = float;
@@ -57,6 +90,14 @@ template <> struct Store<__hip_bfloat16> {
     }
     __device__ __forceinline__ static void st(__hip_bfloat16 *p, long i, float v, float inv) {
         p[i] = __float2bfloat16(v * inv);
+    }
+    __device__ __forceinline__ static float ldb(BufRsrc r, unsigned vo, unsigned so, float s) {
+        const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0);
+        return __bfloat162float(__builtin_bit_cast(__hip_bfloat16, b)) * s;
+    }
+    __device__ __forceinline__ static void stb(BufRsrc r, unsigned vo, unsigned so, float v, float inv) {
+        const __hip_bfloat16 h = __float2bfloat16(v * inv);
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), r, vo, so, 0);
     }
 };
 

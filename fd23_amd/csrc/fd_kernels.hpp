@@ -243,6 +243,31 @@ template <class S, int TZ, bool ZH, bool XH> struct Stage {
             }
         }
     }
+    // the same loads through a buffer resource whose origin is element
+    // (row -HALO, column -HALO) of the shot: pxb = row pitch in bytes,
+    // vcol = this lane's byte offset of column x0 + tx; ty is wave-uniform
+    __device__ __forceinline__ void loadb(BufRsrc rs, unsigned pxb, unsigned vcol, int x0, int z0, int ty,
+                                          float scl) {
+        constexpr unsigned SZ = sizeof(S);
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const int row = RB + ty + i * BY; // global row z0 - HALO + row
+            r[i] = Store<S>::ldb(rs, vcol, (unsigned)(z0 + row) * pxb, scl);
+        }
+        if (XH) {
+            const int tid = ty * TX + (int)threadIdx.x;
+#pragma unroll
+            for (int i = 0; i < NH; i++) {
+                const int e = tid + i * TX * BY;
+                if (NE % (TX * BY) == 0 || e < NE) {
+                    const int row = e >> 3, c = e & 7;
+                    const int col = c < HALO ? c - HALO : TX + c - HALO;
+                    const unsigned vo = (unsigned)(row + HALO) * pxb + (unsigned)(x0 + col + HALO) * SZ;
+                    h[i] = Store<S>::ldb(rs, vo, (unsigned)z0 * pxb, scl);
+                }
+            }
+        }
+    }
     __device__ __forceinline__ void store(C *sm) const {
         const int tx = threadIdx.x, ty = threadIdx.y;
 #pragma unroll
@@ -298,23 +323,26 @@ __device__ __forceinline__ void psv_stress_interior(C ux, C uz, C wx, C wz, C lm
 }
 
 // exact path of one stress cell; psi read from rd and (when own) written to
-// wr; sc: U_x W_x U_z W_z; mp: running max of U_x W_x W_z U_z
-template <class S, class C>
+// wr; sc: U_x W_x U_z W_z; mp: running max of U_x W_x W_z U_z.  PRE: the
+// psi values were loaded ahead (q: U_x W_x W_z U_z of the cell) and rd is
+// not read.
+template <class S, class C, bool PRE = false>
 __device__ __forceinline__ void psv_stress_exact(const Grid &g, int ix, int iz, int shot, C ux, C uz, C wx, C wz,
                                                  C lm, C mm, C ma, C &a, C &b, C &cc, const PsvStressPsi<S> &rd,
                                                  const PsvStressPsi<S> &wr, bool own,
-                                                 const PsvStressProf<C> &pr, const PsiScl4 &sc, float (&mp)[4]) {
+                                                 const PsvStressProf<C> &pr, const PsiScl4 &sc, float (&mp)[4],
+                                                 C q0 = C(0), C q1 = C(0), C q2 = C(0), C q3 = C(0)) {
     using ST = Store<S>;
     const int nx = g.nx, nz = g.nz, npml = g.npml;
     const C gam = lm + C(2.0) * mm;
     if (ix < npml || ix >= nx - npml) {
         const long pi = (long)shot * g.xpsi_stride + xpsi_idx(ix, iz, nx, npml);
-        C p1 = mad(pr.Ux.b[ix], ST::ld(rd.Ux, pi, sc.r[0]), pr.Ux.c[ix] * ux);
+        C p1 = mad(pr.Ux.b[ix], PRE ? q0 : ST::ld(rd.Ux, pi, sc.r[0]), pr.Ux.c[ix] * ux);
         if (own) ST::st(wr.Ux, pi, p1, sc.w[0]);
         const C e1 = mad(ux, pr.Ux.k[ix], p1);
         a = mad(gam, e1, a);
         b = mad(lm, e1, b);
-        C p2 = mad(pr.Wx.b[ix], ST::ld(rd.Wx, pi, sc.r[1]), pr.Wx.c[ix] * wx);
+        C p2 = mad(pr.Wx.b[ix], PRE ? q1 : ST::ld(rd.Wx, pi, sc.r[1]), pr.Wx.c[ix] * wx);
         if (own) ST::st(wr.Wx, pi, p2, sc.w[1]);
         cc = mad(ma, mad(wx, pr.Wx.k[ix], p2), cc);
         if (ST::scaled && own) {
@@ -327,12 +355,12 @@ __device__ __forceinline__ void psv_stress_exact(const Grid &g, int ix, int iz, 
     const bool topcell = iz < npml && g.top;
     if (topcell || iz >= nz - npml) {
         const long pi = (long)shot * g.zpsi_stride + zpsi_idx(ix, iz, nx, nz, npml, topcell);
-        C p1 = mad(pr.Wz.b[iz], ST::ld(rd.Wz, pi, sc.r[3]), pr.Wz.c[iz] * wz);
+        C p1 = mad(pr.Wz.b[iz], PRE ? q2 : ST::ld(rd.Wz, pi, sc.r[3]), pr.Wz.c[iz] * wz);
         if (own) ST::st(wr.Wz, pi, p1, sc.w[3]);
         const C e1 = mad(wz, pr.Wz.k[iz], p1);
         a = mad(lm, e1, a);
         b = mad(gam, e1, b);
-        C p2 = mad(pr.Uz.b[iz], ST::ld(rd.Uz, pi, sc.r[2]), pr.Uz.c[iz] * uz);
+        C p2 = mad(pr.Uz.b[iz], PRE ? q3 : ST::ld(rd.Uz, pi, sc.r[2]), pr.Uz.c[iz] * uz);
         if (own) ST::st(wr.Uz, pi, p2, sc.w[2]);
         cc = mad(ma, mad(uz, pr.Uz.k[iz], p2), cc);
         if (ST::scaled && own) {
@@ -363,7 +391,8 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
 
     TileId t;
     if (!map_tile<TZ, REGION>(w, t)) return;
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    // a wavefront is one row of 64 threads, so threadIdx.y is wave-uniform
+    const int tx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int x0 = t.x0, z0 = t.z0;
     const int nx = g.nx, nz = g.nz, px = g.px;
     const int gx = x0 + tx;
@@ -395,34 +424,73 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
     }
     float mx_xx = 0.f, mx_zz = 0.f, mx_xz = 0.f;
     float mp[4] = {0.f, 0.f, 0.f, 0.f}; // psi U_x W_x W_z U_z
+    // psi read scales and inverse write scales (exact path): U_x W_x U_z W_z
+    PsiScl4 psc{{1.f, 1.f, 1.f, 1.f}, {1.f, 1.f, 1.f, 1.f}};
+    if (ST::scaled && !interior) {
+        psc.r[0] = scl[SC_PU_X]; psc.r[1] = scl[SC_PW_X]; psc.r[2] = scl[SC_PU_Z]; psc.r[3] = scl[SC_PW_Z];
+        psc.w[0] = inv_pow2(scl[kNextScale + SC_PU_X]); psc.w[1] = inv_pow2(scl[kNextScale + SC_PW_X]);
+        psc.w[2] = inv_pow2(scl[kNextScale + SC_PU_Z]); psc.w[3] = inv_pow2(scl[kNextScale + SC_PW_Z]);
+    }
+    constexpr bool kPrePsi = REGION == REGION_EDGE;
+
+    // buffer addressing: byte pitch of a row and this lane's column
+    constexpr unsigned SZ = sizeof(S);
+    const unsigned pxb = (unsigned)px * SZ, vcol = (unsigned)(gx + HALO) * SZ;
+    const unsigned pxm = (unsigned)px * sizeof(C), vmat = (unsigned)gx * sizeof(C);
 
     // material for this thread's cells, reused across the SPB shots
+    const BufRsrc rLAM = buf_rsrc(m.LAM), rMU = buf_rsrc(m.MU), rMUA = buf_rsrc(m.MUA);
     C lam[RZ], mu[RZ], mua[RZ];
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
         const int gz = z0 + lz0 + r;
         const bool live = gx < nx && gz < nz;
-        const long q = (long)gz * px + gx;
-        lam[r] = live ? m.LAM[q] : C(0);
-        mu[r] = live ? m.MU[q] : C(0);
-        mua[r] = live ? m.MUA[q] : C(0);
+        const unsigned so = (unsigned)gz * pxm;
+        lam[r] = live ? Store<C>::ldb(rLAM, vmat, so, 1.f) : C(0);
+        mu[r] = live ? Store<C>::ldb(rMU, vmat, so, 1.f) : C(0);
+        mua[r] = live ? Store<C>::ldb(rMUA, vmat, so, 1.f) : C(0);
     }
 
     const int sh0 = t.group * SPB;
     for (int s = 0; s < SPB; s++) {
         const int shot = sh0 + s;
         if (shot >= g.nshots) break;
-        const long base = (long)shot * g.sstride;
+        // buffer origin of this shot's arrays: element (row -HALO, column -HALO)
+        const long org = (long)shot * g.sstride - (long)HALO * px - HALO;
+        const BufRsrc bU = buf_rsrc(f.U + org), bW = buf_rsrc(f.W + org);
+        const BufRsrc bXX = buf_rsrc(f.Txx + org), bZZ = buf_rsrc(f.Tzz + org), bXZ = buf_rsrc(f.Txz + org);
         Stage<S, TZ, true, true> rU, rW;
-        rU.load(f.U, base, px, x0, z0, sc_u);
-        rW.load(f.W, base, px, x0, z0, sc_w);
+        rU.loadb(bU, pxb, vcol, x0, z0, ty, sc_u);
+        rW.loadb(bW, pxb, vcol, x0, z0, ty, sc_w);
         C txx[RZ], tzz[RZ], txz[RZ];
 #pragma unroll
         for (int r = 0; r < RZ; r++) {
-            const long q = base + (long)(z0 + lz0 + r) * px + gx;
-            txx[r] = ST::ld(f.Txx, q, sc_xx);
-            tzz[r] = ST::ld(f.Tzz, q, sc_zz);
-            txz[r] = ST::ld(f.Txz, q, sc_xz);
+            const unsigned so = (unsigned)(z0 + lz0 + r + HALO) * pxb;
+            txx[r] = ST::ldb(bXX, vcol, so, sc_xx);
+            tzz[r] = ST::ldb(bZZ, vcol, so, sc_zz);
+            txz[r] = ST::ldb(bXZ, vcol, so, sc_xz);
+        }
+        // edge-only kernels: this shot's C-PML memory variables (U_x W_x
+        // W_z U_z) are loaded with the fields, ahead of the barrier
+        C pq[4][kPrePsi ? RZ : 1];
+        if constexpr (kPrePsi) {
+#pragma unroll
+            for (int r = 0; r < RZ; r++) {
+                const int iz = z0 + lz0 + r;
+                pq[0][r] = pq[1][r] = pq[2][r] = pq[3][r] = C(0);
+                if (gx >= nx || iz >= nz || !(upd >> r & 1u)) continue;
+                if (gx < g.npml || gx >= nx - g.npml) {
+                    const long pi = (long)shot * g.xpsi_stride + xpsi_idx(gx, iz, nx, g.npml);
+                    pq[0][r] = ST::ld(ps.Ux, pi, psc.r[0]);
+                    pq[1][r] = ST::ld(ps.Wx, pi, psc.r[1]);
+                }
+                const bool topcell = iz < g.npml && g.top;
+                if (topcell || iz >= nz - g.npml) {
+                    const long pi = (long)shot * g.zpsi_stride + zpsi_idx(gx, iz, nx, nz, g.npml, topcell);
+                    pq[2][r] = ST::ld(ps.Wz, pi, psc.r[3]);
+                    pq[3][r] = ST::ld(ps.Uz, pi, psc.r[2]);
+                }
+            }
         }
         if (s > 0) __syncthreads();
         rU.store(sU);
@@ -443,12 +511,6 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
                 psv_stress_interior(ux, uz, wx, wz, lam[r], mu[r], mua[r], txx[r], tzz[r], txz[r]);
             }
         } else {
-            PsiScl4 sc{{1.f, 1.f, 1.f, 1.f}, {1.f, 1.f, 1.f, 1.f}};
-            if (ST::scaled) {
-                sc.r[0] = scl[SC_PU_X]; sc.r[1] = scl[SC_PW_X]; sc.r[2] = scl[SC_PU_Z]; sc.r[3] = scl[SC_PW_Z];
-                sc.w[0] = inv_pow2(scl[kNextScale + SC_PU_X]); sc.w[1] = inv_pow2(scl[kNextScale + SC_PW_X]);
-                sc.w[2] = inv_pow2(scl[kNextScale + SC_PU_Z]); sc.w[3] = inv_pow2(scl[kNextScale + SC_PW_Z]);
-            }
 #pragma unroll
             for (int r = 0; r < RZ; r++) {
                 const int lz = lz0 + r;
@@ -462,8 +524,13 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
                 const C uz = dm(pu, LW, c0, c1, c2, c3);
                 const C wx = dp(pw, 1, c0, c1, c2, c3);
                 const C wz = dp(pw, LW, c0, c1, c2, c3);
-                psv_stress_exact<S, C>(g, ix, iz, shot, ux, uz, wx, wz, lam[r], mu[r], mua[r], txx[r], tzz[r],
-                                       txz[r], ps, ps, true, pr, sc, mp);
+                if constexpr (kPrePsi)
+                    psv_stress_exact<S, C, true>(g, ix, iz, shot, ux, uz, wx, wz, lam[r], mu[r], mua[r], txx[r],
+                                                 tzz[r], txz[r], ps, ps, true, pr, psc, mp, pq[0][r], pq[1][r],
+                                                 pq[2][r], pq[3][r]);
+                else
+                    psv_stress_exact<S, C>(g, ix, iz, shot, ux, uz, wx, wz, lam[r], mu[r], mua[r], txx[r], tzz[r],
+                                           txz[r], ps, ps, true, pr, psc, mp);
             }
         }
         // fused source injection: the owner of a tap cell adds it after the
@@ -491,10 +558,10 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
         for (int r = 0; r < RZ; r++) {
             const int gz = z0 + lz0 + r;
             if (gx >= nx || gz >= nz) continue;
-            const long q = base + (long)gz * px + gx;
-            ST::st(f.Txx, q, txx[r], wi_xx);
-            ST::st(f.Tzz, q, tzz[r], wi_zz);
-            ST::st(f.Txz, q, txz[r], wi_xz);
+            const unsigned so = (unsigned)(gz + HALO) * pxb;
+            ST::stb(bXX, vcol, so, txx[r], wi_xx);
+            ST::stb(bZZ, vcol, so, tzz[r], wi_zz);
+            ST::stb(bXZ, vcol, so, txz[r], wi_xz);
             if (ST::scaled && amax) { // the maxima are gathered on the step before a check
                 mx_xx = fmaxf(mx_xx, fabsf((float)txx[r]));
                 mx_zz = fmaxf(mx_zz, fabsf((float)tzz[r]));
@@ -556,20 +623,22 @@ __device__ __forceinline__ void psv_velocity_interior(C dxTxx, C dzTzz, C dxTxz,
     u = mad(bu, dzTxz, u); w = mad(bw, dzTzz, w);
 }
 
-// exact path of one velocity cell; sc: Txx_x Txz_x Txz_z Tzz_z; mp: the same
-template <class S, class C>
+// exact path of one velocity cell; sc: Txx_x Txz_x Txz_z Tzz_z; mp: the same.
+// PRE: the psi values were loaded ahead (q, in the same order).
+template <class S, class C, bool PRE = false>
 __device__ __forceinline__ void psv_velocity_exact(const Grid &g, int ix, int iz, int shot, C dxTxx, C dzTzz,
                                                    C dxTxz, C dzTxz, C b_u, C b_w, C &u, C &ww,
                                                    const PsvVelPsi<S> &rd, const PsvVelPsi<S> &wr, bool own,
-                                                   const PsvVelProf<C> &pr, const PsiScl4 &sc, float (&mp)[4]) {
+                                                   const PsvVelProf<C> &pr, const PsiScl4 &sc, float (&mp)[4],
+                                                   C q0 = C(0), C q1 = C(0), C q2 = C(0), C q3 = C(0)) {
     using ST = Store<S>;
     const int nx = g.nx, nz = g.nz, npml = g.npml;
     if (ix < npml || ix >= nx - npml) {
         const long pi = (long)shot * g.xpsi_stride + xpsi_idx(ix, iz, nx, npml);
-        C p1 = mad(pr.Txx_x.b[ix], ST::ld(rd.Txx_x, pi, sc.r[0]), pr.Txx_x.c[ix] * dxTxx);
+        C p1 = mad(pr.Txx_x.b[ix], PRE ? q0 : ST::ld(rd.Txx_x, pi, sc.r[0]), pr.Txx_x.c[ix] * dxTxx);
         if (own) ST::st(wr.Txx_x, pi, p1, sc.w[0]);
         u = mad(b_u, mad(dxTxx, pr.Txx_x.k[ix], p1), u);
-        C p2 = mad(pr.Txz_x.b[ix], ST::ld(rd.Txz_x, pi, sc.r[1]), pr.Txz_x.c[ix] * dxTxz);
+        C p2 = mad(pr.Txz_x.b[ix], PRE ? q1 : ST::ld(rd.Txz_x, pi, sc.r[1]), pr.Txz_x.c[ix] * dxTxz);
         if (own) ST::st(wr.Txz_x, pi, p2, sc.w[1]);
         ww = mad(b_w, mad(dxTxz, pr.Txz_x.k[ix], p2), ww);
         if (ST::scaled && own) {
@@ -583,8 +652,8 @@ __device__ __forceinline__ void psv_velocity_exact(const Grid &g, int ix, int iz
     const bool topcell = iz < npml && g.top;
     if (topcell || iz >= nz - npml) {
         const long pi = (long)shot * g.zpsi_stride + zpsi_idx(ix, iz, nx, nz, npml, topcell);
-        C p1 = mad(pr.Txz_z.b[iz], ST::ld(rd.Txz_z, pi, sc.r[2]), pr.Txz_z.c[iz] * dzTxz);
-        C p2 = mad(pr.Tzz_z.b[iz], ST::ld(rd.Tzz_z, pi, sc.r[3]), pr.Tzz_z.c[iz] * dzTzz);
+        C p1 = mad(pr.Txz_z.b[iz], PRE ? q2 : ST::ld(rd.Txz_z, pi, sc.r[2]), pr.Txz_z.c[iz] * dzTxz);
+        C p2 = mad(pr.Tzz_z.b[iz], PRE ? q3 : ST::ld(rd.Tzz_z, pi, sc.r[3]), pr.Tzz_z.c[iz] * dzTzz);
         if (own) {
             ST::st(wr.Txz_z, pi, p1, sc.w[2]);
             ST::st(wr.Tzz_z, pi, p2, sc.w[3]);
@@ -620,7 +689,8 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
 
     TileId t;
     if (!map_tile<TZ, REGION>(w, t)) return;
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    // a wavefront is one row of 64 threads, so threadIdx.y is wave-uniform
+    const int tx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int x0 = t.x0, z0 = t.z0;
     const int nx = g.nx, nz = g.nz, px = g.px;
     const int gx = x0 + tx;
@@ -648,34 +718,72 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
     }
     float mx_u = 0.f, mx_w = 0.f;
     float mp[4] = {0.f, 0.f, 0.f, 0.f}; // psi Txx_x Txz_x Txz_z Tzz_z
+    PsiScl4 psc{{1.f, 1.f, 1.f, 1.f}, {1.f, 1.f, 1.f, 1.f}};
+    if (ST::scaled && !interior) {
+        psc.r[0] = scl[SC_PTXX_X]; psc.r[1] = scl[SC_PTXZ_X]; psc.r[2] = scl[SC_PTXZ_Z]; psc.r[3] = scl[SC_PTZZ_Z];
+        psc.w[0] = inv_pow2(scl[kNextScale + SC_PTXX_X]); psc.w[1] = inv_pow2(scl[kNextScale + SC_PTXZ_X]);
+        psc.w[2] = inv_pow2(scl[kNextScale + SC_PTXZ_Z]); psc.w[3] = inv_pow2(scl[kNextScale + SC_PTZZ_Z]);
+    }
+    constexpr bool kPrePsi = REGION == REGION_EDGE;
 
+    // buffer addressing: byte pitch of a row and this lane's column
+    constexpr unsigned SZ = sizeof(S);
+    const unsigned pxb = (unsigned)px * SZ, vcol = (unsigned)(gx + HALO) * SZ;
+    const unsigned pxm = (unsigned)px * sizeof(C), vmat = (unsigned)gx * sizeof(C);
+
+    const BufRsrc rBU = buf_rsrc(m.BU), rBW = buf_rsrc(m.BW);
     C bu[RZ], bw[RZ];
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
         const int gz = z0 + lz0 + r;
         const bool live = gx < nx && gz < nz;
-        const long q = (long)gz * px + gx;
-        bu[r] = live ? m.BU[q] : C(0);
-        bw[r] = live ? m.BW[q] : C(0);
+        const unsigned so = (unsigned)gz * pxm;
+        bu[r] = live ? Store<C>::ldb(rBU, vmat, so, 1.f) : C(0);
+        bw[r] = live ? Store<C>::ldb(rBW, vmat, so, 1.f) : C(0);
     }
 
     const int sh0 = t.group * SPB;
     for (int s = 0; s < SPB; s++) {
         const int shot = sh0 + s;
         if (shot >= g.nshots) break;
-        const long base = (long)shot * g.sstride;
+        // buffer origin of this shot's arrays: element (row -HALO, column -HALO)
+        const long org = (long)shot * g.sstride - (long)HALO * px - HALO;
+        const BufRsrc bU = buf_rsrc(f.U + org), bW = buf_rsrc(f.W + org);
+        const BufRsrc bXX = buf_rsrc(f.Txx + org), bZZ = buf_rsrc(f.Tzz + org), bXZ = buf_rsrc(f.Txz + org);
         Stage<S, TZ, false, true> rXX;
         Stage<S, TZ, true, false> rZZ;
         Stage<S, TZ, true, true> rXZ;
-        rXX.load(f.Txx, base, px, x0, z0, sc_xx);
-        rZZ.load(f.Tzz, base, px, x0, z0, sc_zz);
-        rXZ.load(f.Txz, base, px, x0, z0, sc_xz);
+        rXX.loadb(bXX, pxb, vcol, x0, z0, ty, sc_xx);
+        rZZ.loadb(bZZ, pxb, vcol, x0, z0, ty, sc_zz);
+        rXZ.loadb(bXZ, pxb, vcol, x0, z0, ty, sc_xz);
         C nv[2][RZ];
 #pragma unroll
         for (int r = 0; r < RZ; r++) {
-            const long q = base + (long)(z0 + lz0 + r) * px + gx;
-            nv[0][r] = ST::ld(f.U, q, sc_u);
-            nv[1][r] = ST::ld(f.W, q, sc_w);
+            const unsigned so = (unsigned)(z0 + lz0 + r + HALO) * pxb;
+            nv[0][r] = ST::ldb(bU, vcol, so, sc_u);
+            nv[1][r] = ST::ldb(bW, vcol, so, sc_w);
+        }
+        // edge-only kernels: this shot's C-PML memory variables (Txx_x
+        // Txz_x Txz_z Tzz_z) are loaded with the fields, ahead of the barrier
+        C pq[4][kPrePsi ? RZ : 1];
+        if constexpr (kPrePsi) {
+#pragma unroll
+            for (int r = 0; r < RZ; r++) {
+                const int iz = z0 + lz0 + r;
+                pq[0][r] = pq[1][r] = pq[2][r] = pq[3][r] = C(0);
+                if (gx >= nx || iz >= nz || !(upd >> r & 1u)) continue;
+                if (gx < g.npml || gx >= nx - g.npml) {
+                    const long pi = (long)shot * g.xpsi_stride + xpsi_idx(gx, iz, nx, g.npml);
+                    pq[0][r] = ST::ld(ps.Txx_x, pi, psc.r[0]);
+                    pq[1][r] = ST::ld(ps.Txz_x, pi, psc.r[1]);
+                }
+                const bool topcell = iz < g.npml && g.top;
+                if (topcell || iz >= nz - g.npml) {
+                    const long pi = (long)shot * g.zpsi_stride + zpsi_idx(gx, iz, nx, nz, g.npml, topcell);
+                    pq[2][r] = ST::ld(ps.Txz_z, pi, psc.r[2]);
+                    pq[3][r] = ST::ld(ps.Tzz_z, pi, psc.r[3]);
+                }
+            }
         }
         if (s > 0) __syncthreads();
         rXX.store(sXX);
@@ -696,12 +804,6 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
                 psv_velocity_interior(dxTxx, dzTzz, dxTxz, dzTxz, bu[r], bw[r], nv[0][r], nv[1][r]);
             }
         } else {
-            PsiScl4 sc{{1.f, 1.f, 1.f, 1.f}, {1.f, 1.f, 1.f, 1.f}};
-            if (ST::scaled) {
-                sc.r[0] = scl[SC_PTXX_X]; sc.r[1] = scl[SC_PTXZ_X]; sc.r[2] = scl[SC_PTXZ_Z]; sc.r[3] = scl[SC_PTZZ_Z];
-                sc.w[0] = inv_pow2(scl[kNextScale + SC_PTXX_X]); sc.w[1] = inv_pow2(scl[kNextScale + SC_PTXZ_X]);
-                sc.w[2] = inv_pow2(scl[kNextScale + SC_PTXZ_Z]); sc.w[3] = inv_pow2(scl[kNextScale + SC_PTZZ_Z]);
-            }
 #pragma unroll
             for (int r = 0; r < RZ; r++) {
                 const int lz = lz0 + r;
@@ -714,17 +816,22 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
                 const C dzTzz = dm(&sZZ[o], LW, c0, c1, c2, c3);
                 const C dxTxz = dm(&sXZ[o], 1, c0, c1, c2, c3);
                 const C dzTxz = dp(&sXZ[o], LW, c0, c1, c2, c3);
-                psv_velocity_exact<S, C>(g, ix, iz, shot, dxTxx, dzTzz, dxTxz, dzTxz, bu[r], bw[r], nv[0][r],
-                                         nv[1][r], ps, ps, true, pr, sc, mp);
+                if constexpr (kPrePsi)
+                    psv_velocity_exact<S, C, true>(g, ix, iz, shot, dxTxx, dzTzz, dxTxz, dzTxz, bu[r], bw[r],
+                                                   nv[0][r], nv[1][r], ps, ps, true, pr, psc, mp, pq[0][r],
+                                                   pq[1][r], pq[2][r], pq[3][r]);
+                else
+                    psv_velocity_exact<S, C>(g, ix, iz, shot, dxTxx, dzTzz, dxTxz, dzTxz, bu[r], bw[r], nv[0][r],
+                                             nv[1][r], ps, ps, true, pr, psc, mp);
             }
         }
 #pragma unroll
         for (int r = 0; r < RZ; r++) {
             const int gz = z0 + lz0 + r;
             if (gx >= nx || gz >= nz) continue;
-            const long q = base + (long)gz * px + gx;
-            ST::st(f.U, q, nv[0][r], wi_u);
-            ST::st(f.W, q, nv[1][r], wi_w);
+            const unsigned so = (unsigned)(gz + HALO) * pxb;
+            ST::stb(bU, vcol, so, nv[0][r], wi_u);
+            ST::stb(bW, vcol, so, nv[1][r], wi_w);
             if (ST::scaled && amax) { // the maxima are gathered on the step before a check
                 mx_u = fmaxf(mx_u, fabsf((float)nv[0][r]));
                 mx_w = fmaxf(mx_w, fabsf((float)nv[1][r]));
@@ -771,7 +878,8 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
 
     TileId t;
     if (!map_tile<TZ>(w, t)) return;
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    // a wavefront is one row of 64 threads, so threadIdx.y is wave-uniform
+    const int tx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int x0 = t.x0, z0 = t.z0;
     const int nx = g.nx, nz = g.nz, px = g.px, npml = g.npml;
     const int gx = x0 + tx;
@@ -796,29 +904,37 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
     float mx_x = 0.f, mx_z = 0.f;
     float mp[2] = {0.f, 0.f}; // psi V_x V_z
 
+    // buffer addressing: byte pitch of a row and this lane's column
+    constexpr unsigned SZ = sizeof(S);
+    const unsigned pxb = (unsigned)px * SZ, vcol = (unsigned)(gx + HALO) * SZ;
+    const unsigned pxm = (unsigned)px * sizeof(C), vmat = (unsigned)gx * sizeof(C);
+
+    const BufRsrc rMUX = buf_rsrc(m.MUX), rMUZ = buf_rsrc(m.MUZ);
     C mux[RZ], muz[RZ];
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
         const int gz = z0 + lz0 + r;
         const bool live = gx < nx && gz < nz;
-        const long q = (long)gz * px + gx;
-        mux[r] = live ? m.MUX[q] : C(0);
-        muz[r] = live ? m.MUZ[q] : C(0);
+        const unsigned so = (unsigned)gz * pxm;
+        mux[r] = live ? Store<C>::ldb(rMUX, vmat, so, 1.f) : C(0);
+        muz[r] = live ? Store<C>::ldb(rMUZ, vmat, so, 1.f) : C(0);
     }
 
     const int sh0 = t.group * SPB;
     for (int s = 0; s < SPB; s++) {
         const int shot = sh0 + s;
         if (shot >= g.nshots) break;
-        const long base = (long)shot * g.sstride;
+        // buffer origin of this shot's arrays: element (row -HALO, column -HALO)
+        const long org = (long)shot * g.sstride - (long)HALO * px - HALO;
+        const BufRsrc bV = buf_rsrc(f.V + org), bX = buf_rsrc(f.Tx + org), bZ = buf_rsrc(f.Tz + org);
         Stage<S, TZ, true, true> rV;
-        rV.load(f.V, base, px, x0, z0, sc_v);
+        rV.loadb(bV, pxb, vcol, x0, z0, ty, sc_v);
         C vx[RZ], vz[RZ];
 #pragma unroll
         for (int r = 0; r < RZ; r++) {
-            const long q = base + (long)(z0 + lz0 + r) * px + gx;
-            vx[r] = ST::ld(f.Tx, q, sc_x);
-            vz[r] = ST::ld(f.Tz, q, sc_z);
+            const unsigned so = (unsigned)(z0 + lz0 + r + HALO) * pxb;
+            vx[r] = ST::ldb(bX, vcol, so, sc_x);
+            vz[r] = ST::ldb(bZ, vcol, so, sc_z);
         }
         if (s > 0) __syncthreads();
         rV.store(sV);
@@ -898,9 +1014,9 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
         for (int r = 0; r < RZ; r++) {
             const int gz = z0 + lz0 + r;
             if (gx >= nx || gz >= nz) continue;
-            const long q = base + (long)gz * px + gx;
-            ST::st(f.Tx, q, vx[r], wi_x);
-            ST::st(f.Tz, q, vz[r], wi_z);
+            const unsigned so = (unsigned)(gz + HALO) * pxb;
+            ST::stb(bX, vcol, so, vx[r], wi_x);
+            ST::stb(bZ, vcol, so, vz[r], wi_z);
             if (ST::scaled && amax) { // the maxima are gathered on the step before a check
                 mx_x = fmaxf(mx_x, fabsf((float)vx[r]));
                 mx_z = fmaxf(mx_z, fabsf((float)vz[r]));
@@ -933,7 +1049,8 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
 
     TileId t;
     if (!map_tile<TZ>(w, t)) return;
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    // a wavefront is one row of 64 threads, so threadIdx.y is wave-uniform
+    const int tx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
     const int x0 = t.x0, z0 = t.z0;
     const int nx = g.nx, nz = g.nz, px = g.px, npml = g.npml;
     const int gx = x0 + tx;
@@ -960,12 +1077,18 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
     float mx_v = 0.f;
     float mp[2] = {0.f, 0.f}; // psi Tx_x Tz_z
 
+    // buffer addressing: byte pitch of a row and this lane's column
+    constexpr unsigned SZ = sizeof(S);
+    const unsigned pxb = (unsigned)px * SZ, vcol = (unsigned)(gx + HALO) * SZ;
+    const unsigned pxm = (unsigned)px * sizeof(C), vmat = (unsigned)gx * sizeof(C);
+
+    const BufRsrc rBV = buf_rsrc(BV);
     C bv[RZ];
 #pragma unroll
     for (int r = 0; r < RZ; r++) {
         const int gz = z0 + lz0 + r;
         const bool live = gx < nx && gz < nz;
-        bv[r] = live ? BV[(long)gz * px + gx] : C(0);
+        bv[r] = live ? Store<C>::ldb(rBV, vmat, (unsigned)gz * pxm, 1.f) : C(0);
     }
 
     const int sh0 = t.group * SPB;
@@ -973,13 +1096,16 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
         const int shot = sh0 + s;
         if (shot >= g.nshots) break;
         const long base = (long)shot * g.sstride;
+        // buffer origin of this shot's arrays: element (row -HALO, column -HALO)
+        const long org = base - (long)HALO * px - HALO;
+        const BufRsrc bV = buf_rsrc(f.V + org), bX = buf_rsrc(f.Tx + org), bZ = buf_rsrc(f.Tz + org);
         Stage<S, TZ, false, true> rX;
         Stage<S, TZ, true, false> rZ;
-        rX.load(f.Tx, base, px, x0, z0, sc_x);
-        rZ.load(f.Tz, base, px, x0, z0, sc_z);
+        rX.loadb(bX, pxb, vcol, x0, z0, ty, sc_x);
+        rZ.loadb(bZ, pxb, vcol, x0, z0, ty, sc_z);
         C nv[1][RZ];
 #pragma unroll
-        for (int r = 0; r < RZ; r++) nv[0][r] = ST::ld(f.V, base + (long)(z0 + lz0 + r) * px + gx, sc_v);
+        for (int r = 0; r < RZ; r++) nv[0][r] = ST::ldb(bV, vcol, (unsigned)(z0 + lz0 + r + HALO) * pxb, sc_v);
         // stress imaging at the surface: the LDS copy of row 0 (LDS row
         // HALO - z0) takes -Tz[row 1], and the tile owning row 0 writes it back
         const bool img = z0 - HALO <= 0 && z0 + TZ + HALO > 0;
@@ -1069,7 +1195,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
         for (int r = 0; r < RZ; r++) {
             const int gz = z0 + lz0 + r;
             if (gx >= nx || gz >= nz) continue;
-            ST::st(f.V, base + (long)gz * px + gx, nv[0][r], wi_v);
+            ST::stb(bV, vcol, (unsigned)(gz + HALO) * pxb, nv[0][r], wi_v);
             if (ST::scaled && amax) mx_v = fmaxf(mx_v, fabsf((float)nv[0][r]));
         }
         if (ev.clock) fused_record<C, RZ, 1>(ev, it, t.tile * g.nshots + shot, sX, sX, nv, lz0);
