@@ -284,9 +284,15 @@ def accuracy_vs_f64(name, storages, nt, built=None):
 
 
 def calibrate_copy(nbytes=1 << 30, reps=10):
-    """A plain device copy of known size, for scaling HBM counters."""
-    import torch
+    """Device copies of known size: torch's copy and the library's
+    16-byte-lane streaming copy (fd23_device_copy) at several grid sizes,
+    for the practical read+write bandwidth and for scaling HBM counters."""
+    import ctypes
 
+    import torch
+    from fd23_amd import _lib
+
+    lib = _lib.load()
     x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
     torch.cuda.synchronize()
@@ -295,8 +301,18 @@ def calibrate_copy(nbytes=1 << 30, reps=10):
         y.copy_(x)
     torch.cuda.synchronize()
     el = (time.perf_counter() - t0) / reps
-    print(json.dumps({"copy_bytes_each_way": nbytes, "reps": reps, "ms": el * 1e3,
-                      "GBps_read_plus_write": 2 * nbytes / el / 1e9}), flush=True)
+    out = {"copy_bytes_each_way": nbytes, "reps": reps, "torch_copy_GBps": round(2 * nbytes / el / 1e9, 1)}
+    for grid in (1024, 4096, 16384, 65536):
+        lib.fd23_device_copy(ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()), nbytes, grid, None)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            _lib.check(lib.fd23_device_copy(ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()), nbytes,
+                                            grid, None))
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / reps
+        out["fd23_copy_grid%d_GBps" % grid] = round(2 * nbytes / el / 1e9, 1)
+    print(json.dumps(out), flush=True)
 
 
 def bench_serve(nreq, max_active, rate=40.0, models=0):

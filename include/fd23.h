@@ -284,6 +284,12 @@ int fd23_set_scales(fd23_ctx *ctx, const float *scales, int n);
 int fd23_get_scales(fd23_ctx *ctx, float *scales, unsigned *amax_bits, int n);
 /* algorithmic HBM bytes of one stress+velocity step for this context   */
 double fd23_bytes_per_step(const fd23_ctx *ctx);
+/* Device-to-device copy of `bytes` (a multiple of 16) by a grid-stride
+ * kernel of `grid` workgroups with 16-byte lanes, enqueued on `stream`
+ * (hipStream_t, NULL = default): the practical HBM bandwidth the passes
+ * are measured against, and the copy that scales the memory counters.
+ * No reference counterpart.                                             */
+int fd23_device_copy(void *dst, const void *src, size_t bytes, int grid, void *stream);
 /* Time the passes with HIP events on the context stream: one untimed
  * stress+velocity step, then nsteps (1..4096) stress passes back to back,
  * then nsteps velocity passes; fills the average milliseconds of each
