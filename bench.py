@@ -51,6 +51,20 @@ def fused_bytes(wl):
     return S * (10 * wf * N + 8 * wf * (nxp + nzp)) + 5 * wm * N
 
 
+def block_bytes(wl, k):
+    """Algorithmic bytes of one SH block launch of k steps: V, Tx and Tz
+    read once and written once, the four psi read and written in the C-PML
+    cells, the three material arrays read once (the recomputed halos are
+    re-reads, not algorithmic bytes)."""
+    from fd23_amd import _lib
+
+    S, N = wl.nshots, float(wl.nx) * wl.nz
+    wf = {_lib.STORE_F32: 4, _lib.STORE_F64: 8}[wl.storage]
+    nxp = 2.0 * wl.npml * wl.nz
+    nzp = (2.0 if wl.topabsorb else 1.0) * wl.npml * wl.nx
+    return S * (6 * wf * N + 4 * wf * (nxp + nzp)) + 3 * wf * N
+
+
 def pass_bytes(wl, ctx_bytes_per_step=None):
     """Algorithmic bytes of the stress and velocity passes of one step."""
     from fd23_amd import _lib
@@ -105,11 +119,13 @@ def run_gpu(wl, steps, warmup, autotune=True, tile=None, passes=True):
     finite = bool(np.isfinite(seis).all())
     kind = ctx.tile_kind()
     fused = kind == _lib.TILE_FUSED and wl.wave == _lib.WAVE_PSV and wl.punch_after is None
+    # steps per launch of the SH block step, 0 when another path runs
+    block = ctx.tile_shape()[2] if kind == _lib.TILE_BLOCK and wl.punch_after is None else 0
     names = kernel_names(ctx, wl)
     s_ms, v_ms = ctx.time_passes(20) if passes else (None, None)
     ctx.close()
     return dict(elapsed=el, stress_ms=s_ms, vel_ms=v_ms, tile=tile, tune_ms=tune_ms, finite=finite, fused=fused,
-                kernels=names)
+                block=block, kernels=names)
 
 
 STORE_TYPE = {0: "float", 1: "__half", 2: "__hip_bfloat16", 3: "double"}
@@ -118,15 +134,18 @@ STORE_TYPE = {0: "float", 1: "__half", 2: "__hip_bfloat16", 3: "double"}
 def kernel_names(ctx, wl):
     """Kernels of the (stress, velocity) passes as profilers print them, a
     tuple per pass: split tile shapes launch the interior tiles and the edge
-    tiles of a P-SV pass as two instances of one kernel; the fused step is
-    one kernel for both."""
+    tiles of a P-SV pass as two instances of one kernel; the fused step and
+    the SH block step are one kernel for both."""
     from fd23_amd import _lib
 
     rz, spb, seg, blk = ctx.tile_shape()
     kind = ctx.tile_kind()
     t = STORE_TYPE[wl.storage]
     if wl.wave == _lib.WAVE_SH:
-        if kind != _lib.TILE_TILED:  # the fused step is P-SV only: the tiled kernels run
+        if kind == _lib.TILE_BLOCK and wl.punch_after is None:
+            n = ("fd23::k_sh_block<%s, %d, %d, %d>" % (t, seg, rz, blk // rz),)
+            return (n, n)
+        if kind != _lib.TILE_TILED:  # a window runs tile 0's tiled kernels
             rz, spb = ctx.tile_shape(0)[:2]
         return (("fd23::k_sh_stress<%s, %d, %d>" % (t, rz, spb),), ("fd23::k_sh_velocity<%s, %d, %d>" % (t, rz, spb),))
     # a first-arrival window runs the whole-grid tiled kernels (tile 0's
@@ -504,6 +523,11 @@ def measure_line(wl, steps, warmup, autotune=True, tile=None):
     if r["fused"]:
         name, nbytes, kms, kname = "fused", fused_bytes(wl), r["stress_ms"], r["kernels"][0]
         step_bytes = nbytes
+    elif r["block"]:
+        # time_passes reports the time per step: one launch takes `block` steps
+        k = r["block"]
+        name, nbytes, kms, kname = "block", block_bytes(wl, k), r["stress_ms"] * k, r["kernels"][0]
+        step_bytes = nbytes / k
     elif r["stress_ms"] >= r["vel_ms"]:
         name, nbytes, kms, kname = "stress", sb, r["stress_ms"], r["kernels"][0]
         step_bytes = sb + vb
@@ -535,8 +559,8 @@ def measure_line(wl, steps, warmup, autotune=True, tile=None):
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(workload_id(wl), kname), "kernel_name": list(kname),
                      "kernel_ms": round(kms, 5), "algorithmic_bytes": nbytes,
-                     "stress_ms": None if r["fused"] else round(r["stress_ms"], 5),
-                     "velocity_ms": None if r["fused"] else round(r["vel_ms"], 5)},
+                     "stress_ms": None if r["fused"] or r["block"] else round(r["stress_ms"], 5),
+                     "velocity_ms": None if r["fused"] or r["block"] else round(r["vel_ms"], 5)},
         "finite": r["finite"],
     }
     if wl.punch_after is not None:

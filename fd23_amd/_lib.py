@@ -19,7 +19,7 @@ EINVAL, ENOMEM, EHIP, EIO, ESTAB = -1, -2, -3, -4, -5
 WAVE_PSV, WAVE_SH = 0, 1
 STORE_F32, STORE_F16, STORE_BF16, STORE_F64 = 0, 1, 2, 3
 SRC_LINE, SRC_P, SRC_FORCE = 0, 1, 2
-TILE_TILED, TILE_FUSED = 0, 1
+TILE_TILED, TILE_FUSED, TILE_BLOCK = 0, 1, 2
 
 # exported symbols and their signatures (kept in step with include/fd23.h)
 c_int, c_dbl, c_flt, c_vp, c_sz = ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_void_p, ctypes.c_size_t

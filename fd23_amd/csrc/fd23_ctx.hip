@@ -3,6 +3,7 @@
 #include "fd23.h"
 #include "fd_kernels.hpp"
 #include "fd_fused.hpp"
+#include "fd_block.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -47,13 +48,16 @@ extern "C" void fd23_set_error_(const char *msg) { g_err = msg; }
 
 // ---------------------------------------------------------------------------
 // tile shapes: two-pass tiles (rows per thread RZ, tile height 4*RZ, shots
-// per workgroup SPB) and fused P-SV steps (segment height FH rows)
+// per workgroup SPB), fused P-SV steps (segment height FH rows) and SH
+// block steps (BK steps per launch over regions of 64 x BW*BCZ cells, BW
+// wavefronts per workgroup)
 // ---------------------------------------------------------------------------
 // split: P-SV passes launch their interior tiles and their edge tiles as
 // two kernels, each compiled for one path (fewer registers, more waves per
 // SIMD; one more launch per pass, which small grids feel)
 struct TileCfg {
     int rz, spb, fh, fr, split; // fused: segment height fh, rows per block fr
+    int bk, bcz, bw;            // block: steps per launch, rows per thread, wavefronts
 };
 // tile 0, the shape untuned contexts run (the serving engine, the driver),
 // is the one the autotuner picks on most workloads measured
@@ -61,16 +65,19 @@ static const TileCfg kTiles[] = {
     {2, 1, 0, 0, 0},   {4, 1, 0, 0, 0},   {8, 1, 0, 0, 0},    {4, 2, 0, 0, 0},   {4, 4, 0, 0, 0},
     {4, 8, 0, 0, 0},   {8, 2, 0, 0, 0},   {8, 4, 0, 0, 0},    {2, 4, 0, 0, 0},   {2, 8, 0, 0, 0},
     {0, 1, 32, 4, 0},  {0, 1, 64, 4, 0},  {0, 1, 128, 4, 0},  {0, 1, 64, 2, 0},  {2, 1, 0, 0, 1},
-    {4, 1, 0, 0, 1},   {2, 2, 0, 0, 1},   {4, 2, 0, 0, 1}};
+    {4, 1, 0, 0, 1},   {2, 2, 0, 0, 1},   {4, 2, 0, 0, 1},
+    {0, 1, 0, 0, 0, 1, 3, 8},  {0, 1, 0, 0, 0, 1, 3, 16}, {0, 1, 0, 0, 0, 2, 3, 16},
+    {0, 1, 0, 0, 0, 2, 4, 16}, {0, 1, 0, 0, 0, 2, 6, 8}};
 static const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 extern "C" int fd23_num_tiles(void) { return kNumTiles; }
 extern "C" int fd23_tile_shape(int tile, int *rz, int *spb, int *seg_rows, int *block_rows) {
     if (tile < 0 || tile >= kNumTiles) return fail(FD23_EINVAL, "bad tile %d", tile);
-    if (rz) *rz = kTiles[tile].rz;
-    if (spb) *spb = kTiles[tile].spb;
-    if (seg_rows) *seg_rows = kTiles[tile].fh;
-    if (block_rows) *block_rows = kTiles[tile].fr;
+    const TileCfg &t = kTiles[tile];
+    if (rz) *rz = t.bk > 0 ? t.bcz : t.rz;
+    if (spb) *spb = t.spb;
+    if (seg_rows) *seg_rows = t.bk > 0 ? t.bk : t.fh;
+    if (block_rows) *block_rows = t.bk > 0 ? t.bw * t.bcz : t.fr;
     return FD23_OK;
 }
 extern "C" int fd23_tile_split(int tile) {
@@ -79,7 +86,7 @@ extern "C" int fd23_tile_split(int tile) {
 }
 extern "C" int fd23_tile_kind(int tile) {
     if (tile < 0 || tile >= kNumTiles) return fail(FD23_EINVAL, "bad tile %d", tile);
-    return kTiles[tile].fh > 0 ? FD23_TILE_FUSED : FD23_TILE_TILED;
+    return kTiles[tile].bk > 0 ? FD23_TILE_BLOCK : kTiles[tile].fh > 0 ? FD23_TILE_FUSED : FD23_TILE_TILED;
 }
 
 constexpr int kMaxTZ = 32;     // largest tile height (BY * max RZ)
@@ -154,6 +161,15 @@ struct fd23_ctx {
     bool fevents_dirty = true;
     int fevents_tile = -1; // tile the lists were built for
 
+    // per-(tile, shot) lists of the SH block step: taps of every region cell,
+    // receivers of the tile's own cells
+    int *d_btap_start = nullptr, *d_brec_start = nullptr;
+    TapT *d_btaps = nullptr;
+    RecT *d_brecs = nullptr;
+    size_t btap_start_cap = 0, brec_start_cap = 0, btaps_cap = 0, brecs_cap = 0;
+    bool bevents_dirty = true;
+    int bevents_tile = -1;
+
     // per-tile lists of the fused graph loop
     int *d_tap_start = nullptr, *d_rec_start = nullptr;
     TapT *d_tapsT = nullptr;
@@ -172,8 +188,13 @@ struct fd23_ctx {
     int tile = 0;
     bool have_mat = false, have_cpml = false;
 
-    // graphs keyed by (itrecord, steps)
-    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs; // (itrecord, steps, current set)
+    // graphs keyed by (itrecord, steps, current set); flips: the chunk
+    // swaps the buffer sets an odd number of times
+    struct Chunk {
+        hipGraphExec_t exec;
+        bool flips;
+    };
+    std::map<std::tuple<int, int, int>, Chunk> graphs;
 
     // box coupling state (see fd23_box.hip)
     void *box = nullptr;
@@ -198,7 +219,7 @@ const fd23_desc &fd23_ctx_desc(fd23_ctx *c) { return c->d; }
 int *fd23_ctx_clock(fd23_ctx *c) { return c->clock; }
 // captured graphs bake in the per-step launches; the box changes them
 void fd23_ctx_invalidate_graphs(fd23_ctx *c) {
-    for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
+    for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second.exec);
     c->graphs.clear();
 }
 const float *fd23_ctx_scl(fd23_ctx *c) { return c->scl; }
@@ -238,8 +259,8 @@ static size_t layout(const fd23_desc *d, int *px, int *rows, long *sstride, long
     bytes += (size_t)2 * np * 3 * (d->nx + d->nz) * (*csize);
     bytes += (size_t)d->max_frames * S * nc * d->max_rec * (*csize);
     bytes += (size_t)S * nc * d->max_rec * sizeof(long);
-    // P-SV: the second field/psi set of the fused step
-    if (psv) {
+    // the second field/psi set of the fused (P-SV) and block (SH) steps
+    {
         bytes += (size_t)nf * ((size_t)S * (*sstride) + 2 * kGuard) * (*esize);
         bytes += (size_t)np * S * ((*xpsi) + (*zpsi)) * (*esize);
     }
@@ -257,8 +278,7 @@ extern "C" size_t fd23_ctx_device_bytes(const fd23_ctx *c) { return c ? fd23_des
 
 static void free_ctx(fd23_ctx *c) {
     if (!c) return;
-    for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
-    c->graphs.clear();
+    fd23_ctx_invalidate_graphs(c);
     fd23_box_free(c);
     for (auto p : c->field_alloc) if (p) hipFree(p);
     for (auto p : c->psi) if (p) hipFree(p);
@@ -268,7 +288,8 @@ static void free_ctx(fd23_ctx *c) {
     void *others[] = {c->prof, c->d_fieldtab, c->d_comptab, c->d_compfield, c->scl, c->amax, c->clock,
                       c->d_taps, c->d_src, c->d_stf, c->d_roff, c->d_nrec, c->seis, c->d_tap_start,
                       c->d_rec_start, c->d_tapsT, c->d_recs, c->d_ftap_start, c->d_frec_start, c->d_ftaps,
-                      c->d_frecs, c->d_pfirst, c->d_plast, c->d_pwin};
+                      c->d_frecs, c->d_pfirst, c->d_plast, c->d_pwin, c->d_btap_start, c->d_brec_start,
+                      c->d_btaps, c->d_brecs};
     for (auto p : others) if (p) hipFree(p);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -579,6 +600,7 @@ static int upload_sources(fd23_ctx *c) {
     c->src_dirty = false;
     c->events_dirty = true;
     c->fevents_dirty = true;
+    c->bevents_dirty = true;
     return FD23_OK;
 }
 
@@ -610,6 +632,7 @@ extern "C" int fd23_set_receivers(fd23_ctx *c, int shot, int comp, const int *rx
     HIPCHK(hipStreamSynchronize(c->stream));
     c->events_dirty = true;
     c->fevents_dirty = true;
+    c->bevents_dirty = true;
     return FD23_OK;
 }
 
@@ -617,10 +640,19 @@ extern "C" int fd23_set_receivers(fd23_ctx *c, int shot, int comp, const int *rx
 // per-tile event lists for the fused injection / gather of the graph loop
 // ---------------------------------------------------------------------------
 static bool tile_fused(int t) { return kTiles[t].fh > 0; }
-// the fused step runs in fd23_run for P-SV contexts without a box; the
-// staged entry points, the box and SH always use two-pass tiles
+static bool tile_block(int t) { return kTiles[t].bk > 0; }
+static bool scaled_storage(const fd23_ctx *c) {
+    return c->d.storage == FD23_STORE_F16 || c->d.storage == FD23_STORE_BF16;
+}
+// the fused step runs in fd23_run for P-SV contexts without a box, the
+// block step in fd23_run for SH contexts with FP32/FP64 storage; both
+// without a first-arrival window.  The staged entry points, the box and
+// the window use two-pass tiles (tile 0's shape in place of the others).
 static bool use_fused(const fd23_ctx *c) { return tile_fused(c->tile) && is_psv(c) && !c->box && !c->punch; }
-static int pass_tile(const fd23_ctx *c) { return tile_fused(c->tile) ? 0 : c->tile; }
+static bool use_block(const fd23_ctx *c) {
+    return tile_block(c->tile) && !is_psv(c) && !scaled_storage(c) && !c->punch;
+}
+static int pass_tile(const fd23_ctx *c) { return tile_fused(c->tile) || tile_block(c->tile) ? 0 : c->tile; }
 static int tile_rows(const fd23_ctx *c) { return BY * kTiles[pass_tile(c)].rz; }
 
 template <class T> static int upload_vec(fd23_ctx *c, T *&dev, size_t &cap, const std::vector<T> &v) {
@@ -714,8 +746,7 @@ static int build_events(fd23_ctx *c) {
     c->events_dirty = false;
     c->events_tile = pass_tile(c);
     // graphs hold the old list pointers
-    for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
-    c->graphs.clear();
+    fd23_ctx_invalidate_graphs(c);
     return FD23_OK;
 }
 
@@ -775,8 +806,77 @@ static int build_fused_events(fd23_ctx *c) {
     HIPCHK(hipStreamSynchronize(c->stream));
     c->fevents_dirty = false;
     c->fevents_tile = c->tile;
-    for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
-    c->graphs.clear();
+    fd23_ctx_invalidate_graphs(c);
+    return FD23_OK;
+}
+
+// Lists of the SH block step per (tile, shot): the source taps of every
+// cell of the tile's region (the halo recomputes them too), in the order of
+// the shot's tap table, and the receivers of the tile's own cells.  Cells
+// are numbered lz*64 + lx inside the region.
+static int build_block_events(fd23_ctx *c) {
+    if (!c->bevents_dirty && c->bevents_tile == c->tile) return FD23_OK;
+    const int K = kTiles[c->tile].bk, RH = kTiles[c->tile].bw * kTiles[c->tile].bcz;
+    const int H = kBlockReach * K, OX = TX - 2 * H, OZ = RH - 2 * H;
+    const int ntx = (c->d.nx + OX - 1) / OX, ntz = (c->d.nz + OZ - 1) / OZ;
+    const int S = c->d.nshots;
+    const size_t nkey = (size_t)ntx * ntz * S;
+    std::vector<std::vector<TapT>> tb(nkey);
+    std::vector<std::vector<RecT>> rb(nkey);
+    for (int s = 0; s < S; s++) {
+        const SrcInfo &si = c->h_srcinfo[s];
+        for (int i = 0; i < si.ntap; i++) {
+            const Tap &tp = c->h_taps[si.tap0 + i];
+            const int z = (int)(tp.off / c->px), x = (int)(tp.off % c->px);
+            // tiles whose region [t*O - H, t*O - H + extent) holds the cell
+            for (int tz_ = std::max(0, (z - RH + H) / OZ); tz_ < ntz && tz_ * OZ - H <= z; tz_++)
+                for (int tx_ = std::max(0, (x - TX + H) / OX); tx_ < ntx && tx_ * OX - H <= x; tx_++) {
+                    const int lx = x - (tx_ * OX - H), lz = z - (tz_ * OZ - H);
+                    if (lx < 0 || lx >= TX || lz < 0 || lz >= RH) continue;
+                    tb[(size_t)(tz_ * ntx + tx_) * S + s].push_back(TapT{lz * TX + lx, tp.field, tp.m, tp.q, tp.mode});
+                }
+        }
+        for (int comp = 0; comp < c->ncomp; comp++) {
+            const int n = c->h_nrec[(size_t)s * c->ncomp + comp];
+            const long base = ((long)s * c->ncomp + comp) * c->d.max_rec;
+            for (int r = 0; r < n; r++) {
+                const long off = c->h_roff[base + r];
+                if (off < 0) continue; // outside the grid: the frame keeps 0
+                const int z = (int)(off / c->px), x = (int)(off % c->px);
+                const int tx_ = x / OX, tz_ = z / OZ;
+                const int lx = x - (tx_ * OX - H), lz = z - (tz_ * OZ - H);
+                rb[(size_t)(tz_ * ntx + tx_) * S + s].push_back(RecT{lz * TX + lx, comp, (int)(base + r)});
+            }
+        }
+    }
+    std::vector<int> ts(nkey + 1), rs(nkey + 1);
+    std::vector<TapT> taps;
+    std::vector<RecT> recs;
+    for (size_t k = 0; k < nkey; k++) {
+        // the kernel holds a list in LDS
+        if (tb[k].size() > (size_t)kBlockTaps)
+            return fail(FD23_EINVAL, "block step: %zu source taps in one tile (at most %d)", tb[k].size(),
+                        kBlockTaps);
+        ts[k] = (int)taps.size();
+        rs[k] = (int)recs.size();
+        taps.insert(taps.end(), tb[k].begin(), tb[k].end());
+        recs.insert(recs.end(), rb[k].begin(), rb[k].end());
+    }
+    ts[nkey] = (int)taps.size();
+    rs[nkey] = (int)recs.size();
+    // one spare entry at the end of each array: the kernel reads entry
+    // min(i, n-1) of a list, which for an empty last list is this one
+    taps.push_back(TapT{0, 0, 0.0, 0.0, 0});
+    recs.push_back(RecT{0, 0, 0});
+    int r;
+    if ((r = upload_vec(c, c->d_btap_start, c->btap_start_cap, ts))) return r;
+    if ((r = upload_vec(c, c->d_brec_start, c->brec_start_cap, rs))) return r;
+    if ((r = upload_vec(c, c->d_btaps, c->btaps_cap, taps))) return r;
+    if ((r = upload_vec(c, c->d_brecs, c->brecs_cap, recs))) return r;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->bevents_dirty = false;
+    c->bevents_tile = c->tile;
+    fd23_ctx_invalidate_graphs(c);
     return FD23_OK;
 }
 
@@ -791,7 +891,7 @@ static int ensure_alt(fd23_ctx *c) {
         c->alt_fields[f] = (char *)c->alt_alloc[f] + (size_t)(kGuard + (long)HALO * c->px) * c->esize;
     }
     for (int p = 0; p < c->npsi; p++) {
-        const bool xdir = p < 4;
+        const bool xdir = p < c->npsi / 2; // x-psi arrays first (P-SV 4, SH 2)
         const size_t n = std::max<size_t>(1, S * (xdir ? c->xpsi : c->zpsi)) * c->esize;
         if (hipMalloc(&c->alt_psi[p], n) != hipSuccess) return fail(FD23_ENOMEM, "fused buffer set");
         HIPCHK(hipMemsetAsync(c->alt_psi[p], 0, n, c->stream));
@@ -1066,6 +1166,57 @@ static void dispatch_fused(fd23_ctx *c, int k, int itrecord, bool record = true,
     swap_sets(c);
 }
 
+// SH block step: nsub steps (at most the tile's bk) over the current set,
+// written to the other; events: the sources and receivers of the chunk's
+// steps k .. k+nsub-1 (off for timing)
+template <class S, int K, int CZ, int NW>
+static void launch_block_t(fd23_ctx *c, int k, int nsub, int itrecord, bool events) {
+    using C = typename Store<S>::C;
+    using B = BlockShape<K, CZ, NW>;
+    const Grid g = make_grid(c);
+    BlockWork w;
+    w.ntx = (c->d.nx + B::OX - 1) / B::OX;
+    w.ntz = (c->d.nz + B::OZ - 1) / B::OZ;
+    w.nwork = w.ntx * w.ntz * c->d.nshots;
+    w.nsub = nsub;
+    Events ev{};
+    if (events) {
+        ev.clock = c->clock;
+        ev.k = k;
+        ev.itrecord = itrecord;
+        ev.tap_start = c->d_btap_start;
+        ev.taps = c->d_btaps;
+        ev.src = c->d_src;
+        ev.stf = c->d_stf;
+        ev.rec_start = c->d_brec_start;
+        ev.recs = c->d_brecs;
+        ev.seis = c->seis;
+        ev.max_frames = c->seis ? c->d.max_frames : 0;
+        ev.frame_size = (long)c->d.nshots * c->ncomp * c->d.max_rec;
+    }
+    auto fields = [](void *const *f) { return ShFields<S>{(S *)f[0], (S *)f[1], (S *)f[2]}; };
+    auto psi = [](void *const *p) { return ShPsi4<S>{(S *)p[0], (S *)p[1], (S *)p[2], (S *)p[3]}; };
+    ShMat3<C> m{(const C *)c->mat[0], (const C *)c->mat[1], (const C *)c->mat[2]};
+    ShProf4<C> pr{prof_of<C>(c, 0), prof_of<C>(c, 1), prof_of<C>(c, 2), prof_of<C>(c, 3)};
+    hipLaunchKernelGGL((k_sh_block<S, K, CZ, NW>), dim3(round_up(w.nwork, 8)), dim3(TX, NW), 0, c->stream,
+                       fields(c->fields), fields(c->alt_fields), m, psi(c->psi), psi(c->alt_psi), pr, g, w, ev);
+}
+
+template <class S> static void launch_block_h(fd23_ctx *c, int k, int nsub, int itrecord, bool events) {
+    const TileCfg t = kTiles[c->tile];
+    if (t.bk == 1 && t.bw == 8) launch_block_t<S, 1, 3, 8>(c, k, nsub, itrecord, events);
+    else if (t.bk == 1) launch_block_t<S, 1, 3, 16>(c, k, nsub, itrecord, events);
+    else if (t.bw == 8) launch_block_t<S, 2, 6, 8>(c, k, nsub, itrecord, events);
+    else if (t.bcz == 3) launch_block_t<S, 2, 3, 16>(c, k, nsub, itrecord, events);
+    else launch_block_t<S, 2, 4, 16>(c, k, nsub, itrecord, events);
+}
+
+static void dispatch_block(fd23_ctx *c, int k, int nsub, int itrecord, bool events = true) {
+    if (c->d.storage == FD23_STORE_F64) launch_block_h<double>(c, k, nsub, itrecord, events);
+    else launch_block_h<float>(c, k, nsub, itrecord, events);
+    swap_sets(c);
+}
+
 template <class S> static void launch_inject_t(fd23_ctx *c, int it) {
     const int nthr = std::min(256, round_up(c->d.nshots, 64));
     hipLaunchKernelGGL((k_inject<S>), dim3(1), dim3(nthr), 0, c->stream, (S *const *)c->d_fieldtab, c->sstride,
@@ -1122,9 +1273,6 @@ static const Events kNoEvents{};
 // ---------------------------------------------------------------------------
 // FP16/BF16 scales
 // ---------------------------------------------------------------------------
-static bool scaled_storage(const fd23_ctx *c) {
-    return c->d.storage == FD23_STORE_F16 || c->d.storage == FD23_STORE_BF16;
-}
 static const int kScaleEvery = 8;            // steps between scale checks
 // stored max band (log2): grow the scale when the stored maximum passes
 // 2^11 (FP16 overflows at 2^16, so a field may grow 32x between checks),
@@ -1327,27 +1475,56 @@ static int enqueue_step(fd23_ctx *c, int k, int itrecord) {
 
 static const int kGraphSteps = 32;
 
-static int get_graph(fd23_ctx *c, int itrecord, int steps, hipGraphExec_t *out) {
+// the launches of steps k .. of a chunk, `left` steps remaining: a block
+// step takes up to the tile's bk steps, every other path one
+static int enqueue_launch(fd23_ctx *c, int k, int left, int itrecord, int *taken) {
+    if (use_block(c)) {
+        const int n = std::min(kTiles[c->tile].bk, left);
+        dispatch_block(c, k, n, itrecord);
+        *taken = n;
+        return FD23_OK;
+    }
+    *taken = 1;
+    return enqueue_step(c, k, itrecord);
+}
+
+// The graph of a chunk of `steps` steps starting on the current buffer set.
+// Capturing swaps the host's view of the sets as the launches do; the view
+// is put back afterwards, so that the caller swaps once per replay of a
+// chunk that flips.
+static int get_graph(fd23_ctx *c, int itrecord, int steps, fd23_ctx::Chunk *out) {
     const auto key = std::make_tuple(itrecord, steps, c->cur);
     auto it = c->graphs.find(key);
     if (it != c->graphs.end()) {
         *out = it->second;
         return FD23_OK;
     }
+    const int cur0 = c->cur;
     hipGraph_t graph;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int r = FD23_OK;
-    for (int s = 0; s < steps && r == FD23_OK; s++) r = enqueue_step(c, s, itrecord);
+    int r = FD23_OK, taken = 1;
+    for (int s = 0; s < steps && r == FD23_OK; s += taken) r = enqueue_launch(c, s, steps - s, itrecord, &taken);
     if (r == FD23_OK) hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c->stream, c->clock, steps);
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    const bool flips = c->cur != cur0;
+    if (flips) swap_sets(c);
     if (r) return r;
     if (e != hipSuccess) return fail(FD23_EHIP, "graph capture: %s", hipGetErrorString(e));
     hipGraphExec_t exec;
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(FD23_EHIP, "graph instantiate: %s", hipGetErrorString(e));
-    c->graphs[key] = exec;
-    *out = exec;
+    c->graphs[key] = fd23_ctx::Chunk{exec, flips};
+    *out = c->graphs[key];
+    return FD23_OK;
+}
+
+static int launch_chunk(fd23_ctx *c, int itrecord, int steps) {
+    fd23_ctx::Chunk ch;
+    int r = get_graph(c, itrecord, steps, &ch);
+    if (r) return r;
+    HIPCHK(hipGraphLaunch(ch.exec, c->stream));
+    if (ch.flips) swap_sets(c);
     return FD23_OK;
 }
 
@@ -1359,12 +1536,9 @@ static int run_prologue(fd23_ctx *c, int it0, int nt, int itrecord) {
     if (c->box && (r = fd23_box_check_run(c, it0, nt))) return r;
     if ((r = build_events(c))) return r;
     if (use_fused(c) && ((r = ensure_alt(c)) || (r = build_fused_events(c)))) return r;
+    if (use_block(c) && ((r = ensure_alt(c)) || (r = build_block_events(c)))) return r;
     return init_scales(c);
 }
-
-// A fused-step chunk of odd length ends on the other buffer set, which a
-// cached graph cannot track: it is enqueued directly.
-static bool chunk_as_graph(const fd23_ctx *c, int steps) { return !(use_fused(c) && (steps & 1)); }
 
 // nt steps run as full kGraphSteps-step graphs plus one graph of the
 // remaining steps, so every step of a run is replayed from a graph
@@ -1373,34 +1547,40 @@ extern "C" int fd23_run(fd23_ctx *c, int it0, int nt, int itrecord) {
     if (r) return r;
     hipLaunchKernelGGL(k_set_clock, dim3(1), dim3(64), 0, c->stream, c->clock, it0);
     const int full = nt / kGraphSteps, rest = nt % kGraphSteps;
-    if (full > 0) {
-        hipGraphExec_t g;
-        if ((r = get_graph(c, itrecord, kGraphSteps, &g))) return r;
-        for (int i = 0; i < full; i++) HIPCHK(hipGraphLaunch(g, c->stream));
-    }
-    if (rest > 0 && chunk_as_graph(c, rest)) {
-        hipGraphExec_t g;
-        if ((r = get_graph(c, itrecord, rest, &g))) return r;
-        HIPCHK(hipGraphLaunch(g, c->stream));
-    } else if (rest > 0) {
-        for (int i = 0; i < rest; i++)
-            if ((r = enqueue_step(c, i, itrecord))) return r;
-        hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, c->stream, c->clock, rest);
-    }
+    for (int i = 0; i < full; i++)
+        if ((r = launch_chunk(c, itrecord, kGraphSteps))) return r;
+    if (rest > 0 && (r = launch_chunk(c, itrecord, rest))) return r;
     if (c->box) fd23_box_set_step(c, it0 + nt);
     return check_launch();
 }
 
-// the graphs fd23_run(c, it0, nt, itrecord) replays, captured and
-// instantiated now without running, so that run starts with no capture
+// the graphs fd23_run(c, it0, nt, itrecord) replays (a chunk that flips the
+// buffer sets is replayed from both), captured and instantiated now without
+// running, so that run starts with no capture
 extern "C" int fd23_prepare(fd23_ctx *c, int nt, int itrecord) {
     int r = run_prologue(c, 0, nt, itrecord);
     if (r) return r;
-    hipGraphExec_t g;
-    if (nt >= kGraphSteps && (r = get_graph(c, itrecord, kGraphSteps, &g))) return r;
-    const int rest = nt % kGraphSteps;
-    if (rest > 0 && chunk_as_graph(c, rest) && (r = get_graph(c, itrecord, rest, &g))) return r;
-    return FD23_OK;
+    fd23_ctx::Chunk ch;
+    const int full = nt / kGraphSteps, rest = nt % kGraphSteps;
+    bool flips = false;
+    if (full > 0) {
+        if ((r = get_graph(c, itrecord, kGraphSteps, &ch))) return r;
+        flips = ch.flips;
+        if (flips && full > 1) {
+            swap_sets(c);
+            r = get_graph(c, itrecord, kGraphSteps, &ch);
+            swap_sets(c);
+            if (r) return r;
+        }
+    }
+    // the rest starts on the set the full chunks end on
+    if (rest > 0) {
+        const bool other = flips && (full & 1);
+        if (other) swap_sets(c);
+        r = get_graph(c, itrecord, rest, &ch);
+        if (other) swap_sets(c);
+    }
+    return r;
 }
 
 extern "C" int fd23_sync(fd23_ctx *c) {
@@ -1556,10 +1736,11 @@ extern "C" int fd23_set_state(fd23_ctx *c, int shot, const void *buf, size_t byt
 extern "C" int fd23_set_tile(fd23_ctx *c, int tile) {
     if (!c || tile < 0 || tile >= kNumTiles) return fail(FD23_EINVAL, "bad tile %d", tile);
     if (tile_fused(tile) && !is_psv(c)) return fail(FD23_EINVAL, "the fused step is P-SV only");
+    if (tile_block(tile) && (is_psv(c) || scaled_storage(c)))
+        return fail(FD23_EINVAL, "the block step is SH with FP32 or FP64 storage only");
     if (c->tile != tile) {
         c->tile = tile;
-        for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
-        c->graphs.clear();
+        fd23_ctx_invalidate_graphs(c);
     }
     return FD23_OK;
 }
@@ -1573,6 +1754,9 @@ extern "C" double fd23_bytes_per_step(const fd23_ctx *c) {
     const double nzp = (c->d.topabsorb ? 2.0 : 1.0) * c->d.npml * c->d.nx;
     // the fused step reads the five fields once and writes them once
     if (is_psv(c) && use_fused(c)) return S * (10 * wf * N + 8 * wf * (nxp + nzp)) + 5 * wm * N;
+    // the block step reads the state (V Tx Tz, four psi) and the material
+    // once per launch of bk steps and writes the state once
+    if (use_block(c)) return (S * (6 * wf * N + 4 * wf * (nxp + nzp)) + 3 * wm * N) / kTiles[c->tile].bk;
     if (is_psv(c)) return S * (15 * wf * N + 8 * wf * (nxp + nzp)) + 5 * wm * N;
     return S * (9 * wf * N + 4 * wf * (nxp + nzp)) + 3 * wm * N;
 }
@@ -1586,9 +1770,19 @@ extern "C" int fd23_time_passes(fd23_ctx *c, int nsteps, double *stress_ms, doub
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventCreate(&e2));
     const bool sc = scaled_storage(c);
-    const bool fused = use_fused(c);
+    const bool fused = use_fused(c), block = use_block(c);
     if (fused && ((r = ensure_alt(c)) || (r = build_fused_events(c)))) return r;
-    if (fused) {
+    if (block && (r = ensure_alt(c))) return r;
+    if (block) {
+        // nsteps block launches of bk steps back to back (no sources or
+        // receivers): the first value is the time per step, the second 0
+        const int K = kTiles[c->tile].bk;
+        dispatch_block(c, 0, K, 1, false);
+        hipEventRecord(e0, c->stream);
+        for (int i = 0; i < nsteps; i++) dispatch_block(c, 0, K, 1, false);
+        hipEventRecord(e1, c->stream);
+        hipEventRecord(e2, c->stream);
+    } else if (fused) {
         // nsteps fused steps back to back (no recording): the first value
         // is the step time, the second 0
         dispatch_fused(c, 0, 1, false);
@@ -1624,8 +1818,9 @@ extern "C" int fd23_time_passes(fd23_ctx *c, int nsteps, double *stress_ms, doub
     hipEventDestroy(e1);
     hipEventDestroy(e2);
     if (se != hipSuccess) return fail(FD23_EHIP, "%s", hipGetErrorString(se));
-    if (stress_ms) *stress_ms = a / nsteps;
-    if (velocity_ms) *velocity_ms = b / nsteps;
+    const int per = block ? kTiles[c->tile].bk : 1; // steps per launch
+    if (stress_ms) *stress_ms = a / nsteps / per;
+    if (velocity_ms) *velocity_ms = b / nsteps / per;
     return check_launch();
 }
 
@@ -1669,10 +1864,11 @@ extern "C" int fd23_autotune(fd23_ctx *c, int steps, double *best_ms) {
     for (int t = 0; t < kNumTiles; t++) {
         if (kTiles[t].spb > 1 && c->d.nshots < kTiles[t].spb) continue;
         // shapes that would run as another one: SH and a first-arrival
-        // window run split shapes unsplit, and the window and the box run
-        // fused shapes as tile 0
+        // window run split shapes unsplit, the window and the box run fused
+        // shapes as tile 0, and the window runs block shapes as tile 0
         if (kTiles[t].split && (!is_psv(c) || c->punch)) continue;
         if (tile_fused(t) && (!is_psv(c) || c->box || c->punch)) continue;
+        if (tile_block(t) && (is_psv(c) || scaled_storage(c) || c->punch)) continue;
         c->tile = t;
         double a, b;
         fd23_time_passes(c, 1, &a, &b); // warm-up

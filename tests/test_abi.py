@@ -86,3 +86,23 @@ def test_reduction_keys_alignment_check(tmp_path):
     got = dict(line.split() for line in out if line)
     assert got == {"0": "1", "8": "1", "256": "1", "4": "0", "67932": "0", "1": "0", str(3 * 333 * 17 * 8): "1",
                    "null": "0"}
+
+
+def test_tile_table_kinds_and_block_shapes():
+    """Every tile index reports a kind; block shapes report K steps per
+    launch (seg_rows), rows per thread (rz) and region rows (block_rows, a
+    multiple of rz), with a positive tile left inside the 64-column region
+    after the 7*K-cell halos."""
+    lib = _lib.load()
+    n = lib.fd23_num_tiles()
+    kinds = [lib.fd23_tile_kind(t) for t in range(n)]
+    assert set(kinds) == {_lib.TILE_TILED, _lib.TILE_FUSED, _lib.TILE_BLOCK}
+    assert lib.fd23_tile_kind(n) < 0 and lib.fd23_tile_kind(-1) < 0
+    for t in range(n):
+        if kinds[t] != _lib.TILE_BLOCK:
+            continue
+        v = [ctypes.c_int() for _ in range(4)]
+        assert lib.fd23_tile_shape(t, *(ctypes.byref(x) for x in v)) == 0
+        rz, spb, k, rows = (x.value for x in v)
+        assert spb == 1 and k in (1, 2) and rows % rz == 0
+        assert 64 - 14 * k > 0 and rows - 14 * k > 0

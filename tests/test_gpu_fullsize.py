@@ -145,13 +145,14 @@ def _oracle_for(wl, shot, nt):
 
 def test_c1_sh_full_run_vs_oracle():
     """The 400x200 SH configuration over its 2000 steps against the FP32
-    oracle: seismograms within 1e-5 relative L2; the fields at step 700
-    (the wave inside the grid) within 1e-5 relative L2, and at the end,
-    when the absorbed wave has left small residuals, within 1e-5 of the
-    step-700 peak.  The receivers at z=2 lie in the frozen rows of the
-    absorbing top (C-PML profiles are zero for the first 4 rows,
-    cpml.cpp:54-64): rows 0-3 only couple along x and stay zero, so both
-    sides record zeros there."""
+    oracle, on the two-pass tiles (tile 0) and on every block-step shape:
+    seismograms within 1e-5 relative L2; the fields at step 700 (the wave
+    inside the grid) within 1e-5 relative L2, and at the end, when the
+    absorbed wave has left small residuals, within 1e-5 of the step-700
+    peak.  The receivers at z=2 lie in the frozen rows of the absorbing top
+    (C-PML profiles are zero for the first 4 rows, cpml.cpp:54-64): rows
+    0-3 only couple along x and stay zero, so both sides record zeros
+    there."""
     import oracle as O
     from helpers import unpack_cpml
 
@@ -159,16 +160,6 @@ def test_c1_sh_full_run_vs_oracle():
     wl = W.config_sh_400x200().build()
     names = ("V", "Tx", "Tz")
     mid = 700
-    ctx = wl.context()
-    try:
-        ctx.run(0, mid, 1)
-        f_mid = {n: ctx.field(n) for n in names}
-        ctx.run(mid, wl.nt - mid, 1)
-        ctx.sync()
-        g = ctx.seismograms(0, 0, wl.nt)
-        f_end = {n: ctx.field(n) for n in names}
-    finally:
-        ctx.close()
     cp = unpack_cpml(wl.packed, wl.wave, wl.nx, wl.nz, np.float32)
     sim = O.SH(wl.nx, wl.nz, wl.npml, dict(zip(("BV", "MUX", "MUZ"), wl.mat)), cp, wl.topabsorb)
     rx, rz = wl.receivers()
@@ -177,16 +168,31 @@ def test_c1_sh_full_run_vs_oracle():
     v1 = O.run_sh(sim, mid, stf, M, xs, zs, kind=_lib.SRC_FORCE, rx=rx, rz=rz)
     o_mid = {n: getattr(sim, n).copy() for n in names}
     v2 = O.run_sh(sim, wl.nt - mid, stf, M, xs, zs, kind=_lib.SRC_FORCE, rx=rx, rz=rz, it0=mid)
-    e = rel(g, np.concatenate([v1, v2]))
-    _print("seis", e)
-    assert e < 1e-5
-    for n in names:
-        ref, peak = o_mid[n], float(np.abs(o_mid[n]).max())
-        e_mid = rel(f_mid[n], ref)
-        e_end = float(np.abs(f_end[n].astype(np.float64) - getattr(sim, n)).max() / peak)
-        _print(n, "step", mid, e_mid, "end, of the step-%d peak" % mid, e_end)
-        assert peak > 0 and e_mid < 1e-5 and e_end < 1e-5
-        assert not ref[:4].any() and not f_mid[n][:4].any()  # the frozen top rows
+    o_seis = np.concatenate([v1, v2])
+    lib = _lib.load()
+    tiles = [0] + [t for t in range(lib.fd23_num_tiles()) if lib.fd23_tile_kind(t) == _lib.TILE_BLOCK]
+    for tile in tiles:
+        ctx = wl.context()
+        try:
+            ctx.tile = tile
+            ctx.run(0, mid, 1)
+            f_mid = {n: ctx.field(n) for n in names}
+            ctx.run(mid, wl.nt - mid, 1)
+            ctx.sync()
+            g = ctx.seismograms(0, 0, wl.nt)
+            f_end = {n: ctx.field(n) for n in names}
+        finally:
+            ctx.close()
+        e = rel(g, o_seis)
+        _print("tile", tile, "seis", e)
+        assert e < 1e-5
+        for n in names:
+            ref, peak = o_mid[n], float(np.abs(o_mid[n]).max())
+            e_mid = rel(f_mid[n], ref)
+            e_end = float(np.abs(f_end[n].astype(np.float64) - getattr(sim, n)).max() / peak)
+            _print(n, "step", mid, e_mid, "end, of the step-%d peak" % mid, e_end)
+            assert peak > 0 and e_mid < 1e-5 and e_end < 1e-5
+            assert not ref[:4].any() and not f_mid[n][:4].any()  # the frozen top rows
 
 
 def test_c3_batch_first_and_last_shot_vs_oracle():

@@ -177,8 +177,10 @@ def test_all_tiles_bitwise_equal():
     _print("case: tiles")
     p = Problem(_lib.WAVE_PSV, 150, 90, 12, 10.0, 7e-4, 40, seed=9, sources=[(40, 6), (100, 9)])
     ref = None
-    ntile = _lib.load().fd23_num_tiles()
-    for t in range(ntile):
+    lib = _lib.load()
+    for t in range(lib.fd23_num_tiles()):
+        if lib.fd23_tile_kind(t) == _lib.TILE_BLOCK:  # SH only
+            continue
         with p.context() as ctx:
             ctx.tile = t
             ctx.run(0, p.nt, 1)
@@ -235,6 +237,91 @@ def test_fused_step_bitwise_equals_two_pass(nx, nz, npml, top, kind, storage):
         assert st == ref_st
         for a, b in zip(seis, ref_seis):
             assert np.array_equal(a, b)
+
+
+def _block_tiles():
+    lib = _lib.load()
+    return [t for t in range(lib.fd23_num_tiles()) if lib.fd23_tile_kind(t) == _lib.TILE_BLOCK]
+
+
+@pytest.mark.parametrize("nx,nz,npml,top,kind,storage", [
+    (173, 150, 12, False, _lib.SRC_FORCE, _lib.STORE_F32),
+    (130, 97, 10, True, _lib.SRC_P, _lib.STORE_F32),
+    (121, 60, 9, True, _lib.SRC_LINE, _lib.STORE_F64),
+    (17, 16, 3, False, _lib.SRC_FORCE, _lib.STORE_F32),  # smaller than one tile
+])
+def test_sh_block_step_bitwise_equals_two_pass(nx, nz, npml, top, kind, storage):
+    """The SH block step (up to K steps per launch over tiles whose halos
+    the launch recomputes, two buffer sets) runs the two-pass kernels'
+    per-cell code: states and seismograms must be identical across chunks
+    that end on either buffer set (replayed from cached graphs too),
+    launches of fewer steps than K, staged steps between block runs, and
+    sources and receivers on tile edges, grid edges and the surface."""
+    _print("case: sh block", nx, nz, npml, top, kind, storage)
+    srcs = [(nx // 2, 7), (3, 2), (nx - 5, nz // 2)]
+    p = Problem(_lib.WAVE_SH, nx, nz, npml, 10.0, 7e-4, 90, seed=31, topabsorb=top, kind=kind,
+                sources=srcs, rec_z=npml + 2 if top else 1, f64=storage == _lib.STORE_F64)
+    # tile edges of the block shapes: 50 or 36 columns, 10/18/20/34/36 rows
+    ex = np.array([0, nx - 1, 49, 50, 35, 36, 99, 100, 3, 5, 17, 18], np.int32) % nx
+    ez = np.array([nz - 1, 0, 9, 10, 17, 18, 19, 20, 33, 34, 35, 36], np.int32) % nz
+    p.rx = np.concatenate([p.rx, ex])
+    p.rz = np.concatenate([p.rz, ez])
+
+    def run(tile):
+        with p.context(storage=storage) as ctx:
+            ctx.tile = tile
+            ctx.run(0, 37, 1)    # one 32-step chunk and 5 steps
+            ctx.run(37, 28, 1)
+            for it in range(65, 68):
+                ctx.step_stress()
+                ctx.inject(it)
+                ctx.step_velocity()
+                ctx.record(it)
+            ctx.run(68, 7, 1)
+            for it0 in (75, 80, 85):  # the same 5-step chunk three times
+                ctx.run(it0, 5, 1)
+            ctx.sync()
+            st = b"".join(ctx.get_state(s) for s in range(len(srcs)))
+            seis = [ctx.seismograms(s, 0, p.nt) for s in range(len(srcs))]
+        return st, seis
+
+    ref_st, ref_seis = run(0)
+    assert any(np.abs(x).max() > 0 for x in ref_seis)
+    for t in _block_tiles():
+        st, seis = run(t)
+        _print("block tile", t, st == ref_st)
+        assert st == ref_st
+        for a, b in zip(seis, ref_seis):
+            assert np.array_equal(a, b)
+
+
+def test_sh_block_step_vs_oracle():
+    """The block step on its own against the oracle (FP32, 1e-5)."""
+    _print("case: sh block vs oracle")
+    p = Problem(_lib.WAVE_SH, 150, 90, 12, 10.0, 7e-4, 400, seed=150, kind=_lib.SRC_FORCE,
+                sources=[(75, 8)], rec_z=2)
+    sim, (ov,) = p.oracle()
+    for t in _block_tiles():
+        with p.context() as ctx:
+            ctx.tile = t
+            ctx.run(0, p.nt, 1)
+            gv = ctx.seismograms(0, 0, p.nt)
+            fields = {n: ctx.field(n) for n in ("V", "Tx", "Tz")}
+        e = rel(gv, ov)
+        _print("tile", t, "seis", e)
+        assert e < 1e-5
+        for n, f in fields.items():
+            assert rel(f, getattr(sim, n)) < 1e-5
+
+
+def test_block_tiles_refused_where_they_do_not_apply():
+    p = Problem(_lib.WAVE_PSV, 64, 32, 4, 10.0, 7e-4, 4, seed=1)
+    q = Problem(_lib.WAVE_SH, 64, 32, 4, 10.0, 7e-4, 4, seed=1)
+    t = _block_tiles()[0]
+    with p.context() as ctx, pytest.raises(Exception, match="block step"):
+        ctx.tile = t
+    with q.context(storage=_lib.STORE_F16) as ctx, pytest.raises(Exception, match="block step"):
+        ctx.tile = t
 
 
 def test_fused_step_f16_close_to_two_pass():
