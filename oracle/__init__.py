@@ -163,11 +163,13 @@ def ricker(n, dt, f0, t0=None):
 def round_storage(a, fmt):
     """Round the float64 array a in place as stored in fmt ("f32", "f16",
     "bf16") with one power-of-two scale putting its largest magnitude in
-    [2^8, 2^9) (or_round_storage; the storage floor's emulation)."""
+    [2^8, 2^9), or as block-scaled int16 ("blk16": one power-of-two
+    quantum per 64 consecutive values) (or_round_storage; the storage
+    floor's emulation)."""
     assert a.dtype == np.float64 and a.flags.c_contiguous
     f = lib().or_round_storage
     f.restype = None
-    f(_p(a), ctypes.c_long(a.size), ctypes.c_int({"f32": 0, "f16": 1, "bf16": 2}[fmt]))
+    f(_p(a), ctypes.c_long(a.size), ctypes.c_int({"f32": 0, "f16": 1, "bf16": 2, "blk16": 3}[fmt]))
 
 
 def psv_moment(kind, strike, dip, rake, azimuth, h, dt, dtype=np.float32):

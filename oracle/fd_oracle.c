@@ -66,8 +66,45 @@ static double or_round_bf16(double y)
     return (double)f;
 }
 
+/* Block-scaled int16 (fmt 3): each block of 64 consecutive values shares
+ * one power-of-two quantum q putting the block's largest magnitude in
+ * [2^14, 2^15) q; values round to the nearest multiple of q (ties to
+ * even) within |v| <= 32767 q.  A block whose largest magnitude is below
+ * 1e-280 (beneath any exponent range such a format would carry) is
+ * flushed to zero. */
+static void or_round_block16(double *a, long n)
+{
+#pragma omp parallel for
+    for (long b = 0; b < n; b += 64) {
+        const long e_ = b + 64 < n ? b + 64 : n;
+        double m = 0.0;
+        for (long i = b; i < e_; i++) {
+            const double v = fabs(a[i]);
+            if (v > m) m = v;
+        }
+        if (m == 0.0) continue;
+        if (m < 1e-280) {
+            for (long i = b; i < e_; i++) a[i] = 0.0;
+            continue;
+        }
+        int e;
+        frexp(m, &e); /* m in [2^(e-1), 2^e) */
+        const double q = ldexp(1.0, e - 15);
+        for (long i = b; i < e_; i++) {
+            double r = nearbyint(a[i] / q);
+            if (r > 32767.0) r = 32767.0;
+            if (r < -32767.0) r = -32767.0;
+            a[i] = r * q;
+        }
+    }
+}
+
 void or_round_storage(double *a, long n, int fmt)
 {
+    if (fmt == 3) {
+        or_round_block16(a, n);
+        return;
+    }
     double m = 0.0;
 #pragma omp parallel for reduction(max : m)
     for (long i = 0; i < n; i++) {

@@ -57,3 +57,43 @@ def test_c_rounding_equals_python_rounding():
             round_storage(b, fmt)
             O.round_storage(c, fmt)
             assert np.array_equal(b, c), (fmt, scale, np.flatnonzero(b != c)[:5])
+
+
+def test_block16_rounding_properties():
+    """Block-scaled int16 emulation (one power-of-two quantum per 64
+    values): every block's values become integer multiples of a quantum
+    q with the block maximum in [2^14, 2^15) q, the error is at most q/2,
+    values already on that grid are kept, rounding twice changes nothing,
+    and blocks are independent (a large value does not coarsen its
+    neighbours' blocks)."""
+    import oracle as O
+
+    rng = np.random.default_rng(8)
+    a = rng.standard_normal(64 * 50) * 10.0 ** np.repeat(rng.integers(-30, 5, 50), 64)
+    a[64 * 7:64 * 8] = 0.0                      # an all-zero block
+    a[64 * 9:64 * 10] *= 1e-290                 # beneath any exponent range: flushed
+    b = a.copy()
+    O.round_storage(b, "blk16")
+    for k in range(50):
+        x, y = a[64 * k:64 * (k + 1)], b[64 * k:64 * (k + 1)]
+        m = np.abs(x).max()
+        if m == 0.0 or m < 1e-280:
+            assert not y.any()
+            continue
+        q = 2.0 ** (np.floor(np.log2(m)) - 14)
+        assert np.all(np.abs(y - x) <= q / 2 * (1 + 1e-12))
+        assert np.array_equal(y / q, np.round(y / q)) and np.abs(y / q).max() <= 32767
+        assert 2 ** 14 <= np.abs(y).max() / q <= 2 ** 15
+    c = b.copy()
+    O.round_storage(c, "blk16")
+    assert np.array_equal(b, c)
+    # a ragged tail block (n not a multiple of 64)
+    t = a[:100].copy()
+    O.round_storage(t, "blk16")
+    assert np.array_equal(t[:64], b[:64]) and np.all(np.abs(t[64:] - a[64:100]) <= np.abs(a[64:100]).max() * 2 ** -14)
+    # the grid of integers times 2^-3 with the largest at 20000: kept exactly
+    g = np.arange(-32, 32, dtype=np.float64) * 2.0 ** -3
+    g[0] = 20000 * 2.0 ** -3
+    h = g.copy()
+    O.round_storage(h, "blk16")
+    assert np.array_equal(g, h)
