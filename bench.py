@@ -152,7 +152,9 @@ def kernel_names(ctx, wl):
     # shape in place of a fused one)
     window = wl.punch_after is not None
     if kind == _lib.TILE_FUSED and not window:
-        n = ("fd23::k_psv_fused<%s, %d, %d>" % (t, seg, blk),)
+        # block rows 0: the tile-parallel fused step, segments of 4*CZ rows
+        n = ("fd23::k_psv_fused_tile<%s, %d>" % (t, seg // 4),) if blk == 0 else \
+            ("fd23::k_psv_fused<%s, %d, %d>" % (t, seg, blk),)
         return (n, n)
     if kind == _lib.TILE_FUSED:
         rz, spb = ctx.tile_shape(0)[:2]

@@ -67,7 +67,8 @@ static const TileCfg kTiles[] = {
     {0, 1, 32, 4, 0},  {0, 1, 64, 4, 0},  {0, 1, 128, 4, 0},  {0, 1, 64, 2, 0},  {2, 1, 0, 0, 1},
     {4, 1, 0, 0, 1},   {2, 2, 0, 0, 1},   {4, 2, 0, 0, 1},
     {0, 1, 0, 0, 0, 1, 3, 8},  {0, 1, 0, 0, 0, 1, 3, 16}, {0, 1, 0, 0, 0, 2, 3, 16},
-    {0, 1, 0, 0, 0, 2, 4, 16}, {0, 1, 0, 0, 0, 2, 6, 8}};
+    {0, 1, 0, 0, 0, 2, 4, 16}, {0, 1, 0, 0, 0, 2, 6, 8},
+    {0, 1, 16, 0, 0}, {0, 1, 32, 0, 0}};
 static const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 extern "C" int fd23_num_tiles(void) { return kNumTiles; }
@@ -1146,8 +1147,74 @@ static void launch_fused_t(fd23_ctx *c, int k, int itrecord, bool record, unsign
                        m, spa, spb, vpa, vpb, spr, vpr, g, w, ev, c->scl, amax);
 }
 
+// tile-parallel fused step (block rows 0 in the tile table): segments of
+// 4*CZ rows, FP32/FP64 storage only
+template <class S, int CZ> static void launch_fused_tile_t(fd23_ctx *c, int k, int itrecord, bool record) {
+    using C = typename Store<S>::C;
+    const Grid g = make_grid(c);
+    // strips x segments; the interior rectangle holds the tiles whose stress
+    // cells (the strip's 64 columns, the segment's rows and 4 on each side)
+    // are all at full order with no C-PML and no free surface
+    const int OZ = 4 * CZ;
+    Work w;
+    w.ntx = (c->d.nx + FO - 1) / FO;
+    w.ntz = (c->d.nz + OZ - 1) / OZ;
+    w.ngroups = c->d.nshots;
+    const int lo = std::max(c->d.npml, 4), ztop = c->d.topabsorb ? lo : 4;
+    w.ix0 = (lo + HALO + FO - 1) / FO; // x0 - 4 >= lo
+    w.ix1 = c->d.nx - lo - HALO - FO >= 0 ? (c->d.nx - lo - HALO - FO) / FO + 1 : 0; // x0 + 60 <= nx - lo
+    w.iz0 = (ztop + HALO + OZ - 1) / OZ;          // z0 - 4 >= ztop
+    w.iz1 = c->d.nz - lo - HALO - OZ >= 0 ? (c->d.nz - lo - HALO - OZ) / OZ + 1 : 0; // z0 + OZ + 4 <= nz - lo
+    if (w.ix1 <= w.ix0 || w.iz1 <= w.iz0) w.ix0 = w.ix1 = w.iz0 = w.iz1 = 0;
+    const int inner = (w.ix1 - w.ix0) * (w.iz1 - w.iz0);
+    FusedEvents ev{};
+    ev.clock = c->clock;
+    ev.k = k;
+    ev.itrecord = itrecord;
+    ev.tap_start = c->d_ftap_start;
+    ev.taps = c->d_ftaps;
+    ev.src = c->d_src;
+    ev.stf = c->d_stf;
+    ev.rec_start = c->d_frec_start;
+    ev.recs = c->d_frecs;
+    ev.seis = c->seis;
+    ev.max_frames = (c->seis && record) ? c->d.max_frames : 0;
+    ev.frame_size = (long)c->d.nshots * c->ncomp * c->d.max_rec;
+    auto fields = [](void *const *f) {
+        return PsvFields<S>{(S *)f[0], (S *)f[1], (S *)f[2], (S *)f[3], (S *)f[4]};
+    };
+    PsvMat5<C> m{(const C *)c->mat[0], (const C *)c->mat[1], (const C *)c->mat[2], (const C *)c->mat[3],
+                 (const C *)c->mat[4]};
+    PsvStressPsi<S> spa{(S *)c->psi[2], (S *)c->psi[3], (S *)c->psi[6], (S *)c->psi[7]};
+    PsvStressPsi<S> spb{(S *)c->alt_psi[2], (S *)c->alt_psi[3], (S *)c->alt_psi[6], (S *)c->alt_psi[7]};
+    PsvVelPsi<S> vpa{(S *)c->psi[0], (S *)c->psi[1], (S *)c->psi[5], (S *)c->psi[4]};
+    PsvVelPsi<S> vpb{(S *)c->alt_psi[0], (S *)c->alt_psi[1], (S *)c->alt_psi[5], (S *)c->alt_psi[4]};
+    PsvStressProf<C> spr{prof_of<C>(c, 2), prof_of<C>(c, 3), prof_of<C>(c, 6), prof_of<C>(c, 7)};
+    PsvVelProf<C> vpr{prof_of<C>(c, 0), prof_of<C>(c, 1), prof_of<C>(c, 5), prof_of<C>(c, 4)};
+    Work wi = w, we = w;
+    wi.ntiles = inner;
+    wi.nwork = inner * w.ngroups;
+    we.ntiles = w.ntx * w.ntz - inner;
+    we.nwork = we.ntiles * w.ngroups;
+    if (wi.nwork > 0)
+        hipLaunchKernelGGL((k_psv_fused_tile<S, CZ, REGION_INTERIOR>), dim3(round_up(wi.nwork, 8)), dim3(FW, 4), 0,
+                           c->stream, fields(c->fields), fields(c->alt_fields), m, spa, spb, vpa, vpb, spr, vpr, g,
+                           wi, ev);
+    if (we.nwork > 0)
+        hipLaunchKernelGGL((k_psv_fused_tile<S, CZ, REGION_EDGE>), dim3(round_up(we.nwork, 8)), dim3(FW, 4), 0,
+                           c->stream, fields(c->fields), fields(c->alt_fields), m, spa, spb, vpa, vpb, spr, vpr, g,
+                           we, ev);
+}
+
 template <class S> static void launch_fused_h(fd23_ctx *c, int k, int itrecord, bool record, unsigned *amax) {
     const TileCfg t = kTiles[c->tile];
+    if (t.fr == 0) {
+        if constexpr (!Store<S>::scaled) {
+            if (t.fh == 16) launch_fused_tile_t<S, 4>(c, k, itrecord, record);
+            else launch_fused_tile_t<S, 8>(c, k, itrecord, record);
+        }
+        return;
+    }
     if (t.fr == 2) launch_fused_t<S, 64, 2>(c, k, itrecord, record, amax);
     else if (t.fh == 32) launch_fused_t<S, 32, 4>(c, k, itrecord, record, amax);
     else if (t.fh == 64) launch_fused_t<S, 64, 4>(c, k, itrecord, record, amax);
@@ -1738,6 +1805,8 @@ extern "C" int fd23_set_tile(fd23_ctx *c, int tile) {
     if (tile_fused(tile) && !is_psv(c)) return fail(FD23_EINVAL, "the fused step is P-SV only");
     if (tile_block(tile) && (is_psv(c) || scaled_storage(c)))
         return fail(FD23_EINVAL, "the block step is SH with FP32 or FP64 storage only");
+    if (tile_fused(tile) && kTiles[tile].fr == 0 && scaled_storage(c))
+        return fail(FD23_EINVAL, "the tile-parallel fused step keeps FP32 or FP64 storage");
     if (c->tile != tile) {
         c->tile = tile;
         fd23_ctx_invalidate_graphs(c);
@@ -1869,6 +1938,7 @@ extern "C" int fd23_autotune(fd23_ctx *c, int steps, double *best_ms) {
         if (kTiles[t].split && (!is_psv(c) || c->punch)) continue;
         if (tile_fused(t) && (!is_psv(c) || c->box || c->punch)) continue;
         if (tile_block(t) && (is_psv(c) || scaled_storage(c) || c->punch)) continue;
+        if (tile_fused(t) && kTiles[t].fr == 0 && scaled_storage(c)) continue;
         c->tile = t;
         double a, b;
         fd23_time_passes(c, 1, &a, &b); // warm-up

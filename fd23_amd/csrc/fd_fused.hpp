@@ -372,4 +372,232 @@ __global__ __launch_bounds__(FW) void k_psv_fused(PsvFields<S> fa, PsvFields<S> 
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Tile-parallel fused P-SV step (FP32/FP64 storage).  The same strips of
+// 64 stress columns (56 owned) and segments of OZ = 4*CZ rows as
+// k_psv_fused, but a workgroup of four wavefronts takes one (segment,
+// strip, shot) as a 2D tile with every global load issued before its first
+// barrier, as the two-pass kernels do, instead of marching one wavefront
+// down the segment:
+//  - U and W of rows z0-8 .. z0+OZ+7 and columns x0-8 .. x0+63 are staged
+//    in LDS;
+//  - wavefront ty computes the new stresses of rows z0-4+ty*(CZ+2) ..
+//    (CZ+2 rows, 64 lanes: the owned cells and the 4-cell ring the owned
+//    velocities read), stores the owned ones, and puts all of them in the
+//    same LDS space once every wavefront has read U and W;
+//  - it then computes the velocities of rows z0+ty*CZ .. (CZ rows, lanes
+//    4-59) from LDS.
+// Per-cell arithmetic is k_psv_fused's (psv_stress_* / psv_velocity_*), so
+// results are bitwise equal to the two-pass kernels.  Tiles (strips x
+// segments, one shot group per shot) are mapped by map_tile as the
+// two-pass kernels': REGION_INTERIOR compiles the interior path only, for
+// the rectangle of tiles whose stress cells are all interior, REGION_EDGE
+// the exact path for the frame around it.
+// ---------------------------------------------------------------------------
+template <class S, int CZ, int REGION>
+__global__ __launch_bounds__(FW * 4) void k_psv_fused_tile(PsvFields<S> fa, PsvFields<S> fb,
+                                                           PsvMat5<typename Store<S>::C> m, PsvStressPsi<S> spa,
+                                                           PsvStressPsi<S> spb, PsvVelPsi<S> vpa, PsvVelPsi<S> vpb,
+                                                           PsvStressProf<typename Store<S>::C> spr,
+                                                           PsvVelProf<typename Store<S>::C> vpr, Grid g, Work w,
+                                                           FusedEvents ev) {
+    using C = typename Store<S>::C;
+    using ST = Store<S>;
+    static_assert(!ST::scaled, "the tile-parallel fused step keeps FP32 or FP64 storage");
+    constexpr int NW = 4;
+    constexpr int OZ = NW * CZ;             // owned rows
+    constexpr int SR = CZ + 2;              // stress rows per wavefront
+    constexpr int LWU = FW + 2 * HALO;      // staged U/W columns
+    constexpr int NU = (OZ + 4 * HALO) * LWU;
+    constexpr int NT = (OZ + 2 * HALO) * FW;
+    constexpr int NL = 2 * NU > 3 * NT ? 2 * NU : 3 * NT;
+    __shared__ C lds[NL];
+    C *sU = lds, *sW = lds + NU;                            // rows z0-8 .., columns x0-8 ..
+    C *sXX = lds, *sZZ = lds + NT, *sXZ = lds + 2 * NT;     // rows z0-4 .., lanes
+
+    TileId t;
+    if (!map_tile<OZ, REGION>(w, t)) return;
+    const int S_ = g.nshots;
+    const int shot = t.group;
+    const int strip = t.tile % w.ntx;
+    const int seg = t.tile / w.ntx;
+    const int lane = threadIdx.x, ty = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+    const int tid = ty * FW + lane;
+    const int nx = g.nx, nz = g.nz, px = g.px;
+    const int x0 = strip * FO;
+    const int xc = x0 - HALO + lane;
+    const bool xin = xc >= 0 && xc < nx;
+    const bool own_x = lane >= HALO && lane < FW - HALO && xin;
+    const int z0 = seg * OZ;
+    const long base = (long)shot * g.sstride;
+    const int lo = max(g.npml, 4);
+    const int ztop = g.top ? lo : 4;
+    const bool xint = x0 - HALO >= lo && x0 + FO + HALO <= nx - lo;
+
+    // rows outside the domain read the zero pad rows (clamped), columns left
+    // of it the pad columns; columns right of it may read the next row, used
+    // only by lanes outside the domain, whose results are dropped
+    auto rowc = [&](int row) { return min(max(row, -HALO), nz + HALO); };
+    const int xcol = lane < 4 ? x0 - 2 * HALO + lane : x0 + FO + HALO + (lane - 4); // lanes 0-7: the 8 extra columns
+
+    // ---- every global load of the tile
+    constexpr int UR = (OZ + 4 * HALO) / NW; // staged rows per wavefront
+    C u_[UR], w_[UR], ue[UR], we[UR];
+#pragma unroll
+    for (int i = 0; i < UR; i++) {
+        const long r = base + (long)rowc(z0 - 2 * HALO + ty * UR + i) * px;
+        u_[i] = ST::ld(fa.U, r + xc, 1.f);
+        w_[i] = ST::ld(fa.W, r + xc, 1.f);
+        ue[i] = lane < 8 ? ST::ld(fa.U, r + xcol, 1.f) : C(0);
+        we[i] = lane < 8 ? ST::ld(fa.W, r + xcol, 1.f) : C(0);
+    }
+    C oxx[SR], ozz[SR], oxz[SR], lam[SR], mu[SR], mua[SR];
+#pragma unroll
+    for (int i = 0; i < SR; i++) {
+        const int rs = z0 - HALO + ty * SR + i;
+        const long r = (long)rowc(rs) * px + xc;
+        oxx[i] = ST::ld(fa.Txx, base + r, 1.f);
+        ozz[i] = ST::ld(fa.Tzz, base + r, 1.f);
+        oxz[i] = ST::ld(fa.Txz, base + r, 1.f);
+        lam[i] = m.LAM[r];
+        mu[i] = m.MU[r];
+        mua[i] = m.MUA[r];
+    }
+    C bu[CZ], bw[CZ];
+#pragma unroll
+    for (int i = 0; i < CZ; i++) {
+        const long r = (long)rowc(z0 + ty * CZ + i) * px + xc;
+        bu[i] = m.BU[r];
+        bw[i] = m.BW[r];
+    }
+    const int it = ev.clock[0] + ev.k;
+    double sv = 0.0;
+    const bool src_on = src_sample(ev.src[shot], ev.stf, it, sv);
+    const int t0 = ev.tap_start[strip * S_ + shot], t1 = ev.tap_start[strip * S_ + shot + 1];
+    const int rkey = (seg * w.ntx + strip) * S_ + shot;
+    const int r0 = ev.rec_start[rkey], r1 = ev.rec_start[rkey + 1];
+    const bool rec_step = (it % ev.itrecord) == 0 && it / ev.itrecord < ev.max_frames;
+
+#pragma unroll
+    for (int i = 0; i < UR; i++) {
+        const int row = (ty * UR + i) * LWU;
+        sU[row + lane + HALO] = u_[i];
+        sW[row + lane + HALO] = w_[i];
+        if (lane < 8) {
+            const int o = lane < 4 ? lane : FW + lane;
+            sU[row + o] = ue[i];
+            sW[row + o] = we[i];
+        }
+    }
+    __syncthreads();
+
+    const PsiScl4 one{{1.f, 1.f, 1.f, 1.f}, {1.f, 1.f, 1.f, 1.f}};
+    float mp[4] = {0.f, 0.f, 0.f, 0.f};
+    const C k0 = C(1225. / 1024.), k1 = C(245. / 3072.), k2 = C(49. / 5120.), k3 = C(5. / 7168.);
+
+    // ---- new stresses of rows z0-4+ty*SR .. (all 64 lanes)
+#pragma unroll
+    for (int i = 0; i < SR; i++) {
+        const int rs = z0 - HALO + ty * SR + i;
+        const int ru = rs - z0 + 2 * HALO; // staged row
+        const C *pu = &sU[ru * LWU + lane + HALO], *pw = &sW[ru * LWU + lane + HALO];
+        C a = oxx[i], bb = ozz[i], cc = oxz[i];
+        if (REGION == REGION_INTERIOR || (rs >= 0 && rs < nz && xin)) {
+            if (REGION == REGION_INTERIOR || (REGION == REGION_ALL && xint && rs >= ztop && rs < nz - lo)) {
+                const C ux = dm(pu, 1, k0, k1, k2, k3), wx = dp(pw, 1, k0, k1, k2, k3);
+                const C uz = dm(pu, LWU, k0, k1, k2, k3), wz = dp(pw, LWU, k0, k1, k2, k3);
+                psv_stress_interior(ux, uz, wx, wz, lam[i], mu[i], mua[i], a, bb, cc);
+            } else {
+                C c0, c1, c2, c3;
+                Coef<C>::get(order_at(xc, rs, nx, nz), c0, c1, c2, c3);
+                const C ux = dm(pu, 1, c0, c1, c2, c3), wx = dp(pw, 1, c0, c1, c2, c3);
+                const C uz = dm(pu, LWU, c0, c1, c2, c3), wz = dp(pw, LWU, c0, c1, c2, c3);
+                const bool own = own_x && rs >= z0 && rs < z0 + OZ;
+                psv_stress_exact<S, C>(g, xc, rs, shot, ux, uz, wx, wz, lam[i], mu[i], mua[i], a, bb, cc, spa, spb,
+                                       own, spr, one, mp);
+            }
+            // source taps after the cell's update, in the reference's order
+            if (src_on) {
+                for (int e = t0; e < t1; e++) {
+                    const TapF tp = ev.taps[e];
+                    if (tp.row != rs || tp.lane != lane) continue;
+                    if (tp.field == 2) a = apply_tap(a, tp, sv);
+                    else if (tp.field == 3) bb = apply_tap(bb, tp, sv);
+                    else cc = apply_tap(cc, tp, sv);
+                }
+            }
+            if (own_x && rs >= z0 && rs < z0 + OZ) {
+                const long q = base + (long)rs * px + xc;
+                ST::st(fb.Txx, q, a, 1.f);
+                ST::st(fb.Tzz, q, bb, 1.f);
+                ST::st(fb.Txz, q, cc, 1.f);
+            }
+        } else {
+            a = bb = cc = C(0); // outside the domain the new stresses are zero
+        }
+        oxx[i] = a;
+        ozz[i] = bb;
+        oxz[i] = cc;
+    }
+    // the owned cells' U and W, read before the LDS space takes the stresses
+    C u[CZ], ww[CZ];
+#pragma unroll
+    for (int i = 0; i < CZ; i++) {
+        const int ru = ty * CZ + i + 2 * HALO;
+        u[i] = sU[ru * LWU + lane + HALO];
+        ww[i] = sW[ru * LWU + lane + HALO];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < SR; i++) {
+        const int rt = (ty * SR + i) * FW + lane;
+        sXX[rt] = oxx[i];
+        sZZ[rt] = ozz[i];
+        sXZ[rt] = oxz[i];
+    }
+    __syncthreads();
+
+    // ---- velocities of rows z0+ty*CZ .. (lanes 4-59)
+#pragma unroll
+    for (int i = 0; i < CZ; i++) {
+        const int rv = z0 + ty * CZ + i;
+        if (!own_x || rv >= nz) continue;
+        const int o = (ty * CZ + i + HALO) * FW + lane;
+        if (REGION == REGION_INTERIOR || (REGION == REGION_ALL && xint && rv >= ztop && rv < nz - lo)) {
+            const C dxTxx = dp(&sXX[o], 1, k0, k1, k2, k3), dxTxz = dm(&sXZ[o], 1, k0, k1, k2, k3);
+            const C dzTzz = dm(&sZZ[o], FW, k0, k1, k2, k3), dzTxz = dp(&sXZ[o], FW, k0, k1, k2, k3);
+            psv_velocity_interior(dxTxx, dzTzz, dxTxz, dzTxz, bu[i], bw[i], u[i], ww[i]);
+        } else {
+            C c0, c1, c2, c3;
+            Coef<C>::get(order_at(xc, rv, nx, nz), c0, c1, c2, c3);
+            const C dxTxx = dp(&sXX[o], 1, c0, c1, c2, c3), dxTxz = dm(&sXZ[o], 1, c0, c1, c2, c3);
+            const C dzTzz = dm(&sZZ[o], FW, c0, c1, c2, c3), dzTxz = dp(&sXZ[o], FW, c0, c1, c2, c3);
+            psv_velocity_exact<S, C>(g, xc, rv, shot, dxTxx, dzTzz, dxTxz, dzTxz, bu[i], bw[i], u[i], ww[i], vpa,
+                                     vpb, true, vpr, one, mp);
+        }
+        const long q = base + (long)rv * px + xc;
+        ST::st(fb.U, q, u[i], 1.f);
+        ST::st(fb.W, q, ww[i], 1.f);
+    }
+
+    // ---- receivers of the tile: park U and W in LDS, one entry per thread
+    if (rec_step && r1 > r0) {
+        __syncthreads();
+        C *pU = lds, *pW = lds + OZ * FW;
+#pragma unroll
+        for (int i = 0; i < CZ; i++) {
+            pU[(ty * CZ + i) * FW + lane] = u[i];
+            pW[(ty * CZ + i) * FW + lane] = ww[i];
+        }
+        __syncthreads();
+        C *seis = (C *)ev.seis + (long)(it / ev.itrecord) * ev.frame_size;
+        for (int e = r0 + tid; e < r1; e += FW * NW) {
+            const RecF rc = ev.recs[e];
+            const int c = (rc.row - z0) * FW + rc.lane;
+            seis[rc.out] = rc.comp == 0 ? pU[c] : pW[c];
+        }
+    }
+}
+
 } // namespace fd23

@@ -192,7 +192,12 @@ def test_all_tiles_bitwise_equal():
         assert st == ref
 
 
-FUSED = (10, 11, 12, 13)  # fused P-SV step: segments of 32/64/128 rows in blocks of 4, 64 rows in 2
+def _fused_tiles():
+    """Fused P-SV shapes: the strip-marching step (segments of 32/64/128 rows
+    in blocks of 4, 64 rows in 2) and the tile-parallel step (segments of
+    16 and 32 rows, block rows 0)."""
+    lib = _lib.load()
+    return [t for t in range(lib.fd23_num_tiles()) if lib.fd23_tile_kind(t) == _lib.TILE_FUSED]
 
 
 @pytest.mark.parametrize("nx,nz,npml,top,kind,storage", [
@@ -231,7 +236,7 @@ def test_fused_step_bitwise_equals_two_pass(nx, nz, npml, top, kind, storage):
 
     ref_st, ref_seis = run(0, 3)
     assert any(np.abs(x).max() > 0 for x in ref_seis)
-    for t in FUSED:
+    for t in _fused_tiles():
         st, seis = run(t, 3)
         _print("fused tile", t, st == ref_st)
         assert st == ref_st
