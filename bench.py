@@ -102,7 +102,7 @@ def run_gpu(wl, steps, warmup, autotune=True, tile=None, passes=True):
     if tile is not None:
         ctx.tile = tile
     elif autotune:
-        _, tune_ms = ctx.autotune(4)
+        _, tune_ms = ctx.autotune(16)
     tile = ctx.tile
     # the graphs of both runs are captured before any step runs
     ctx.prepare(warmup)
