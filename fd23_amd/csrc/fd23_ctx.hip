@@ -854,7 +854,7 @@ static int build_block_events(fd23_ctx *c) {
     std::vector<TapT> taps;
     std::vector<RecT> recs;
     for (size_t k = 0; k < nkey; k++) {
-        // the kernel holds a list in LDS
+        // the kernel holds a list in the lanes of a wavefront
         if (tb[k].size() > (size_t)kBlockTaps)
             return fail(FD23_EINVAL, "block step: %zu source taps in one tile (at most %d)", tb[k].size(),
                         kBlockTaps);

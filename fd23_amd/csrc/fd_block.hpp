@@ -47,7 +47,24 @@ struct BlockWork {
 };
 
 constexpr int kBlockReach = 7; // cells one SH step moves information
-constexpr int kBlockTaps = 32;  // most source taps one (tile, shot) list may hold
+constexpr int kBlockTaps = 32;  // most source taps one (tile, shot) list may hold (lanes of a wavefront)
+
+// tap e of the list whose entry i lane i of the wavefront holds (e uniform)
+__device__ __forceinline__ TapT lane_tap(const TapT &t, int e) {
+    auto rl64 = [e](double d) {
+        const unsigned long long b = __builtin_bit_cast(unsigned long long, d);
+        const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, e);
+        const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), e);
+        return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+    };
+    TapT r;
+    r.cell = __builtin_amdgcn_readlane(t.cell, e);
+    r.field = __builtin_amdgcn_readlane(t.field, e);
+    r.mode = __builtin_amdgcn_readlane(t.mode, e);
+    r.m = rl64(t.m);
+    r.q = rl64(t.q);
+    return r;
+}
 
 template <int K, int CZ, int NW> struct BlockShape {
     static constexpr int H = kBlockReach * K; // halo
@@ -68,7 +85,6 @@ __global__ __launch_bounds__(TX *NW) void k_sh_block(ShFields<S> fa, ShFields<S>
     constexpr int H = B::H, LW = B::LW, LH = B::LH;
     __shared__ C sV[LH * LW], sX[LH * LW], sZ[LH * LW];
     __shared__ C sZP[6][B::RH]; // z profiles of the region's rows: V_z b c k, Tz_z b c k
-    __shared__ TapT sTap[kBlockTaps];
 
     // tiles of one shot are consecutive, and consecutive logical workgroups
     // run on one XCD, so neighbours read each other's halos through one L2
@@ -140,9 +156,11 @@ __global__ __launch_bounds__(TX *NW) void k_sh_block(ShFields<S> fa, ShFields<S>
         pVz[r] = pa.V_z[zi];
     }
 
-    // Level 2: the region's source taps (list entry min(tid, n-1); the lists
-    // end with one spare entry, so an empty list reads that), the source
-    // samples of the launch's steps, the thread's first receiver.
+    // Level 2: the region's source taps (lane i of every wavefront holds list
+    // entry min(i, n-1), read back with readlane; the lists end with one
+    // spare entry, so an empty list reads that), the source samples of the
+    // launch's steps, the thread's first receiver.  Nothing waits for them
+    // before the first stress half needs the taps.
     const int ntap = e1 - e0;
     TapT tap{};
     double sval[K];
@@ -155,7 +173,7 @@ __global__ __launch_bounds__(TX *NW) void k_sh_block(ShFields<S> fa, ShFields<S>
         sval[j] = 0;
     }
     if (ev.clock) {
-        tap = ev.taps[e0 + min(tid, max(ntap - 1, 0))];
+        tap = ev.taps[e0 + min(tx, max(ntap - 1, 0))]; // lane i: entry i of the list
         rec0 = ev.recs[min(re0, max(r1 - 1, r0))];
 #pragma unroll
         for (int j = 0; j < K; j++) {
@@ -180,7 +198,7 @@ __global__ __launch_bounds__(TX *NW) void k_sh_block(ShFields<S> fa, ShFields<S>
     }
 
     // LDS: cells outside the region 0 (the rows above and below, the
-    // columns left and right), the region's V, the z profiles, the taps
+    // columns left and right), the region's V, the z profiles
     for (int i = tid; i < LH * LW; i += TX * NW) {
         const int r = i / LW, col = i - r * LW;
         if (r < HALO || r >= LH - HALO || col < HALO || col >= LW - HALO) {
@@ -196,7 +214,6 @@ __global__ __launch_bounds__(TX *NW) void k_sh_block(ShFields<S> fa, ShFields<S>
 #pragma unroll
         for (int q = 0; q < 6; q++) sZP[q][tid] = zin ? zp[q] : C(0);
     }
-    if (tid < ntap) sTap[tid] = tap;
     __syncthreads();
 
     // grid row 0 inside the region: the velocity half of each step first
@@ -266,7 +283,7 @@ __global__ __launch_bounds__(TX *NW) void k_sh_block(ShFields<S> fa, ShFields<S>
         }
         // stress taps of the source (Tx, Tz), after the cell's update
         for (int e = 0; e < nt_; e++) {
-            const TapT tp = sTap[e];
+            const TapT tp = lane_tap(tap, e);
             const int lx = tp.cell % TX, lz = tp.cell / TX;
             if (tp.field == 0 || lx != tx || lz < lz0 || lz >= lz0 + CZ) continue;
 #pragma unroll
@@ -296,7 +313,7 @@ __global__ __launch_bounds__(TX *NW) void k_sh_block(ShFields<S> fa, ShFields<S>
 
         // point force on V, before the cell's velocity update
         for (int e = 0; e < nt_; e++) {
-            const TapT tp = sTap[e];
+            const TapT tp = lane_tap(tap, e);
             const int lx = tp.cell % TX, lz = tp.cell / TX;
             if (tp.field != 0 || lx != tx || lz < lz0 || lz >= lz0 + CZ) continue;
 #pragma unroll
