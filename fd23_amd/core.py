@@ -230,7 +230,11 @@ class Context:
 
     def tile_shape(self, tile=None):
         """(rz, spb, seg_rows, block_rows) of a tile index (default: the
-        current one); seg_rows and block_rows are 0 for tiled kinds."""
+        current one): seg_rows and block_rows are 0 for tiled kinds; the
+        fused P-SV step has segments of seg_rows rows in blocks of
+        block_rows (0: the tile-parallel step); the SH block step runs
+        seg_rows steps per launch over regions of block_rows rows, rz rows
+        per thread."""
         t = self.tile if tile is None else tile
         v = [ctypes.c_int() for _ in range(4)]
         check(self.lib.fd23_tile_shape(t, *(ctypes.byref(x) for x in v)))

@@ -261,15 +261,17 @@ int fd23_num_tiles(void);
  * tiles of 64 columns x 4*rz rows (rz rows per thread, spb shots per
  * workgroup) staged through LDS; FD23_TILE_FUSED, the fused P-SV step
  * (stress and velocity in one pass over two buffer sets, segments of
- * seg_rows rows in blocks of block_rows; 0 for tiled kinds);
+ * seg_rows rows in blocks of block_rows, or block_rows 0: the
+ * tile-parallel step, FP32/FP64 storage only; 0 for tiled kinds);
  * FD23_TILE_BLOCK, the SH block step (seg_rows = K steps per launch over
- * regions of 64 columns x 4*rz rows, each region a tile plus a 7*K-cell
- * halo that the launch recomputes, over two buffer sets).  The fused step
- * applies to fd23_run of P-SV contexts without a box, the block step to
- * fd23_run of SH contexts with FP32/FP64 storage, both without a
- * first-arrival window; the staged entry points, the box and the window
- * run the tiled kernels (tile 0's shape).  No reference counterpart: the
- * reference launches two kernels per step (SH/main.cpp:93-141).         */
+ * regions of 64 columns x block_rows rows, rz rows per thread, each
+ * region a tile plus a 7*K-cell halo that the launch recomputes, over two
+ * buffer sets).  The fused step applies to fd23_run of P-SV contexts
+ * without a box, the block step to fd23_run of SH contexts with FP32/FP64
+ * storage, both without a first-arrival window; the staged entry points,
+ * the box and the window run the tiled kernels (tile 0's shape).  No
+ * reference counterpart: the reference launches two kernels per step
+ * (SH/main.cpp:93-141).                                                 */
 enum { FD23_TILE_TILED = 0, FD23_TILE_FUSED = 1, FD23_TILE_BLOCK = 2 };
 int fd23_tile_shape(int tile, int *rz, int *spb, int *seg_rows, int *block_rows);
 /* FD23_TILE_* of a tile index, negative for a bad index                 */

@@ -320,6 +320,9 @@ def test_sh_block_step_vs_oracle():
 
 
 def test_block_tiles_refused_where_they_do_not_apply():
+    """SH block shapes on P-SV or FP16 contexts, and the tile-parallel
+    fused shapes (block rows 0) on FP16 contexts, are refused with an
+    error naming the path; the staged entry points and tile 0 still run."""
     p = Problem(_lib.WAVE_PSV, 64, 32, 4, 10.0, 7e-4, 4, seed=1)
     q = Problem(_lib.WAVE_SH, 64, 32, 4, 10.0, 7e-4, 4, seed=1)
     t = _block_tiles()[0]
@@ -327,6 +330,13 @@ def test_block_tiles_refused_where_they_do_not_apply():
         ctx.tile = t
     with q.context(storage=_lib.STORE_F16) as ctx, pytest.raises(Exception, match="block step"):
         ctx.tile = t
+    with p.context(storage=_lib.STORE_F16) as ctx:
+        ftile = [f for f in _fused_tiles() if ctx.tile_shape(f)[3] == 0]
+        assert ftile
+        with pytest.raises(Exception, match="tile-parallel fused step"):
+            ctx.tile = ftile[0]
+        ctx.run(0, p.nt, 1)
+        assert np.isfinite(ctx.seismograms(0, 0, p.nt)).all()
 
 
 def test_fused_step_f16_close_to_two_pass():
