@@ -135,6 +135,25 @@ def test_open_window_equals_no_window_bitwise(wave, tile):
         assert np.array_equal(fa[0][n], fb[0][n]), n
 
 
+def test_sh_block_shapes_under_a_window_run_the_two_pass_kernels():
+    """A block shape set on a context with a first-arrival window runs tile
+    0's two-pass kernels (the block step has no window): bitwise equal to
+    tile 0 under the same window, and to the oracle."""
+    _print("case: sh block shapes under a window")
+    lib = _lib.load()
+    blocks = [t for t in range(lib.fd23_num_tiles()) if lib.fd23_tile_kind(t) == _lib.TILE_BLOCK]
+    p = Problem(_lib.WAVE_SH, 128, 64, 10, 10.0, 7e-4, 300, seed=13, kind=_lib.SRC_FORCE,
+                sources=[(40, 12)], rec_z=2)
+    w = _window(p)
+    ref, rf = _run(p, w, 0)
+    _check(p, ref, rf, w, 1e-5)
+    for t in blocks:
+        seis, fields = _run(p, w, t)
+        assert np.array_equal(seis[0][0], ref[0][0]), t
+        for n in fields[0]:
+            assert np.array_equal(fields[0][n], rf[0][n]), (t, n)
+
+
 def test_window_refusals():
     _print("case: punch refusals")
     p = Problem(_lib.WAVE_PSV, 128, 64, 10, 10.0, 7e-4, 10, seed=16, rec_z=3)
