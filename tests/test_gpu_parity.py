@@ -1,7 +1,7 @@
 """HIP path vs the oracle and vs the reference's own checkpoint fixtures.
 
 All calls go through libfd23.so's C ABI.  Tolerances: 1e-5 relative L2 on
-seismograms and fields for FP32 (the bar the north-star sets), 1e-6 on
+seismograms and fields for FP32 (the stated parity bar), 1e-6 on
 the reference's step-0/1 checkpoints, 1e-10 for FP64 storage vs the FP64
 oracle.
 """

@@ -87,7 +87,7 @@ def _rel(a, b):
 
 
 def test_fp64_gpu_reference_equals_fp64_oracle():
-    _print("case: C4 size FP64 GPU vs FP64 oracle, shot 0, %d steps" % NT)
+    _print("case: 4096x2048 FP64 GPU vs FP64 oracle, shot 0, %d steps" % NT)
     g = _gpu(_lib.STORE_F64)
     err = _rel(g[0], _oracle_f64())
     _print("fp64 gpu vs oracle", err)
@@ -96,7 +96,7 @@ def test_fp64_gpu_reference_equals_fp64_oracle():
 
 @pytest.mark.parametrize("storage,fmt", [(_lib.STORE_F16, "f16"), (_lib.STORE_BF16, "bf16")])
 def test_half_storage_at_its_floor(storage, fmt):
-    _print("case: C4 size %s vs FP64, 8 shots, %d steps" % (fmt, NT))
+    _print("case: 4096x2048 %s vs FP64, 8 shots, %d steps" % (fmt, NT))
     ref = _gpu(_lib.STORE_F64)
     got = _gpu(storage)
     assert np.isfinite(got).all()
