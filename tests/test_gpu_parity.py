@@ -300,6 +300,32 @@ def test_sh_block_step_bitwise_equals_two_pass(nx, nz, npml, top, kind, storage)
             assert np.array_equal(a, b)
 
 
+def test_sh_block_step_restart_from_saved_state():
+    """A block-step run stopped after a chunk that leaves the state in the
+    second buffer set, saved, and continued in a new context from that
+    state equals the uninterrupted run bitwise (state and the seismograms
+    after the restart)."""
+    _print("case: sh block restart")
+    p = Problem(_lib.WAVE_SH, 150, 90, 12, 10.0, 7e-4, 80, seed=41, kind=_lib.SRC_FORCE,
+                sources=[(75, 8), (30, 20)], rec_z=2)
+    t = _block_tiles()[0]
+    with p.context() as a:
+        a.tile = t
+        a.run(0, 37, 1)
+        saved = [a.get_state(s) for s in range(2)]
+        a.run(37, 43, 1)
+        ref = [a.get_state(s) for s in range(2)]
+        ref_seis = [a.seismograms(s, 0, p.nt)[37:] for s in range(2)]
+    with p.context() as b:
+        b.tile = t
+        for s in range(2):
+            b.set_state(saved[s], s)
+        b.run(37, 43, 1)
+        assert [b.get_state(s) for s in range(2)] == ref
+        for s in range(2):
+            assert np.array_equal(b.seismograms(s, 0, p.nt)[37:], ref_seis[s])
+
+
 def test_sh_block_step_vs_oracle():
     """The block step on its own against the oracle (FP32, 1e-5)."""
     _print("case: sh block vs oracle")
