@@ -68,7 +68,7 @@ static const TileCfg kTiles[] = {
     {4, 1, 0, 0, 1},   {2, 2, 0, 0, 1},   {4, 2, 0, 0, 1},
     {0, 1, 0, 0, 0, 1, 3, 8},  {0, 1, 0, 0, 0, 1, 3, 16}, {0, 1, 0, 0, 0, 2, 3, 16},
     {0, 1, 0, 0, 0, 2, 4, 16}, {0, 1, 0, 0, 0, 2, 6, 8},
-    {0, 1, 16, 0, 0}, {0, 1, 32, 0, 0}};
+    {0, 1, 16, 0, 0}, {0, 1, 32, 0, 0}, {1, 1, 0, 0, 0}};
 static const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 extern "C" int fd23_num_tiles(void) { return kNumTiles; }
@@ -1093,7 +1093,7 @@ template <class S> static void launch_pass(fd23_ctx *c, bool stress, const Event
     TILE_CASE(4, 1, false) TILE_CASE(2, 1, false) TILE_CASE(8, 1, false) TILE_CASE(4, 2, false)
     TILE_CASE(4, 4, false) TILE_CASE(4, 8, false) TILE_CASE(8, 2, false) TILE_CASE(8, 4, false)
     TILE_CASE(2, 4, false) TILE_CASE(2, 8, false) TILE_CASE(2, 1, true) TILE_CASE(4, 1, true)
-    TILE_CASE(2, 2, true) TILE_CASE(4, 2, true)
+    TILE_CASE(2, 2, true) TILE_CASE(4, 2, true) TILE_CASE(1, 1, false)
 #undef TILE_CASE
 }
 
