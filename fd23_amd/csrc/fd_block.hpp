@@ -6,19 +6,19 @@
 // grid and loads the region around it, 64 columns x NW*CZ rows: the tile
 // plus a halo of H = 7*K cells on every side.  It then runs the K steps on
 // the region in LDS and registers, recomputing the halo, and writes only
-// the tile.  One
-// SH step moves information 7 cells (the stress at x reads V at x-3..x+4,
-// the velocity at x reads Tx at x-4..x+3, and the same in z), so after k
-// steps the cells within 7k of the region's border are stale while the
-// tile is exact.  Reads come from one buffer set and the tile is written
-// to the other (ping-pong): neighbouring workgroups read the same halo
-// cells in the launch.
+// the tile.  One SH step moves information 7 cells (the stress at x reads V
+// at x-3..x+4, the velocity at x reads Tx at x-4..x+3, and the same in z),
+// so after k steps the cells within 7k of the region's border are stale
+// while the tile is exact.  Reads come from one buffer set and the tile is
+// written to the other (ping-pong): neighbouring workgroups read the same
+// halo cells in the launch.
 //
 // Thread (tx, ty) holds column tx and rows ty*CZ .. ty*CZ+CZ-1 of the
-// region (few rows per thread and many wavefronts per workgroup, so that
-// each SIMD interleaves several wavefronts through the LDS latencies): V, Tx, Tz, the four C-PML memory variables and the three material
-// arrays of its cells in registers.  V, Tx and Tz are mirrored in LDS for the
-// neighbours' derivatives; LDS cells outside the region hold 0.  Cells
+// region -- few rows per thread and many wavefronts per workgroup, so that
+// each SIMD interleaves several wavefronts through the LDS latencies -- and
+// keeps V, Tx, Tz, the four C-PML memory variables and the three material
+// arrays of its cells in registers.  V, Tx and Tz are mirrored in LDS for
+// the neighbours' derivatives; LDS cells outside the region hold 0.  Cells
 // outside the grid hold 0 and are never updated, as the padding of the
 // arrays in HBM.
 //
