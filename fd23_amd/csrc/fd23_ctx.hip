@@ -68,7 +68,8 @@ static const TileCfg kTiles[] = {
     {4, 1, 0, 0, 1},   {2, 2, 0, 0, 1},   {4, 2, 0, 0, 1},
     {0, 1, 0, 0, 0, 1, 3, 8},  {0, 1, 0, 0, 0, 1, 3, 16}, {0, 1, 0, 0, 0, 2, 3, 16},
     {0, 1, 0, 0, 0, 2, 4, 16}, {0, 1, 0, 0, 0, 2, 6, 8},
-    {0, 1, 16, 0, 0}, {0, 1, 32, 0, 0}, {1, 1, 0, 0, 0}};
+    {0, 1, 16, 0, 0}, {0, 1, 32, 0, 0}, {1, 1, 0, 0, 0},
+    {0, 1, 0, 0, 0, 2, 4, 10}, {0, 1, 0, 0, 0, 2, 3, 12}};
 static const int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 
 extern "C" int fd23_num_tiles(void) { return kNumTiles; }
@@ -1274,6 +1275,8 @@ template <class S> static void launch_block_h(fd23_ctx *c, int k, int nsub, int 
     if (t.bk == 1 && t.bw == 8) launch_block_t<S, 1, 3, 8>(c, k, nsub, itrecord, events);
     else if (t.bk == 1) launch_block_t<S, 1, 3, 16>(c, k, nsub, itrecord, events);
     else if (t.bw == 8) launch_block_t<S, 2, 6, 8>(c, k, nsub, itrecord, events);
+    else if (t.bw == 10) launch_block_t<S, 2, 4, 10>(c, k, nsub, itrecord, events);
+    else if (t.bw == 12) launch_block_t<S, 2, 3, 12>(c, k, nsub, itrecord, events);
     else if (t.bcz == 3) launch_block_t<S, 2, 3, 16>(c, k, nsub, itrecord, events);
     else launch_block_t<S, 2, 4, 16>(c, k, nsub, itrecord, events);
 }
