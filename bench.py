@@ -323,7 +323,8 @@ def calibrate_copy(nbytes=1 << 30, reps=10):
     torch.cuda.synchronize()
     el = (time.perf_counter() - t0) / reps
     out = {"copy_bytes_each_way": nbytes, "reps": reps, "torch_copy_GBps": round(2 * nbytes / el / 1e9, 1)}
-    for grid in (1024, 4096, 16384, 65536):
+    # the last grid gives each thread one 16-byte load and store
+    for grid in (1024, 4096, 16384, 65536, nbytes // (16 * 256)):
         lib.fd23_device_copy(ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()), nbytes, grid, None)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -333,6 +334,27 @@ def calibrate_copy(nbytes=1 << 30, reps=10):
         torch.cuda.synchronize()
         el = (time.perf_counter() - t0) / reps
         out["fd23_copy_grid%d_GBps" % grid] = round(2 * nbytes / el / 1e9, 1)
+    del x, y
+    # the passes' access pattern: arrays of 4096 x 16384 FP32 (256 MiB each,
+    # above the 256 MB MALL) in one allocation, 64 x 16 tiles, timed with
+    # HIP events
+    nx, nz = 4096, 16384
+    arrs = torch.zeros((11, nx * nz), dtype=torch.float32, device="cuda")
+    ptr = [arrs[k].data_ptr() for k in range(11)]
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    # (7, 2): the velocity pass's pattern, (8, 3): the stress pass's
+    for nr, nw in ((1, 1), (3, 1), (7, 2), (8, 3)):
+        rd = (ctypes.c_void_p * nr)(*ptr[:nr])
+        wr = (ctypes.c_void_p * nw)(*ptr[8:8 + nw])
+        _lib.check(lib.fd23_device_stream_tiles(rd, nr, wr, nw, nx, nz, None))
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            _lib.check(lib.fd23_device_stream_tiles(rd, nr, wr, nw, nx, nz, None))
+        e1.record()
+        torch.cuda.synchronize()
+        el = e0.elapsed_time(e1) / 1e3 / reps
+        out["tiles_r%d_w%d_GBps" % (nr, nw)] = round((nr + nw) * nx * nz * 4 / el / 1e9, 1)
     print(json.dumps(out), flush=True)
 
 

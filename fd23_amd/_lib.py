@@ -107,6 +107,7 @@ SIGNATURES = {
     "fd23_autotune": (c_int, [c_vp, c_int, P_dbl]),
     "fd23_bytes_per_step": (c_dbl, [c_vp]),
     "fd23_device_copy": (c_int, [c_vp, c_vp, c_sz, c_int, c_vp]),
+    "fd23_device_stream_tiles": (c_int, [c_vp, c_int, c_vp, c_int, c_int, c_int, c_vp]),
     "fd23_time_passes": (c_int, [c_vp, c_int, P_dbl, P_dbl]),
     "fd23_cpml_profile": (c_int, [c_int, c_int, c_dbl, c_dbl, c_dbl, c_dbl, c_dbl, c_vp, c_vp, c_vp]),
     "fd23_cpml_pack": (c_int, [c_int, c_int, c_int, c_int, c_dbl, c_dbl, c_dbl, c_dbl, c_vp]),

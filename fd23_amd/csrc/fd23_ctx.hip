@@ -1999,3 +1999,76 @@ extern "C" int fd23_device_copy(void *dst, const void *src, size_t bytes, int gr
                        (const float4 *)src, bytes / 16);
     return check_launch();
 }
+
+// ---------------------------------------------------------------------------
+// The passes' memory pattern without their arithmetic: NR arrays read and
+// NW written, nx x nz FP32 each, in the passes' 64 x 16 tiles (64 x 4
+// threads, 4 rows each) and their XCD band order.  out_k = sum of the
+// reads + k.  The stress pass reads 8 arrays and writes 3, the velocity
+// pass reads 7 and writes 2.
+// ---------------------------------------------------------------------------
+struct StreamArrays {
+    const float *r[8];
+    float *w[3];
+};
+template <int NR, int NW>
+__global__ __launch_bounds__(TX *BY) void k_stream_tiles(StreamArrays a, int nx, int ntx, int ntiles) {
+    const int q = gridDim.x >> 3;
+    const int L = (int)(blockIdx.x & 7) * q + (int)(blockIdx.x >> 3);
+    if (L >= ntiles) return;
+    const int tz = L / ntx, tx = L - tz * ntx;
+    const int x = tx * TX + (int)threadIdx.x, z0 = tz * 16 + (int)threadIdx.y * 4;
+    float acc[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const size_t i = (size_t)(z0 + r) * nx + x;
+        float s = a.r[0][i];
+#pragma unroll
+        for (int k = 1; k < NR; ++k) s += a.r[k][i];
+        acc[r] = s;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const size_t i = (size_t)(z0 + r) * nx + x;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) a.w[k][i] = acc[r] + (float)k;
+    }
+}
+
+template <int NR>
+static void launch_stream_tiles(int nw, const StreamArrays &a, int nx, int ntx, int ntiles, hipStream_t s) {
+    const dim3 grid(round_up(ntiles, 8)), block(TX, BY);
+    if (nw == 1) hipLaunchKernelGGL((k_stream_tiles<NR, 1>), grid, block, 0, s, a, nx, ntx, ntiles);
+    else if (nw == 2) hipLaunchKernelGGL((k_stream_tiles<NR, 2>), grid, block, 0, s, a, nx, ntx, ntiles);
+    else hipLaunchKernelGGL((k_stream_tiles<NR, 3>), grid, block, 0, s, a, nx, ntx, ntiles);
+}
+
+extern "C" int fd23_device_stream_tiles(const float *const *reads, int nreads, float *const *writes, int nwrites,
+                                        int nx, int nz, void *stream) {
+    if (!reads || !writes || nreads < 1 || nreads > 8 || nwrites < 1 || nwrites > 3)
+        return fail(FD23_EINVAL, "tiled stream needs 1..8 arrays read and 1..3 written");
+    if (nx < TX || nz < 16 || nx % TX != 0 || nz % 16 != 0 || (long)(nx / TX) * (nz / 16) > (1L << 30))
+        return fail(FD23_EINVAL, "tiled stream needs nx a multiple of 64 and nz a multiple of 16");
+    StreamArrays a{};
+    for (int k = 0; k < nreads; ++k) {
+        if (!reads[k]) return fail(FD23_EINVAL, "tiled stream: null array");
+        a.r[k] = reads[k];
+    }
+    for (int k = 0; k < nwrites; ++k) {
+        if (!writes[k]) return fail(FD23_EINVAL, "tiled stream: null array");
+        a.w[k] = writes[k];
+    }
+    const int ntx = nx / TX, ntiles = ntx * (nz / 16);
+    const hipStream_t s = (hipStream_t)stream;
+    switch (nreads) {
+    case 1: launch_stream_tiles<1>(nwrites, a, nx, ntx, ntiles, s); break;
+    case 2: launch_stream_tiles<2>(nwrites, a, nx, ntx, ntiles, s); break;
+    case 3: launch_stream_tiles<3>(nwrites, a, nx, ntx, ntiles, s); break;
+    case 4: launch_stream_tiles<4>(nwrites, a, nx, ntx, ntiles, s); break;
+    case 5: launch_stream_tiles<5>(nwrites, a, nx, ntx, ntiles, s); break;
+    case 6: launch_stream_tiles<6>(nwrites, a, nx, ntx, ntiles, s); break;
+    case 7: launch_stream_tiles<7>(nwrites, a, nx, ntx, ntiles, s); break;
+    default: launch_stream_tiles<8>(nwrites, a, nx, ntx, ntiles, s); break;
+    }
+    return check_launch();
+}

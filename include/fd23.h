@@ -298,6 +298,15 @@ double fd23_bytes_per_step(const fd23_ctx *ctx);
  * are measured against, and the copy that scales the memory counters.
  * No reference counterpart.                                             */
 int fd23_device_copy(void *dst, const void *src, size_t bytes, int grid, void *stream);
+/* The passes' memory pattern without their arithmetic, enqueued on
+ * `stream`: nreads (1..8) FP32 arrays read and nwrites (1..3) written,
+ * nx x nz each (nx a multiple of 64, nz of 16), in the passes' 64 x 16
+ * tiles and XCD band order; writes[k] = sum of the reads + k.  The
+ * bandwidth a pass's access pattern sustains (the stress pass reads 8
+ * arrays and writes 3, the velocity pass reads 7 and writes 2).  No
+ * reference counterpart.                                                */
+int fd23_device_stream_tiles(const float *const *reads, int nreads, float *const *writes, int nwrites,
+                             int nx, int nz, void *stream);
 /* Time the passes with HIP events on the context stream: one untimed
  * stress+velocity step, then nsteps (1..4096) stress passes back to back,
  * then nsteps velocity passes; fills the average milliseconds of each
