@@ -106,3 +106,22 @@ def test_tile_table_kinds_and_block_shapes():
         rz, spb, k, rows = (x.value for x in v)
         assert spb == 1 and k in (1, 2) and rows % rz == 0
         assert 64 - 14 * k > 0 and rows - 14 * k > 0
+
+
+def test_calibration_kernels_reject_bad_arguments_without_gpu_work():
+    """Argument checks of the copy and tiled-stream calibration entry
+    points run before any launch."""
+    lib = _lib.load()
+    buf = (ctypes.c_float * 16)()
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    assert lib.fd23_device_copy(p, p, 20, 1, None) == _lib.EINVAL          # not a multiple of 16
+    assert lib.fd23_device_copy(None, p, 16, 1, None) == _lib.EINVAL
+    arr = (ctypes.c_void_p * 9)(*([p.value] * 9))
+    assert lib.fd23_device_stream_tiles(arr, 0, arr, 1, 64, 16, None) == _lib.EINVAL   # no array read
+    assert lib.fd23_device_stream_tiles(arr, 9, arr, 1, 64, 16, None) == _lib.EINVAL   # more than 8
+    assert lib.fd23_device_stream_tiles(arr, 1, arr, 4, 64, 16, None) == _lib.EINVAL   # more than 3 written
+    assert lib.fd23_device_stream_tiles(arr, 1, arr, 1, 96, 16, None) == _lib.EINVAL   # nx not a multiple of 64
+    assert lib.fd23_device_stream_tiles(arr, 1, arr, 1, 64, 24, None) == _lib.EINVAL   # nz not a multiple of 16
+    nul = (ctypes.c_void_p * 2)(p.value, None)
+    assert lib.fd23_device_stream_tiles(nul, 2, arr, 1, 64, 16, None) == _lib.EINVAL
+    assert b"null" in lib.fd23_last_error()
