@@ -55,7 +55,7 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong), ("failed", ctypes.c_longlong),
                 ("queued", ctypes.c_longlong), ("running", ctypes.c_longlong), ("launches", ctypes.c_longlong),
                 ("merged", ctypes.c_longlong), ("peak_active", ctypes.c_longlong), ("bytes_in_use", c_sz),
-                ("budget", c_sz), ("cell_updates", c_dbl)]
+                ("budget", c_sz), ("cell_updates", c_dbl), ("joined", ctypes.c_longlong)]
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
@@ -75,6 +75,8 @@ SIGNATURES = {
     "fd23_set_cpml": (c_int, [c_vp, c_vp]),
     "fd23_set_source": (c_int, [c_vp, c_int, c_int, c_int, c_int, c_vp, c_int, c_vp]),
     "fd23_set_receivers": (c_int, [c_vp, c_int, c_int, c_vp, c_vp, c_int]),
+    "fd23_set_shot_slot": (c_int, [c_vp, c_int, c_int, c_int]),
+    "fd23_reset_shot": (c_int, [c_vp, c_int]),
     "fd23_step_stress": (c_int, [c_vp]),
     "fd23_inject": (c_int, [c_vp, c_int]),
     "fd23_step_velocity": (c_int, [c_vp]),

@@ -117,6 +117,15 @@ class Context:
         check(self.lib.fd23_set_receivers(self.h, shot, comp, _ptr(rx), _ptr(rz), rx.size))
         self.nrec[shot, comp] = rx.size
 
+    def set_shot_slot(self, shot, active=True, t0=0):
+        """Start shot `shot` at step t0 (its samples and frames count from
+        there), or make its slot idle (not stepped by run)."""
+        check(self.lib.fd23_set_shot_slot(self.h, shot, 1 if active else 0, t0))
+
+    def reset_shot(self, shot):
+        """Zero one shot's fields, C-PML memory and seismogram entries."""
+        check(self.lib.fd23_reset_shot(self.h, shot))
+
     def set_scales(self, scales):
         s = np.ascontiguousarray(scales, np.float32)
         check(self.lib.fd23_set_scales(self.h, _ptr(s), s.size))

@@ -98,6 +98,7 @@ static int round_up(int a, int b) { return (a + b - 1) / b * b; }
 
 struct HostSrc {
     int kind = -1, xs = 0, zs = 0;
+    int t0 = 0; // the shot's first step (fd23_set_shot_slot)
     std::vector<double> stf;
     double M[3] = {0, 0, 0};
 };
@@ -188,6 +189,9 @@ struct fd23_ctx {
     void *seis = nullptr;
 
     int tile = 0;
+    // idle shot slots 0..63 (fd23_set_shot_slot): a kernel argument of the
+    // graph loop's launches, so a change invalidates the captured graphs
+    unsigned long long idle = 0;
     bool have_mat = false, have_cpml = false;
 
     // graphs keyed by (itrecord, steps, current set); flips: the chunk
@@ -219,8 +223,11 @@ long fd23_ctx_sstride(fd23_ctx *c) { return c->sstride; }
 void *&fd23_ctx_box(fd23_ctx *c) { return c->box; }
 const fd23_desc &fd23_ctx_desc(fd23_ctx *c) { return c->d; }
 int *fd23_ctx_clock(fd23_ctx *c) { return c->clock; }
-// captured graphs bake in the per-step launches; the box changes them
+// captured graphs bake in the per-step launches; the box changes them.  A
+// replay may still run on the context stream: it ends before its graph is
+// destroyed.
 void fd23_ctx_invalidate_graphs(fd23_ctx *c) {
+    if (!c->graphs.empty()) hipStreamSynchronize(c->stream);
     for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second.exec);
     c->graphs.clear();
 }
@@ -572,6 +579,7 @@ static int upload_sources(fd23_ctx *c) {
         SrcInfo si{};
         si.tap0 = (int)taps.size();
         si.stf0 = (long)stf.size();
+        si.t0 = h.t0;
         if (h.kind >= 0) {
             build_taps(c, h, taps);
             si.lsrc = (int)h.stf.size();
@@ -635,6 +643,60 @@ extern "C" int fd23_set_receivers(fd23_ctx *c, int shot, int comp, const int *rx
     c->events_dirty = true;
     c->fevents_dirty = true;
     c->bevents_dirty = true;
+    return FD23_OK;
+}
+
+// ---------------------------------------------------------------------------
+// shot slots: shots that start while the context runs
+// ---------------------------------------------------------------------------
+static bool slots_default(const fd23_ctx *c) {
+    if (c->idle) return false;
+    for (int s = 0; s < c->d.nshots; s++)
+        if (c->src[s].t0 != 0) return false;
+    return true;
+}
+
+extern "C" int fd23_set_shot_slot(fd23_ctx *c, int shot, int active, int t0) {
+    if (!c || shot < 0 || shot >= c->d.nshots) return fail(FD23_EINVAL, "bad shot %d", shot);
+    if (t0 < 0) return fail(FD23_EINVAL, "a shot's first step must be >= 0");
+    if (!active && shot >= 64) return fail(FD23_EINVAL, "only shots 0..63 can be idle");
+    if ((c->box || c->punch) && (!active || t0 != 0))
+        return fail(FD23_EINVAL, "the box and the first-arrival window run every shot from step 0");
+    if (c->src[shot].t0 != t0) {
+        c->src[shot].t0 = t0;
+        c->src_dirty = true; // rebuilds the lists, and with them the graphs
+    }
+    if (shot < 64) {
+        const unsigned long long bit = 1ull << shot, was = c->idle;
+        c->idle = active ? was & ~bit : was | bit;
+        if (c->idle != was) fd23_ctx_invalidate_graphs(c); // a kernel argument
+    }
+    return FD23_OK;
+}
+
+extern "C" int fd23_reset_shot(fd23_ctx *c, int shot) {
+    if (!c || shot < 0 || shot >= c->d.nshots) return fail(FD23_EINVAL, "bad shot %d", shot);
+    const size_t es = c->esize;
+    for (int set = 0; set < 2; set++) {
+        void *const *f = set ? c->alt_fields : c->fields;
+        void *const *q = set ? c->alt_psi : c->psi;
+        if (set && !c->alt_alloc[0]) break; // no second buffer set yet
+        for (int i = 0; i < c->nfield; i++)
+            HIPCHK(hipMemset2DAsync((char *)f[i] + (size_t)shot * c->sstride * es, (size_t)c->px * es, 0,
+                                    (size_t)c->d.nx * es, c->d.nz, c->stream));
+        const int half = c->npsi / 2;
+        for (int p = 0; p < c->npsi; p++) {
+            const size_t n = p < half ? c->xpsi : c->zpsi;
+            if (n) HIPCHK(hipMemsetAsync((char *)q[p] + (size_t)shot * n * es, 0, n * es, c->stream));
+        }
+    }
+    // the shot's seismogram entries of every frame (receivers off the grid
+    // are never written)
+    if (c->seis && c->d.max_rec > 0) {
+        const size_t frame = (size_t)c->d.nshots * c->ncomp * c->d.max_rec * c->csize;
+        const size_t mine = (size_t)c->ncomp * c->d.max_rec * c->csize;
+        HIPCHK(hipMemset2DAsync((char *)c->seis + shot * mine, frame, 0, mine, c->d.max_frames, c->stream));
+    }
     return FD23_OK;
 }
 
@@ -992,6 +1054,7 @@ static Events make_events(const fd23_ctx *c, int k, int itrecord) {
     e.max_frames = c->seis ? c->d.max_frames : 0;
     e.frame_size = (long)c->d.nshots * c->ncomp * c->d.max_rec;
     e.pwin = c->punch ? c->d_pwin : nullptr;
+    e.idle = c->idle;
     return e;
 }
 
@@ -1133,6 +1196,7 @@ static void launch_fused_t(fd23_ctx *c, int k, int itrecord, bool record, unsign
     ev.seis = c->seis;
     ev.max_frames = (c->seis && record) ? c->d.max_frames : 0;
     ev.frame_size = (long)c->d.nshots * c->ncomp * c->d.max_rec;
+    ev.idle = c->idle;
     auto fields = [](void *const *f) {
         return PsvFields<S>{(S *)f[0], (S *)f[1], (S *)f[2], (S *)f[3], (S *)f[4]};
     };
@@ -1181,6 +1245,7 @@ template <class S, int CZ> static void launch_fused_tile_t(fd23_ctx *c, int k, i
     ev.seis = c->seis;
     ev.max_frames = (c->seis && record) ? c->d.max_frames : 0;
     ev.frame_size = (long)c->d.nshots * c->ncomp * c->d.max_rec;
+    ev.idle = c->idle;
     auto fields = [](void *const *f) {
         return PsvFields<S>{(S *)f[0], (S *)f[1], (S *)f[2], (S *)f[3], (S *)f[4]};
     };
@@ -1261,6 +1326,7 @@ static void launch_block_t(fd23_ctx *c, int k, int nsub, int itrecord, bool even
         ev.seis = c->seis;
         ev.max_frames = c->seis ? c->d.max_frames : 0;
         ev.frame_size = (long)c->d.nshots * c->ncomp * c->d.max_rec;
+        ev.idle = c->idle;
     }
     auto fields = [](void *const *f) { return ShFields<S>{(S *)f[0], (S *)f[1], (S *)f[2]}; };
     auto psi = [](void *const *p) { return ShPsi4<S>{(S *)p[0], (S *)p[1], (S *)p[2], (S *)p[3]}; };
@@ -1310,7 +1376,7 @@ template <class S> static void launch_record_t(fd23_ctx *c, int it, int itrecord
     const int nblk = (int)std::min<long>((total + 255) / 256, 1024);
     hipLaunchKernelGGL((k_record<S>), dim3(nblk), dim3(256), 0, c->stream, (S *const *)c->d_comptab, c->ncomp,
                        c->sstride, c->d_roff, c->d_nrec, c->d.max_rec, c->d.nshots, (C *)c->seis,
-                       c->d.max_frames, it, itrecord, c->scl, c->d_compfield);
+                       c->d.max_frames, it, itrecord, c->scl, c->d_compfield, c->d_src);
 }
 
 static void dispatch_record(fd23_ctx *c, int it, int itrecord) {
@@ -1511,6 +1577,8 @@ extern "C" int fd23_step_velocity(fd23_ctx *c) {
 extern "C" int fd23_record(fd23_ctx *c, int it, int itrecord) {
     if (!c) return fail(FD23_EINVAL, "null context");
     if (itrecord < 1) return fail(FD23_EINVAL, "itrecord must be >= 1");
+    int r = upload_sources(c); // the shots' first steps
+    if (r) return r;
     dispatch_record(c, it, itrecord);
     return check_launch();
 }
@@ -1603,6 +1671,8 @@ static int run_prologue(fd23_ctx *c, int it0, int nt, int itrecord) {
     int r = ready(c);
     if (r) return r;
     if (nt < 0 || it0 < 0 || itrecord < 1) return fail(FD23_EINVAL, "bad run range");
+    if ((c->box || c->punch) && !slots_default(c))
+        return fail(FD23_EINVAL, "the box and the first-arrival window run every shot from step 0");
     if (c->box && (r = fd23_box_check_run(c, it0, nt))) return r;
     if ((r = build_events(c))) return r;
     if (use_fused(c) && ((r = ensure_alt(c)) || (r = build_fused_events(c)))) return r;

@@ -57,14 +57,37 @@ struct Request {
     std::vector<char> seis;       // [shot][comp][frame][rec]
 };
 
+// A request inside a context: its shots occupy some of the context's shot
+// slots and start at context step t0 (fd23_set_shot_slot), so its
+// seismograms read back from frame 0 as if it had run alone.
+struct Member {
+    std::shared_ptr<Request> q;
+    std::vector<int> slots; // the context's shot slot of each request shot
+    int t0 = 0;
+    bool done = false;
+    int end() const { return t0 + q->r.nt; }
+};
+
 struct Group {
     fd23_ctx *ctx = nullptr;
-    int slot = -1;
-    std::vector<std::shared_ptr<Request>> reqs;
-    int nshots = 0, nt = 0, itrecord = 1, next_it = 0;
+    int slot = -1;                  // stream slot
+    std::vector<Member> mem;
+    std::vector<int> owner;         // per shot slot: member index, -1 idle
+    int nshots = 0, itrecord = 1, max_frames = 0;
+    int next_it = 0;                // context steps enqueued
+    int done_it = 0;                // context steps completed
     size_t bytes = 0;
-    std::deque<hipEvent_t> inflight;
+    std::deque<std::pair<hipEvent_t, int>> inflight; // chunk event, the step it reaches
     bool failed = false;
+    bool box = false;
+    // the end of the next member to finish after step it, or -1
+    int next_end(int it) const {
+        int e = -1;
+        for (const auto &m : mem)
+            if (!m.done && m.end() > it && (e < 0 || m.end() < e)) e = m.end();
+        return e;
+    }
+    int free_slots() const { return (int)std::count(owner.begin(), owner.end(), -1); }
 };
 
 int material_count(int wave) { return wave == FD23_WAVE_PSV ? 5 : 3; }
@@ -73,12 +96,14 @@ int material_count(int wave) { return wave == FD23_WAVE_PSV ? 5 : 3; }
 bool compatible(const Request &a, const Request &b) {
     const fd23_request &x = a.r, &y = b.r;
     if (x.model_key == 0 || x.model_key != y.model_key) return false;
-    if (std::make_tuple(x.wave, a.storage, x.nx, x.nz, x.npml, x.topabsorb, x.nt, x.itrecord, x.src_kind,
-                        x.lsrc, x.nrec, x.box) !=
-        std::make_tuple(y.wave, b.storage, y.nx, y.nz, y.npml, y.topabsorb, y.nt, y.itrecord, y.src_kind, y.lsrc,
-                        y.nrec, y.box))
+    // what the context holds: the model, grid, time step, C-PML, storage,
+    // receiver capacity and box.  Sources are set per shot, and the number
+    // of steps may differ: each request's shots keep their own first step
+    // and its results are read out when its last step is done.
+    if (std::make_tuple(x.wave, a.storage, x.nx, x.nz, x.npml, x.topabsorb, x.itrecord, x.nrec, x.box) !=
+        std::make_tuple(y.wave, b.storage, y.nx, y.nz, y.npml, y.topabsorb, y.itrecord, y.nrec, y.box))
         return false;
-    if (x.dx != y.dx || x.dt != y.dt || a.stf != b.stf || a.M != b.M || a.cpml != b.cpml) return false;
+    if (x.dx != y.dx || x.dt != y.dt || a.cpml != b.cpml) return false;
     if (x.box &&
         (std::make_tuple(x.box_x0, x.box_z0, x.box_bx, x.box_bz, x.box_incident) !=
              std::make_tuple(y.box_x0, y.box_z0, y.box_bx, y.box_bz, y.box_incident) ||
@@ -103,8 +128,9 @@ fd23_desc make_desc(const Request &q, int nshots, void *stream) {
     return d;
 }
 
-size_t batch_bytes(const Request &q, int nshots) {
+size_t batch_bytes(const Request &q, int nshots, int frames) {
     fd23_desc d = make_desc(q, nshots, nullptr);
+    d.max_frames = frames;
     size_t b = fd23_desc_device_bytes(&d);
     if (q.r.box) b += (size_t)nshots * 5 * q.r.box_bx * q.r.box_bz * q.csize + 256;
     return b + ((size_t)1 << 20); // graphs, event lists, tables
@@ -136,12 +162,32 @@ struct fd23_server {
     bool schedule();
     bool retire();
     int start_group(Group &g);
+    int place(Group &g, Member &m); // sources and receivers of a member's slots
+    int join(Group &g, const std::shared_ptr<Request> &q, const std::vector<int> &slots);
+    int harvest(Group &g, Member &m);
     void finish_group(Group &g);
+    void complete(Request &q, bool failed_ctx);
 };
 
+int fd23_server::place(Group &g, Member &m) {
+    const Request &q = *m.q;
+    int r;
+    for (int s = 0; s < q.r.nshots; s++) {
+        const int shot = m.slots[s];
+        if (q.r.src_kind >= 0 &&
+            (r = fd23_set_source(g.ctx, shot, q.r.src_kind, q.xs[s], q.zs[s], q.stf.data(), q.r.lsrc, q.M.data())))
+            return r;
+        for (int c = 0; c < q.ncomp; c++)
+            if (q.r.nrec > 0 && (r = fd23_set_receivers(g.ctx, shot, c, q.rx.data(), q.rz.data(), q.r.nrec)))
+                return r;
+    }
+    return FD23_OK;
+}
+
 int fd23_server::start_group(Group &g) {
-    const Request &q0 = *g.reqs[0];
+    const Request &q0 = *g.mem[0].q;
     fd23_desc d = make_desc(q0, g.nshots, streams[g.slot]);
+    d.max_frames = g.max_frames; // the longest member
     int r;
     if ((r = fd23_ctx_create(&d, &g.ctx))) return r;
     if (q0.from_model) {
@@ -152,19 +198,8 @@ int fd23_server::start_group(Group &g) {
     }
     if (r) return r;
     if ((r = fd23_set_cpml(g.ctx, q0.cpml.data()))) return r;
-    int shot = 0;
-    for (auto &q : g.reqs) {
-        for (int s = 0; s < q->r.nshots; s++, shot++) {
-            if (q->r.src_kind >= 0 &&
-                (r = fd23_set_source(g.ctx, shot, q->r.src_kind, q->xs[s], q->zs[s], q->stf.data(), q->r.lsrc,
-                                     q->M.data())))
-                return r;
-            for (int c = 0; c < q->ncomp; c++)
-                if (q->r.nrec > 0 &&
-                    (r = fd23_set_receivers(g.ctx, shot, c, q->rx.data(), q->rz.data(), q->r.nrec)))
-                    return r;
-        }
-    }
+    for (auto &m : g.mem)
+        if ((r = place(g, m))) return r;
     const fd23_request &b = q0.r;
     if (b.box && (r = fd23_box_configure(g.ctx, -1, b.box_x0, b.box_z0, b.box_bx, b.box_bz, b.box_incident,
                                          b.box_angle, b.box_f0, b.box_t0, b.box_amp, b.box_vp, b.box_vs,
@@ -184,9 +219,13 @@ int fd23_server::start_group(Group &g) {
     return FD23_OK;
 }
 
-// admission: returns true if anything was admitted
+// admission: returns true if anything was admitted.  A queued request first
+// joins a running context that holds a compatible request and enough idle
+// shot slots (its shots start at that context's next step); the rest are
+// merged into new contexts as HBM and stream slots allow.
 bool fd23_server::admit() {
     std::vector<std::vector<std::shared_ptr<Request>>> batches;
+    std::vector<std::tuple<Group *, std::shared_ptr<Request>, std::vector<int>>> joins;
     {
         std::lock_guard<std::mutex> lk(mu);
         size_t free_dev = 0, total_dev = 0;
@@ -194,6 +233,25 @@ bool fd23_server::admit() {
         int idle = 0;
         for (bool b : slot_busy) idle += !b;
         std::vector<bool> taken(queue.size(), false);
+        // joins into running contexts (slots reserved here, placed below)
+        for (size_t i = 0; i < queue.size(); i++) {
+            const Request &q = *queue[i];
+            for (auto &gp : groups) {
+                Group &g = *gp;
+                if (g.failed || g.box || q.r.box || g.mem.empty() || !compatible(*g.mem[0].q, q) ||
+                    q.nframes > g.max_frames || g.free_slots() < q.r.nshots || g.next_end(g.next_it) < 0)
+                    continue;
+                std::vector<int> slots;
+                for (int s = 0; s < g.nshots && (int)slots.size() < q.r.nshots; s++)
+                    if (g.owner[s] == -1) {
+                        g.owner[s] = -2; // reserved
+                        slots.push_back(s);
+                    }
+                joins.emplace_back(&g, queue[i], slots);
+                taken[i] = true;
+                break;
+            }
+        }
         size_t planned = 0;
         for (size_t i = 0; i < queue.size() && idle > 0; i++) {
             if (taken[i]) continue;
@@ -205,7 +263,9 @@ bool fd23_server::admit() {
                 pick.push_back(j);
                 shots += queue[j]->r.nshots;
             }
-            const size_t need = batch_bytes(*queue[i], shots);
+            int frames = 0;
+            for (size_t k : pick) frames = std::max(frames, queue[k]->nframes);
+            const size_t need = batch_bytes(*queue[i], shots, frames);
             if (pick.size() == 1 && need > d.hbm_budget) {
                 // can never fit: fail it rather than let it wait forever
                 taken[i] = true;
@@ -233,21 +293,52 @@ bool fd23_server::admit() {
         for (size_t i = 0; i < queue.size(); i++)
             if (!taken[i]) rest.push_back(queue[i]);
         queue.swap(rest);
-        if (batches.empty()) return false;
+        if (batches.empty() && joins.empty()) return false;
         const auto now = Clock::now();
         for (auto &b : batches)
             for (auto &q : b) {
                 q->state = FD23_REQ_RUNNING;
                 q->t_admit = now;
             }
+        for (auto &j : joins) {
+            std::get<1>(j)->state = FD23_REQ_RUNNING;
+            std::get<1>(j)->t_admit = now;
+        }
+    }
+    for (auto &j : joins) {
+        Group &g = *std::get<0>(j);
+        const int r = join(g, std::get<1>(j), std::get<2>(j));
+        std::lock_guard<std::mutex> lk(mu);
+        if (r) {
+            const char *m = fd23_last_error();
+            Request &q = *std::get<1>(j);
+            q.err = r;
+            q.msg = m ? m : "";
+            q.state = FD23_REQ_FAILED;
+            q.t_done = Clock::now();
+            st.failed++;
+            cv_done.notify_all();
+        } else {
+            st.joined++;
+        }
     }
     for (auto &b : batches) {
         auto g = std::make_unique<Group>();
-        g->reqs = b;
-        for (auto &q : b) g->nshots += q->r.nshots;
-        g->nt = b[0]->r.nt;
+        int shot = 0;
+        for (auto &q : b) {
+            Member m;
+            m.q = q;
+            for (int s = 0; s < q->r.nshots; s++) m.slots.push_back(shot++);
+            g->mem.push_back(std::move(m));
+            g->max_frames = std::max(g->max_frames, q->nframes);
+        }
+        g->nshots = shot;
+        g->owner.assign(shot, 0);
+        for (size_t k = 0; k < g->mem.size(); k++)
+            for (int s : g->mem[k].slots) g->owner[s] = (int)k;
         g->itrecord = b[0]->r.itrecord;
-        g->bytes = batch_bytes(*b[0], g->nshots);
+        g->box = b[0]->r.box != 0;
+        g->bytes = batch_bytes(*b[0], g->nshots, g->max_frames);
         for (size_t s = 0; s < slot_busy.size(); s++)
             if (!slot_busy[s]) {
                 g->slot = (int)s;
@@ -276,94 +367,165 @@ bool fd23_server::admit() {
     return true;
 }
 
-// give every resident context its next chunk; true if anything was queued
+// a request enters idle slots of a running context: its shots start at the
+// context's next step, from zero fields
+int fd23_server::join(Group &g, const std::shared_ptr<Request> &q, const std::vector<int> &slots) {
+    // the chunks in flight finish first: the slots' arrays and the event
+    // lists change under them
+    if (hipStreamSynchronize(streams[g.slot]) != hipSuccess) return FD23_EHIP;
+    Member m;
+    m.q = q;
+    m.slots = slots;
+    m.t0 = g.next_it;
+    int r = FD23_OK;
+    for (int s : slots)
+        if ((r = fd23_reset_shot(g.ctx, s)) || (r = fd23_set_shot_slot(g.ctx, s, 1, m.t0))) break;
+    if (!r) r = place(g, m);
+    if (r) {
+        for (int s : slots) {
+            g.owner[s] = -1;
+            (void)fd23_set_shot_slot(g.ctx, s, 0, 0);
+        }
+        return r;
+    }
+    for (int s : slots) g.owner[s] = (int)g.mem.size();
+    g.mem.push_back(std::move(m));
+    return FD23_OK;
+}
+
+// give every resident context its next chunk; true if anything was queued.
+// Chunks end where a request ends, so that its results are final when the
+// chunk's event completes.
 bool fd23_server::schedule() {
     bool any = false;
     for (auto &gp : groups) {
         Group &g = *gp;
-        while (!g.failed && g.next_it < g.nt && g.inflight.size() < 2) {
-            const int n = std::min(d.chunk_steps, g.nt - g.next_it);
+        while (!g.failed && g.inflight.size() < 2) {
+            const int e = g.next_end(g.next_it);
+            if (e < 0) break;
+            const int n = std::min(d.chunk_steps, e - g.next_it);
             int r = fd23_run(g.ctx, g.next_it, n, g.itrecord);
-            hipEvent_t e = nullptr;
-            if (!r && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) r = FD23_EHIP;
-            if (!r && hipEventRecord(e, streams[g.slot]) != hipSuccess) r = FD23_EHIP;
+            hipEvent_t ev = nullptr;
+            if (!r && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) r = FD23_EHIP;
+            if (!r && hipEventRecord(ev, streams[g.slot]) != hipSuccess) r = FD23_EHIP;
             if (r) {
-                if (e) (void)hipEventDestroy(e);
+                if (ev) (void)hipEventDestroy(ev);
                 const char *m = fd23_last_error();
                 std::lock_guard<std::mutex> lk(mu);
-                for (auto &q : g.reqs) {
-                    q->err = r;
-                    q->msg = m && *m ? m : "chunk launch failed";
+                for (auto &mb : g.mem) {
+                    mb.q->err = r;
+                    mb.q->msg = m && *m ? m : "chunk launch failed";
                 }
                 g.failed = true;
                 break;
             }
-            g.inflight.push_back(e);
             g.next_it += n;
+            g.inflight.emplace_back(ev, g.next_it);
             any = true;
         }
     }
     return any;
 }
 
+// copy a finished member's seismograms out
+int fd23_server::harvest(Group &g, Member &m) {
+    Request &q = *m.q;
+    const size_t per = (size_t)q.nframes * q.r.nrec * q.csize;
+    q.seis.assign(per * q.r.nshots * q.ncomp, 0);
+    std::vector<char> tmp(per);
+    for (int s = 0; s < q.r.nshots; s++)
+        for (int c = 0; c < q.ncomp; c++) {
+            if (q.r.nrec == 0) continue;
+            int r = fd23_get_seismograms(g.ctx, m.slots[s], c, (float *)tmp.data(), q.nframes);
+            if (r) return r;
+            std::memcpy(q.seis.data() + ((size_t)s * q.ncomp + c) * per, tmp.data(), per);
+        }
+    return FD23_OK;
+}
+
+// state and statistics of a request leaving the engine (lock held)
+void fd23_server::complete(Request &q, bool failed_ctx) {
+    q.t_done = Clock::now();
+    if (failed_ctx && !q.err) {
+        q.err = FD23_EHIP;
+        q.msg = "context failed";
+    }
+    q.state = q.err ? FD23_REQ_FAILED : FD23_REQ_DONE;
+    if (q.err) {
+        st.failed++;
+    } else {
+        st.completed++;
+        st.cell_updates += (double)q.r.nshots * q.r.nx * q.r.nz * q.r.nt;
+    }
+}
+
 void fd23_server::finish_group(Group &g) {
     // the stream is idle here (last event completed) unless the group failed
     if (g.ctx) (void)hipStreamSynchronize(streams[g.slot]);
-    int shot0 = 0;
-    for (auto &q : g.reqs) {
-        if (!g.failed && !q->err) {
-            const size_t per = (size_t)q->nframes * q->r.nrec * q->csize;
-            q->seis.assign(per * q->r.nshots * q->ncomp, 0);
-            std::vector<char> tmp(per);
-            for (int s = 0; s < q->r.nshots && !q->err; s++)
-                for (int c = 0; c < q->ncomp && !q->err; c++) {
-                    if (q->r.nrec == 0) continue;
-                    int r = fd23_get_seismograms(g.ctx, shot0 + s, c, (float *)tmp.data(), q->nframes);
-                    if (r) {
-                        q->err = r;
-                        q->msg = fd23_last_error();
-                        break;
-                    }
-                    std::memcpy(q->seis.data() + ((size_t)s * q->ncomp + c) * per, tmp.data(), per);
-                }
+    for (auto &m : g.mem) {
+        if (m.done) continue;
+        if (!g.failed && !m.q->err) {
+            int r = harvest(g, m);
+            if (r) {
+                m.q->err = r;
+                m.q->msg = fd23_last_error();
+            }
         }
-        shot0 += q->r.nshots;
     }
-    for (hipEvent_t e : g.inflight) (void)hipEventDestroy(e);
+    for (auto &e : g.inflight) (void)hipEventDestroy(e.first);
     g.inflight.clear();
     if (g.ctx) fd23_ctx_destroy(g.ctx);
     g.ctx = nullptr;
     used -= g.bytes;
     slot_busy[g.slot] = false;
-    const auto now = Clock::now();
     std::lock_guard<std::mutex> lk(mu);
-    for (auto &q : g.reqs) {
-        q->t_done = now;
-        if (g.failed && !q->err) {
-            q->err = FD23_EHIP;
-            q->msg = "context failed";
+    for (auto &m : g.mem)
+        if (!m.done) {
+            m.done = true;
+            complete(*m.q, g.failed);
         }
-        q->state = q->err ? FD23_REQ_FAILED : FD23_REQ_DONE;
-        if (q->err) {
-            st.failed++;
-        } else {
-            st.completed++;
-            st.cell_updates += (double)q->r.nshots * q->r.nx * q->r.nz * q->r.nt;
-        }
-    }
     cv_done.notify_all();
 }
 
-// retire finished contexts; true if any retired
+// retire finished requests (their slots go idle) and finished contexts;
+// true if any retired
 bool fd23_server::retire() {
     bool any = false;
     for (size_t i = 0; i < groups.size();) {
         Group &g = *groups[i];
-        while (!g.inflight.empty() && hipEventQuery(g.inflight.front()) == hipSuccess) {
-            (void)hipEventDestroy(g.inflight.front());
+        while (!g.inflight.empty() && hipEventQuery(g.inflight.front().first) == hipSuccess) {
+            g.done_it = g.inflight.front().second;
+            (void)hipEventDestroy(g.inflight.front().first);
             g.inflight.pop_front();
         }
-        if (g.failed || (g.next_it >= g.nt && g.inflight.empty())) {
+        bool open = false; // a member still runs
+        if (!g.failed)
+            for (size_t k = 0; k < g.mem.size(); k++) {
+                Member &m = g.mem[k];
+                if (m.done) continue;
+                if (m.end() > g.done_it) {
+                    open = true;
+                    continue;
+                }
+                if (g.next_end(g.next_it) < 0 && g.inflight.empty()) continue; // the group ends: harvested there
+                // a request ends inside a running context: results out, slots idle
+                int r = harvest(g, m);
+                for (int s : m.slots) {
+                    g.owner[s] = -1;
+                    // a box context runs every shot (and takes no joins)
+                    if (!r && !g.box) r = fd23_set_shot_slot(g.ctx, s, 0, 0);
+                }
+                if (r) {
+                    m.q->err = r;
+                    m.q->msg = fd23_last_error();
+                }
+                m.done = true;
+                any = true;
+                std::lock_guard<std::mutex> lk(mu);
+                complete(*m.q, false);
+                cv_done.notify_all();
+            }
+        if (g.failed || (!open && g.inflight.empty())) {
             finish_group(g);
             groups.erase(groups.begin() + i);
             any = true;

@@ -92,6 +92,9 @@ __global__ __launch_bounds__(TX *NW) void k_sh_block(ShFields<S> fa, ShFields<S>
     const int L = (int)(blockIdx.x & 7) * q + (int)(blockIdx.x >> 3);
     if (L >= w.nwork) return;
     const int ntiles = w.ntx * w.ntz;
+    // idle shot slots are stepped too (their state is reset before reuse):
+    // a test here would put a wait on the kernel arguments ahead of the
+    // prologue's loads, and this kernel runs where that latency counts
     const int shot = L / ntiles;
     const int tile = L - shot * ntiles;
     const int tzi = tile / w.ntx, txi = tile - tzi * w.ntx;
@@ -178,8 +181,8 @@ __global__ __launch_bounds__(TX *NW) void k_sh_block(ShFields<S> fa, ShFields<S>
 #pragma unroll
         for (int j = 0; j < K; j++) {
             // src_sample, with the index clamped instead of a branch
-            const int t = it0 + j;
-            son[j] = ntap > 0 && j < w.nsub && si.lsrc > 0 && (t < si.lsrc || si.clamp);
+            const int t = it0 + j - si.t0;
+            son[j] = ntap > 0 && j < w.nsub && si.lsrc > 0 && t >= 0 && (t < si.lsrc || si.clamp);
             sval[j] = ev.stf[son[j] ? si.stf0 + min(t, si.lsrc - 1) : 0];
         }
     }
@@ -368,8 +371,10 @@ __global__ __launch_bounds__(TX *NW) void k_sh_block(ShFields<S> fa, ShFields<S>
         // receivers of the tile (the lists hold tile cells only), one per
         // thread from the V just written to LDS; the next writes of sV come
         // after the next stress half's barrier
-        if (ev.clock && it % ev.itrecord == 0 && it / ev.itrecord < ev.max_frames) {
-            C *seis = (C *)ev.seis + (long)(it / ev.itrecord) * ev.frame_size;
+        // frames count from the shot's first step
+        const int ts = it - si.t0;
+        if (ev.clock && ts >= 0 && ts % ev.itrecord == 0 && ts / ev.itrecord < ev.max_frames) {
+            C *seis = (C *)ev.seis + (long)(ts / ev.itrecord) * ev.frame_size;
             for (int e = re0; e < r1; e += TX * NW) {
                 const RecT rc = e == re0 ? rec0 : ev.recs[e];
                 const int lx = rc.cell % TX, lz = rc.cell / TX;

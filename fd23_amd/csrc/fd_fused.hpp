@@ -46,6 +46,7 @@ struct FusedEvents {
     void *seis;
     int max_frames;
     long frame_size;
+    unsigned long long idle;       // idle shot slots 0..63, not stepped
 };
 template <class C> struct PsvMat5 {
     const C *BU, *BW, *MU, *MUA, *LAM; // pitch px
@@ -85,6 +86,7 @@ __global__ __launch_bounds__(FW) void k_psv_fused(PsvFields<S> fa, PsvFields<S> 
     if (L >= w.nwork) return;
     const int S_ = g.nshots;
     const int shot = L % S_;
+    if (shot_idle(ev.idle, shot)) return; // uniform per workgroup
     const int rest = L / S_;
     const int strip = rest % w.nstrip;
     const int seg = rest / w.nstrip;
@@ -102,12 +104,14 @@ __global__ __launch_bounds__(FW) void k_psv_fused(PsvFields<S> fa, PsvFields<S> 
     const bool xint = x0 - HALO >= lo && x0 + FO + HALO <= nx - lo;
     const int it = ev.clock[0] + ev.k;
     double sv = 0.0;
-    const bool src_on = src_sample(ev.src[shot], ev.stf, it, sv);
+    const SrcInfo si = ev.src[shot];
+    const bool src_on = src_sample(si, ev.stf, it, sv);
     const int t0 = ev.tap_start[strip * S_ + shot], t1 = ev.tap_start[strip * S_ + shot + 1];
     const int rkey = (seg * w.nstrip + strip) * S_ + shot;
     int r0 = ev.rec_start[rkey];
     const int r1 = ev.rec_start[rkey + 1];
-    const bool rec_step = (it % ev.itrecord) == 0 && it / ev.itrecord < ev.max_frames;
+    const int ts = it - si.t0; // step of the shot: frames count from its first step
+    const bool rec_step = ts >= 0 && (ts % ev.itrecord) == 0 && ts / ev.itrecord < ev.max_frames;
 
     // scales: fields U W Txx Tzz Txz read (current) and written (next)
     float sc_u = 1.f, sc_w = 1.f, sc_xx = 1.f, sc_zz = 1.f, sc_xz = 1.f;
@@ -323,7 +327,7 @@ __global__ __launch_bounds__(FW) void k_psv_fused(PsvFields<S> fa, PsvFields<S> 
                 __syncthreads();
                 int r = r0;
                 while (r < r1 && ev.recs[r].row == rv) r++;
-                C *seis = (C *)ev.seis + (long)(it / ev.itrecord) * ev.frame_size;
+                C *seis = (C *)ev.seis + (long)(ts / ev.itrecord) * ev.frame_size;
                 for (int e = r0 + lane; e < r; e += FW) {
                     const RecF rc = ev.recs[e];
                     seis[rc.out] = rc.comp == 0 ? pU[rc.lane] : pW[rc.lane];
@@ -420,6 +424,7 @@ __global__ __launch_bounds__(FW * 4) void k_psv_fused_tile(PsvFields<S> fa, PsvF
     if (!map_tile<OZ, REGION>(w, t)) return;
     const int S_ = g.nshots;
     const int shot = t.group;
+    if (shot_idle(ev.idle, shot)) return; // uniform per workgroup
     const int strip = t.tile % w.ntx;
     const int seg = t.tile / w.ntx;
     const int lane = threadIdx.x, ty = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
@@ -473,11 +478,13 @@ __global__ __launch_bounds__(FW * 4) void k_psv_fused_tile(PsvFields<S> fa, PsvF
     }
     const int it = ev.clock[0] + ev.k;
     double sv = 0.0;
-    const bool src_on = src_sample(ev.src[shot], ev.stf, it, sv);
+    const SrcInfo si = ev.src[shot];
+    const bool src_on = src_sample(si, ev.stf, it, sv);
     const int t0 = ev.tap_start[strip * S_ + shot], t1 = ev.tap_start[strip * S_ + shot + 1];
     const int rkey = (seg * w.ntx + strip) * S_ + shot;
     const int r0 = ev.rec_start[rkey], r1 = ev.rec_start[rkey + 1];
-    const bool rec_step = (it % ev.itrecord) == 0 && it / ev.itrecord < ev.max_frames;
+    const int ts = it - si.t0; // step of the shot: frames count from its first step
+    const bool rec_step = ts >= 0 && (ts % ev.itrecord) == 0 && ts / ev.itrecord < ev.max_frames;
 
 #pragma unroll
     for (int i = 0; i < UR; i++) {
@@ -591,7 +598,7 @@ __global__ __launch_bounds__(FW * 4) void k_psv_fused_tile(PsvFields<S> fa, PsvF
             pW[(ty * CZ + i) * FW + lane] = ww[i];
         }
         __syncthreads();
-        C *seis = (C *)ev.seis + (long)(it / ev.itrecord) * ev.frame_size;
+        C *seis = (C *)ev.seis + (long)(ts / ev.itrecord) * ev.frame_size;
         for (int e = r0 + tid; e < r1; e += FW * NW) {
             const RecF rc = ev.recs[e];
             const int c = (rc.row - z0) * FW + rc.lane;

@@ -130,6 +130,8 @@ struct SrcInfo {
     int lsrc;
     int clamp;      // 1: it >= lsrc uses stf[lsrc-1] ('p' / stype 13)
     long stf0;      // offset of this shot's samples in the stf table
+    int t0;         // the shot's first step: sample i is injected at step
+                    // t0 + i, frame f recorded at step t0 + f*itrecord
 };
 struct Events {
     const int *clock;     // nullptr: no fused events
@@ -148,7 +150,16 @@ struct Events {
     // workgroup has work (a block may be open, or the shot has a tap or
     // receiver in the tile), z < it < w when every block is open
     const int4 *pwin;
+    // shots 0..63 whose bit is set are idle slots, not stepped
+    unsigned long long idle;
 };
+
+// Idle shot slot (a kernel argument, so the test waits on no memory):
+// the graph loop skips such a shot's work.  Shots from 64 on are never
+// idle.
+__device__ __forceinline__ bool shot_idle(unsigned long long idle, int shot) {
+    return shot < 64 && ((idle >> shot) & 1ull);
+}
 
 // first-arrival window (the graph loop applies it: the step index is known
 // there only).  Bit r of rows_open is set when row r of the thread's RZ
@@ -189,7 +200,8 @@ __device__ __forceinline__ int tile_window(const Grid &g, const Events &ev, cons
 // without clamping), PSV/clean_nobox/point_source/cu_point_source.cu:47,110-112
 __device__ __forceinline__ bool src_sample(const SrcInfo &si, const double *stf, int it, double &sv) {
     if (si.lsrc <= 0) return false;
-    int t = it;
+    int t = it - si.t0;
+    if (t < 0) return false;
     if (t >= si.lsrc) {
         if (!si.clamp) return false;
         t = si.lsrc - 1;
@@ -455,6 +467,7 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
     for (int s = 0; s < SPB; s++) {
         const int shot = sh0 + s;
         if (shot >= g.nshots) break;
+        if (ev.clock && shot_idle(ev.idle, shot)) continue; // uniform per workgroup
         // buffer origin of this shot's arrays: element (row -HALO, column -HALO)
         const long org = (long)shot * g.sstride - (long)HALO * px - HALO;
         const BufRsrc bU = buf_rsrc(f.U + org), bW = buf_rsrc(f.W + org);
@@ -585,10 +598,11 @@ __global__ __launch_bounds__(TX *BY) void k_psv_stress(PsvFields<S> f, PsvStress
 // the tile's receivers read them.  Uniform per workgroup.
 template <class C, int RZ, int NC>
 __device__ __forceinline__ void fused_record(const Events &ev, int it, int key, C *park0, C *park1,
-                                             const C (&val)[NC][RZ], int lz0) {
+                                             const C (&val)[NC][RZ], int lz0, int t0) {
     const int e0 = ev.rec_start[key], e1 = ev.rec_start[key + 1];
-    if (e1 <= e0 || it % ev.itrecord != 0) return;
-    const int frame = it / ev.itrecord;
+    const int ts = it - t0; // step of the shot
+    if (e1 <= e0 || ts < 0 || ts % ev.itrecord != 0) return;
+    const int frame = ts / ev.itrecord;
     if (frame >= ev.max_frames) return;
     __syncthreads();
 #pragma unroll
@@ -746,6 +760,9 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
     for (int s = 0; s < SPB; s++) {
         const int shot = sh0 + s;
         if (shot >= g.nshots) break;
+        // idle slots: skipped with one shot per workgroup (the test in a
+        // two-shot loop costs these kernels registers); stepping one is harmless
+        if (SPB == 1 && ev.clock && shot_idle(ev.idle, shot)) continue;
         // buffer origin of this shot's arrays: element (row -HALO, column -HALO)
         const long org = (long)shot * g.sstride - (long)HALO * px - HALO;
         const BufRsrc bU = buf_rsrc(f.U + org), bW = buf_rsrc(f.W + org);
@@ -837,7 +854,7 @@ __global__ __launch_bounds__(TX *BY) void k_psv_velocity(PsvFields<S> f, PsvVelM
                 mx_w = fmaxf(mx_w, fabsf((float)nv[1][r]));
             }
         }
-        if (ev.clock) fused_record<C, RZ, 2>(ev, it, t.tile * g.nshots + shot, sXX, sZZ, nv, lz0);
+        if (ev.clock) fused_record<C, RZ, 2>(ev, it, t.tile * g.nshots + shot, sXX, sZZ, nv, lz0, ev.src[shot].t0);
     }
     if (ST::scaled) {
         amax_update(amax, SC_U, mx_u);
@@ -924,6 +941,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_stress(ShFields<S> f, ShStressMat
     for (int s = 0; s < SPB; s++) {
         const int shot = sh0 + s;
         if (shot >= g.nshots) break;
+        if (ev.clock && shot_idle(ev.idle, shot)) continue; // uniform per workgroup
         // buffer origin of this shot's arrays: element (row -HALO, column -HALO)
         const long org = (long)shot * g.sstride - (long)HALO * px - HALO;
         const BufRsrc bV = buf_rsrc(f.V + org), bX = buf_rsrc(f.Tx + org), bZ = buf_rsrc(f.Tz + org);
@@ -1095,6 +1113,9 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
     for (int s = 0; s < SPB; s++) {
         const int shot = sh0 + s;
         if (shot >= g.nshots) break;
+        // idle slots: skipped with one shot per workgroup (the test in a
+        // two-shot loop costs these kernels registers); stepping one is harmless
+        if (SPB == 1 && ev.clock && shot_idle(ev.idle, shot)) continue;
         const long base = (long)shot * g.sstride;
         // buffer origin of this shot's arrays: element (row -HALO, column -HALO)
         const long org = base - (long)HALO * px - HALO;
@@ -1198,7 +1219,7 @@ __global__ __launch_bounds__(TX *BY) void k_sh_velocity(ShFields<S> f, const typ
             ST::stb(bV, vcol, (unsigned)(gz + HALO) * pxb, nv[0][r], wi_v);
             if (ST::scaled && amax) mx_v = fmaxf(mx_v, fabsf((float)nv[0][r]));
         }
-        if (ev.clock) fused_record<C, RZ, 1>(ev, it, t.tile * g.nshots + shot, sX, sX, nv, lz0);
+        if (ev.clock) fused_record<C, RZ, 1>(ev, it, t.tile * g.nshots + shot, sX, sX, nv, lz0, ev.src[shot].t0);
     }
     if (ST::scaled) {
         amax_update(amax, SC_V, mx_v);
@@ -1250,18 +1271,20 @@ template <class S>
 __global__ void k_record(S *const *comps, int ncomp, long sstride, const long *__restrict__ roff,
                          const int *__restrict__ nrec, int max_rec, int nshots,
                          typename Store<S>::C *__restrict__ seis, int max_frames, int it, int itrecord,
-                         const float *__restrict__ scl, const int *__restrict__ comp_field) {
+                         const float *__restrict__ scl, const int *__restrict__ comp_field,
+                         const SrcInfo *__restrict__ src) {
     using C = typename Store<S>::C;
     using ST = Store<S>;
-    if (it % itrecord != 0) return;
-    const int frame = it / itrecord;
-    if (frame >= max_frames) return;
     const long total = (long)nshots * ncomp * max_rec;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
         const int r = (int)(i % max_rec);
         const int sc_ = (int)(i / max_rec);
         const int comp = sc_ % ncomp, shot = sc_ / ncomp;
         if (r >= nrec[shot * ncomp + comp]) continue;
+        // frames count from the shot's first step
+        const int ts = it - src[shot].t0;
+        if (ts < 0 || ts % itrecord != 0 || ts / itrecord >= max_frames) continue;
+        const int frame = ts / itrecord;
         const long o = roff[i];
         C v = C(0);
         if (o >= 0) {

@@ -149,6 +149,21 @@ int fd23_set_source(fd23_ctx *ctx, int shot, int kind, int xs, int zs,
 int fd23_set_receivers(fd23_ctx *ctx, int shot, int comp, const int *rx,
                        const int *rz, int n);
 
+/* Shot slots, for callers that start shots while a context runs (no
+ * counterpart in the reference, whose shots all start at step 0).  A shot
+ * with first step t0 injects its source sample i at step t0 + i and
+ * records its frame f at step t0 + f*itrecord, so its seismograms read
+ * back from frame 0 as if it had run alone from step 0.  An idle shot
+ * (active 0; shots 0..63) is not stepped by fd23_run.  Defaults: active,
+ * t0 = 0.  A change takes effect on the next fd23_run (its graphs are
+ * captured again).  Contexts with a box or a first-arrival window run
+ * every shot from step 0: fd23_run refuses other slots there.           */
+int fd23_set_shot_slot(fd23_ctx *ctx, int shot, int active, int t0);
+/* Zero one shot's fields, C-PML memory (both buffer sets) and seismogram
+ * entries, enqueued on the context stream: the state of a shot that has
+ * not started.                                                           */
+int fd23_reset_shot(fd23_ctx *ctx, int shot);
+
 /* ---- stepping --------------------------------------------------------
  * One stage each, in the reference's order (main.cpp:121-150):
  * cu_step_stress (fd/cu_fd.cu:391-483), source injection, cu_step_velocity

@@ -13,12 +13,18 @@
  *  - batching: queued requests with the same non-zero model_key and the
  *    same grid, time step, wavelet, moment tensor and box are merged into
  *    one context (one launch steps all their shots, reusing the material);
+ *    their numbers of steps may differ;
+ *  - joining: a request that arrives while a compatible context runs
+ *    takes that context's idle shot slots (left by requests that have
+ *    finished) at the next chunk boundary, its shots starting from rest at
+ *    that step (fd23_set_shot_slot); contexts with a box take no joins;
  *  - continuous scheduling: every resident context advances in chunks of
  *    chunk_steps graph-replayed steps on its own HIP stream, at most two
- *    chunks queued at a time, so requests that arrive while others run are
- *    admitted at the next chunk boundary and their work overlaps on the
- *    device; a context is retired, its seismograms copied out and its
- *    memory returned as soon as its last chunk completes.
+ *    chunks queued at a time and each ending where a request ends, so
+ *    requests that arrive while others run are admitted at the next chunk
+ *    boundary and their work overlaps on the device; a request's
+ *    seismograms are copied out as soon as its last step completes, and a
+ *    context is retired, its memory returned, when its last request ends.
  *
  * All device work runs on the engine's own thread; submit/wait/result are
  * safe to call from any thread.  Requests with FP16/BF16 storage and a box
@@ -84,6 +90,7 @@ typedef struct fd23_server_stats {
     long long peak_active;    /* most contexts resident at once            */
     size_t bytes_in_use, budget;
     double cell_updates;      /* completed shots * nx * nz * nt            */
+    long long joined;         /* requests that entered a running context   */
 } fd23_server_stats;
 
 int fd23_server_create(const fd23_server_desc *desc, fd23_server **out);

@@ -99,6 +99,36 @@ def test_merged_requests_share_a_launch():
     assert st["launches"] == 2 and st["merged"] == 2, st
 
 
+def test_a_request_joins_a_running_context():
+    """Requests on one problem with different numbers of steps share a
+    context; a request that arrives after the shorter one has finished takes
+    its idle shot slot while the longer one still runs (the only stream slot
+    is busy, so joining is its one way in).  Each result equals its alone
+    run."""
+    P, F32 = _lib.WAVE_PSV, _lib.STORE_F32
+    blocker = _requests()[5]
+    long_ = make_solve(P, F32, 256, 128, 8.0, 3000, 2, 31, tag="long")
+    short = make_solve(P, F32, 256, 128, 8.0, 200, 1, 31, tag="short")
+    late = make_solve(P, F32, 256, 128, 8.0, 300, 1, 31, tag="late")
+    short.xs = short.xs + 11
+    late.xs = late.xs + 23
+    alone = _alone([long_, short, late])
+    with Server(max_active=1, chunk_steps=64, max_batch_shots=16) as srv:
+        ids = {"blocker": srv.submit(blocker), "long": srv.submit(long_), "short": srv.submit(short)}
+        assert srv.wait(ids["short"], 120.0) == _lib.REQ_DONE
+        ids["late"] = srv.submit(late)
+        out = {}
+        for tag, rid in ids.items():
+            assert srv.wait(rid, 180.0) == _lib.REQ_DONE, tag
+            out[tag] = srv.result(rid)[0]
+        st = srv.stats()
+    print("stats", st, flush=True)
+    assert st["launches"] == 2 and st["merged"] == 2 and st["joined"] == 1, st
+    for s in (long_, short, late):
+        assert out[s.tag].shape == alone[s.tag].shape
+        np.testing.assert_array_equal(out[s.tag], alone[s.tag])
+
+
 def test_server_matches_direct_context():
     s = _requests()[2]
     with Server(max_active=1) as srv:
