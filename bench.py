@@ -358,15 +358,18 @@ def calibrate_copy(nbytes=1 << 30, reps=10):
     print(json.dumps(out), flush=True)
 
 
-def bench_serve(nreq, max_active, rate=40.0, models=0):
+def bench_serve(nreq, max_active, rate=40.0, models=0, spare=None):
     """Replay the mixed request stream through the serving engine (Poisson
     arrivals at `rate` per second; models > 0: shot gathers on that many
-    shared problems, see serving_stream)."""
+    shared problems, see serving_stream, whose contexts keep `spare` idle
+    shot slots (default 8) for gathers that arrive while they run)."""
     from fd23_amd import _lib
     from fd23_amd.serve import Server, replay, serving_stream
 
     stream = serving_stream(nreq, rate=rate, models=models)
-    with Server(max_active=max_active, chunk_steps=256, max_batch_shots=64) as srv:
+    if spare is None:
+        spare = 8 if models else 0
+    with Server(max_active=max_active, chunk_steps=256, max_batch_shots=64, spare_shots=spare) as srv:
         recs, wall = replay(srv, stream)
         st = srv.stats()
     ok = [r for r in recs if r.get("ok")]
@@ -388,7 +391,7 @@ def bench_serve(nreq, max_active, rate=40.0, models=0):
         "dtype": "f32",
         "data": "synthetic request stream (Poisson %g/s, seeded layered + GRF models)" % rate,
         "config": {"workload": "serving_mixed_sh_psv" if not models else "serving_shared_%d_models" % models,
-                   "requests": nreq, "max_active": max_active, "chunk_steps": 256, "rate_per_s": rate},
+                   "requests": nreq, "max_active": max_active, "spare_shots": spare, "chunk_steps": 256, "rate_per_s": rate},
         "requests_per_s": round(len(ok) / wall, 3),
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p99": round(float(np.percentile(lat, 99)), 4),
                       "max": round(float(lat.max()), 4)},
@@ -400,7 +403,7 @@ def bench_serve(nreq, max_active, rate=40.0, models=0):
         "completed": len(ok),
         "failed": len(recs) - len(ok),
         "finite": all(r["finite"] for r in ok),
-        "engine": {k: st[k] for k in ("launches", "merged", "peak_active")},
+        "engine": {k: st[k] for k in ("launches", "merged", "joined", "peak_active")},
         "mix": {"psv": sum(r["wave"] == _lib.WAVE_PSV for r in recs), "box": sum(r["box"] for r in recs),
                 "f16": sum(r["storage"] == _lib.STORE_F16 for r in recs)},
     }
@@ -480,6 +483,8 @@ def main():
     ap.add_argument("--max-active", type=int, default=8, help="serve: resident contexts")
     ap.add_argument("--rate", type=float, default=40.0, help="serve: Poisson arrivals per second")
     ap.add_argument("--models", type=int, default=0, help="serve: shot gathers on this many shared problems")
+    ap.add_argument("--spare-shots", type=int, default=None,
+                    help="serve: idle shot slots a context keeps for joining requests (default 8 with --models)")
     ap.add_argument("--storage", choices=sorted(STORAGE), default=None)
     ap.add_argument("--accuracy", action="store_true",
                     help="also report seismogram error against an FP64 run of the same inputs")
@@ -498,7 +503,8 @@ def main():
         calibrate_copy()
         return
     if args.workload == "serve":
-        print(json.dumps(bench_serve(args.requests, args.max_active, args.rate, args.models)), flush=True)
+        print(json.dumps(bench_serve(args.requests, args.max_active, args.rate, args.models, args.spare_shots)),
+              flush=True)
         return
     if args.workload == "material":
         print(json.dumps(bench_material()), flush=True)

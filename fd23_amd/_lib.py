@@ -36,7 +36,7 @@ class Desc(ctypes.Structure):
 
 class ServerDesc(ctypes.Structure):
     _fields_ = [("hbm_budget", c_sz), ("max_active", c_int), ("chunk_steps", c_int), ("max_batch_shots", c_int),
-                ("autotune", c_int)]
+                ("autotune", c_int), ("spare_shots", c_int)]
 
 
 class Request(ctypes.Structure):

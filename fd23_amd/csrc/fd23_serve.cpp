@@ -163,6 +163,13 @@ struct fd23_server {
     bool retire();
     int start_group(Group &g);
     int place(Group &g, Member &m); // sources and receivers of a member's slots
+    // shot slots of a new context for a batch of `shots` shots: spares for
+    // requests that may join (shared problem, no box), within the 64 slots
+    // the kernels can leave idle
+    int capacity(const Request &q, int shots) const {
+        if (d.spare_shots <= 0 || q.r.model_key == 0 || q.r.box || shots >= 64) return shots;
+        return std::min(shots + d.spare_shots, 64);
+    }
     int join(Group &g, const std::shared_ptr<Request> &q, const std::vector<int> &slots);
     int harvest(Group &g, Member &m);
     void finish_group(Group &g);
@@ -200,6 +207,8 @@ int fd23_server::start_group(Group &g) {
     if ((r = fd23_set_cpml(g.ctx, q0.cpml.data()))) return r;
     for (auto &m : g.mem)
         if ((r = place(g, m))) return r;
+    for (int s = 0; s < g.nshots; s++)
+        if (g.owner[s] == -1 && (r = fd23_set_shot_slot(g.ctx, s, 0, 0))) return r;
     const fd23_request &b = q0.r;
     if (b.box && (r = fd23_box_configure(g.ctx, -1, b.box_x0, b.box_z0, b.box_bx, b.box_bz, b.box_incident,
                                          b.box_angle, b.box_f0, b.box_t0, b.box_amp, b.box_vp, b.box_vs,
@@ -265,7 +274,7 @@ bool fd23_server::admit() {
             }
             int frames = 0;
             for (size_t k : pick) frames = std::max(frames, queue[k]->nframes);
-            const size_t need = batch_bytes(*queue[i], shots, frames);
+            const size_t need = batch_bytes(*queue[i], capacity(*queue[i], shots), frames);
             if (pick.size() == 1 && need > d.hbm_budget) {
                 // can never fit: fail it rather than let it wait forever
                 taken[i] = true;
@@ -332,8 +341,8 @@ bool fd23_server::admit() {
             g->mem.push_back(std::move(m));
             g->max_frames = std::max(g->max_frames, q->nframes);
         }
-        g->nshots = shot;
-        g->owner.assign(shot, 0);
+        g->nshots = capacity(*b[0], shot); // spare slots idle until a request joins
+        g->owner.assign(g->nshots, -1);
         for (size_t k = 0; k < g->mem.size(); k++)
             for (int s : g->mem[k].slots) g->owner[s] = (int)k;
         g->itrecord = b[0]->r.itrecord;
@@ -568,7 +577,8 @@ extern "C" int fd23_server_create(const fd23_server_desc *desc, fd23_server **ou
         fd23_set_error_("null argument");
         return FD23_EINVAL;
     }
-    if (desc->max_active < 1 || desc->max_active > 64 || desc->chunk_steps < 1 || desc->max_batch_shots < 1) {
+    if (desc->max_active < 1 || desc->max_active > 64 || desc->chunk_steps < 1 || desc->max_batch_shots < 1 ||
+        desc->spare_shots < 0) {
         fd23_set_error_("bad server description");
         return FD23_EINVAL;
     }

@@ -68,9 +68,11 @@ class Solve:
 
 
 class Server:
-    def __init__(self, hbm_budget=0, max_active=8, chunk_steps=256, max_batch_shots=64, autotune=False):
+    def __init__(self, hbm_budget=0, max_active=8, chunk_steps=256, max_batch_shots=64, autotune=False,
+                 spare_shots=0):
         self.lib = _lib.load()
-        d = _lib.ServerDesc(int(hbm_budget), max_active, chunk_steps, max_batch_shots, int(bool(autotune)))
+        d = _lib.ServerDesc(int(hbm_budget), max_active, chunk_steps, max_batch_shots, int(bool(autotune)),
+                            int(spare_shots))
         h = ctypes.c_void_p()
         check(self.lib.fd23_server_create(ctypes.byref(d), ctypes.byref(h)))
         self.h = h

@@ -49,6 +49,10 @@ typedef struct fd23_server_desc {
     int chunk_steps;     /* steps per scheduling chunk (>= 1)                */
     int max_batch_shots; /* shots merged into one context (>= 1)             */
     int autotune;        /* 1 = tune the tile once per (grid, shots) shape   */
+    int spare_shots;     /* idle shot slots a new context keeps for requests
+                          * that join it while it runs (requests with a
+                          * non-zero model_key and no box; a context holds
+                          * at most 64 shots with spares); 0 = none         */
 } fd23_server_desc;
 
 /* One solve.  All pointers are read during fd23_server_submit only (the

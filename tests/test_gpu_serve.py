@@ -129,6 +129,34 @@ def test_a_request_joins_a_running_context():
         np.testing.assert_array_equal(out[s.tag], alone[s.tag])
 
 
+def test_an_arrival_joins_spare_slots():
+    """A context for a shared problem keeps idle shot slots (spare_shots);
+    a gather that arrives while it runs starts in them at the next chunk
+    boundary instead of waiting for a stream slot.  Both results equal their
+    alone runs."""
+    P, F32 = _lib.WAVE_PSV, _lib.STORE_F32
+    first = make_solve(P, F32, 256, 128, 8.0, 6000, 2, 33, tag="first")
+    later = make_solve(P, F32, 256, 128, 8.0, 700, 3, 33, tag="later")
+    later.xs = later.xs + 5
+    alone = _alone([first, later])
+    with Server(max_active=1, chunk_steps=64, max_batch_shots=16, spare_shots=4) as srv:
+        a = srv.submit(first)
+        t0 = time.time()
+        while srv.wait(a, 0.0) == _lib.REQ_QUEUED:  # until the first request is admitted
+            assert time.time() - t0 < 60.0
+            time.sleep(0.001)
+        b = srv.submit(later)
+        out = {}
+        for tag, rid in (("first", a), ("later", b)):
+            assert srv.wait(rid, 180.0) == _lib.REQ_DONE, tag
+            out[tag] = srv.result(rid)[0]
+        st = srv.stats()
+    print("stats", st, flush=True)
+    assert st["launches"] == 1 and st["joined"] == 1, st
+    for s in (first, later):
+        np.testing.assert_array_equal(out[s.tag], alone[s.tag])
+
+
 def test_server_matches_direct_context():
     s = _requests()[2]
     with Server(max_active=1) as srv:
