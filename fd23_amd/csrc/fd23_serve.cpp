@@ -172,6 +172,7 @@ struct fd23_server {
     }
     int join(Group &g, const std::shared_ptr<Request> &q, const std::vector<int> &slots);
     int harvest(Group &g, Member &m);
+    int quiet(Group &g, int shot);
     void finish_group(Group &g);
     void complete(Request &q, bool failed_ctx);
 };
@@ -436,6 +437,16 @@ bool fd23_server::schedule() {
     return any;
 }
 
+// an idle slot: zero state and no source samples, so that it stays zero on
+// the paths that still step idle slots (shared FP16/BF16 scales never see
+// it), and skipped where the kernels can
+int fd23_server::quiet(Group &g, int shot) {
+    int r;
+    if ((r = fd23_reset_shot(g.ctx, shot))) return r;
+    if ((r = fd23_set_source(g.ctx, shot, FD23_SRC_LINE, 0, 0, nullptr, 0, nullptr))) return r;
+    return fd23_set_shot_slot(g.ctx, shot, 0, 0);
+}
+
 // copy a finished member's seismograms out
 int fd23_server::harvest(Group &g, Member &m) {
     Request &q = *m.q;
@@ -522,7 +533,7 @@ bool fd23_server::retire() {
                 for (int s : m.slots) {
                     g.owner[s] = -1;
                     // a box context runs every shot (and takes no joins)
-                    if (!r && !g.box) r = fd23_set_shot_slot(g.ctx, s, 0, 0);
+                    if (!r && !g.box) r = quiet(g, s);
                 }
                 if (r) {
                     m.q->err = r;

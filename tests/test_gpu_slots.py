@@ -99,3 +99,27 @@ def test_slot_arguments_are_checked():
         assert lib.fd23_set_shot_slot(ctx.h, 0, 1, -1) == _lib.EINVAL
         assert lib.fd23_set_shot_slot(ctx.h, 1, 1, 0) == _lib.EINVAL   # one shot only
         assert lib.fd23_reset_shot(ctx.h, 3) == _lib.EINVAL
+
+
+@pytest.mark.parametrize("storage", [_lib.STORE_F16, _lib.STORE_F32])
+def test_a_quieted_slot_stays_zero(storage):
+    """The serving engine quiets the slot of a finished request: zero state
+    (fd23_reset_shot), no source samples, idle.  On the paths that still
+    step idle slots (velocity kernels with two shots per workgroup) its
+    state must stay exactly zero, so FP16/BF16 scales, shared by the shots
+    of a context, never see it."""
+    tile = _tile("two shots per workgroup")
+    print("case: quiet slot, storage", storage, "tile", tile, flush=True)
+    p = Problem(_lib.WAVE_PSV, 130, 97, 10, 10.0, 7e-4, 130, seed=43, sources=[(40, 9), (90, 12)])
+    with p.context(storage=storage, max_frames=60) as ctx:
+        ctx.tile = tile
+        ctx.run(0, 50, ITREC)              # both shots carry a wavefield
+        ctx.reset_shot(1)
+        ctx.set_source(1, _lib.SRC_LINE, 0, 0, np.zeros(0), np.zeros(3))
+        ctx.set_shot_slot(1, False)
+        ctx.run(50, 60, ITREC)
+        ctx.sync()
+        st1 = np.frombuffer(ctx.get_state(1), np.float32)  # the compute type of both storages
+        st0 = np.frombuffer(ctx.get_state(0), np.float32)
+    assert np.abs(st0).max() > 0
+    assert np.all(st1 == 0)
