@@ -176,11 +176,11 @@ int fd23_record(fd23_ctx *ctx, int it, int itrecord);
 
 /* The whole loop for it in [it0, it0+nt): stress, inject, velocity,
  * record every itrecord steps -- captured in HIP graphs and replayed
- * (32-step chunks plus one graph of the remaining nt % 32 steps).
+ * (128-step chunks plus one graph of the remaining nt % 128 steps).
  * Equivalent to the singlegpu() loop body (main.cpp:115-168).           */
 int fd23_run(fd23_ctx *ctx, int it0, int nt, int itrecord);
 /* Captures and instantiates the HIP graphs fd23_run(ctx, *, nt, itrecord)
- * replays (a 32-step chunk and one of nt % 32 steps), without running
+ * replays (a 128-step chunk and one of nt % 128 steps), without running
  * them, so that run spends no time on capture; no reference counterpart
  * (the reference launches every kernel from the host).                 */
 int fd23_prepare(fd23_ctx *ctx, int nt, int itrecord);

@@ -158,7 +158,8 @@ def test_graph_run_equals_staged_calls(wave, kind):
     """fd23_run (graph loop, injection and gather fused into the passes) is
     bitwise the staged reference order stress / inject / velocity / record."""
     _print("case: graph vs staged", wave, kind)
-    p = Problem(wave, 130, 70, 10, 10.0, 7e-4, 70, seed=5, kind=kind,
+    # 140 steps: one full 128-step graph chunk and a graph of the rest
+    p = Problem(wave, 130, 70, 10, 10.0, 7e-4, 140, seed=5, kind=kind,
                 sources=[(60, 6), (61, 30)], rec_z=4)
     with p.context() as a, p.context() as b:
         a.run(0, p.nt, 2)
@@ -212,7 +213,7 @@ def test_fused_step_bitwise_equals_two_pass(nx, nz, npml, top, kind, storage):
     strip edges (strips own 56 columns)."""
     _print("case: fused", nx, nz, npml, top, kind, storage)
     srcs = [(nx // 2, 7), (5, 3), (nx - 6, nz // 2)]
-    p = Problem(_lib.WAVE_PSV, nx, nz, npml, 10.0, 7e-4, 75, seed=21, topabsorb=top, kind=kind,
+    p = Problem(_lib.WAVE_PSV, nx, nz, npml, 10.0, 7e-4, 171, seed=21, topabsorb=top, kind=kind,
                 sources=srcs, rec_z=npml + 3 if top else 2, f64=storage == _lib.STORE_F64)
     p.rx = np.concatenate([p.rx, np.array([0, nx - 1, 55, 56, 111, 112], np.int32)])
     p.rz = np.concatenate([p.rz, np.array([nz - 1, 0, nz // 2, nz // 2, 60, 61], np.int32)])
@@ -220,15 +221,15 @@ def test_fused_step_bitwise_equals_two_pass(nx, nz, npml, top, kind, storage):
     def run(tile, staged_tail):
         with p.context(storage=storage) as ctx:
             ctx.tile = tile
-            ctx.run(0, 37, 1)
-            ctx.run(37, 28, 1)
-            for it in range(65, 65 + staged_tail):
+            ctx.run(0, 133, 1)   # one 128-step chunk and 5 steps
+            ctx.run(133, 28, 1)
+            for it in range(161, 161 + staged_tail):
                 ctx.step_stress()
                 ctx.inject(it)
                 ctx.step_velocity()
                 ctx.record(it)
             if staged_tail < 10:
-                ctx.run(65 + staged_tail, 10 - staged_tail, 1)
+                ctx.run(161 + staged_tail, 10 - staged_tail, 1)
             ctx.sync()
             st = b"".join(ctx.get_state(s) for s in range(len(srcs)))
             seis = [ctx.seismograms(s, c, p.nt) for s in range(len(srcs)) for c in range(2)]
@@ -264,7 +265,7 @@ def test_sh_block_step_bitwise_equals_two_pass(nx, nz, npml, top, kind, storage)
     sources and receivers on tile edges, grid edges and the surface."""
     _print("case: sh block", nx, nz, npml, top, kind, storage)
     srcs = [(nx // 2, 7), (3, 2), (nx - 5, nz // 2)]
-    p = Problem(_lib.WAVE_SH, nx, nz, npml, 10.0, 7e-4, 90, seed=31, topabsorb=top, kind=kind,
+    p = Problem(_lib.WAVE_SH, nx, nz, npml, 10.0, 7e-4, 186, seed=31, topabsorb=top, kind=kind,
                 sources=srcs, rec_z=npml + 2 if top else 1, f64=storage == _lib.STORE_F64)
     # tile edges of the block shapes: 50 or 36 columns, 10/18/20/34/36 rows
     ex = np.array([0, nx - 1, 49, 50, 35, 36, 99, 100, 3, 5, 17, 18], np.int32) % nx
@@ -275,15 +276,15 @@ def test_sh_block_step_bitwise_equals_two_pass(nx, nz, npml, top, kind, storage)
     def run(tile):
         with p.context(storage=storage) as ctx:
             ctx.tile = tile
-            ctx.run(0, 37, 1)    # one 32-step chunk and 5 steps
-            ctx.run(37, 28, 1)
-            for it in range(65, 68):
+            ctx.run(0, 133, 1)   # one 128-step chunk and 5 steps
+            ctx.run(133, 28, 1)
+            for it in range(161, 164):
                 ctx.step_stress()
                 ctx.inject(it)
                 ctx.step_velocity()
                 ctx.record(it)
-            ctx.run(68, 7, 1)
-            for it0 in (75, 80, 85):  # the same 5-step chunk three times
+            ctx.run(164, 7, 1)
+            for it0 in (171, 176, 181):  # the same 5-step chunk three times
                 ctx.run(it0, 5, 1)
             ctx.sync()
             st = b"".join(ctx.get_state(s) for s in range(len(srcs)))
