@@ -104,6 +104,7 @@ def run_gpu(wl, steps, warmup, autotune=True, tile=None, passes=True):
     elif autotune:
         _, tune_ms = ctx.autotune(16)
     tile = ctx.tile
+    chains = ctx.chains  # two chains of passes over the two halves of the shots
     # the graphs of both runs are captured before any step runs
     ctx.prepare(warmup)
     ctx.prepare(steps)
@@ -125,7 +126,7 @@ def run_gpu(wl, steps, warmup, autotune=True, tile=None, passes=True):
     s_ms, v_ms = ctx.time_passes(20) if passes else (None, None)
     ctx.close()
     return dict(elapsed=el, stress_ms=s_ms, vel_ms=v_ms, tile=tile, tune_ms=tune_ms, finite=finite, fused=fused,
-                block=block, kernels=names)
+                block=block, kernels=names, chains=chains)
 
 
 STORE_TYPE = {0: "float", 1: "__half", 2: "__hip_bfloat16", 3: "double"}
@@ -581,7 +582,8 @@ def measure_line(wl, steps, warmup, autotune=True, tile=None):
         "storage": sname,
         "data": "synthetic (seeded layered + GRF model, Ricker source)",
         "config": {"workload": workload_id(wl), "nx": wl.nx, "nz": wl.nz, "nshots": wl.nshots,
-                   "npml": wl.npml, "dx": wl.dx, "dt": wl.dt, "nt": wl.nt, "tile": r["tile"]},
+                   "npml": wl.npml, "dx": wl.dx, "dt": wl.dt, "nt": wl.nt, "tile": r["tile"],
+                   "two_chains": r["chains"]},
         "shots_per_s": round(wl.nshots / (ms * 1e-3 * wl.nt), 4),
         "hbm_fraction_step": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "bytes_per_step": step_bytes,
@@ -615,7 +617,8 @@ def bench_batching(steps, warmup, single_steps=512):
     b = measure_line(make_workload("batched").build(), steps, warmup)
     one = measure_line(make_workload("single1024").build(), single_steps, warmup)
     return {"workload": "psv_1024x512_x64_fp32", "shots_per_s": b["shots_per_s"], "value": b["value"],
-            "ms_per_step": b["ms_per_step"], "steps": steps, "roofline_frac": b["roofline"]["frac"],
+            "ms_per_step": b["ms_per_step"], "steps": steps, "tile": b["config"]["tile"],
+            "two_chains": b["config"]["two_chains"], "roofline_frac": b["roofline"]["frac"],
             "kernel_ms": b["roofline"]["kernel_ms"], "kernel_name": b["roofline"]["kernel_name"],
             "single_shot": {"shots_per_s": one["shots_per_s"], "value": one["value"],
                             "ms_per_step": one["ms_per_step"], "steps": single_steps},

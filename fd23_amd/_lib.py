@@ -107,6 +107,8 @@ SIGNATURES = {
     "fd23_tile_kind": (c_int, [c_int]),
     "fd23_tile_split": (c_int, [c_int]),
     "fd23_autotune": (c_int, [c_vp, c_int, P_dbl]),
+    "fd23_set_chains": (c_int, [c_vp, c_int]),
+    "fd23_get_chains": (c_int, [c_vp]),
     "fd23_bytes_per_step": (c_dbl, [c_vp]),
     "fd23_device_copy": (c_int, [c_vp, c_vp, c_sz, c_int, c_vp]),
     "fd23_device_stream_tiles": (c_int, [c_vp, c_int, c_vp, c_int, c_int, c_int, c_vp]),

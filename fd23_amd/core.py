@@ -263,6 +263,16 @@ class Context:
             check(k)
         return bool(k)
 
+    @property
+    def chains(self):
+        """True when run() captures two chains of passes over the two halves
+        of the shot groups."""
+        return self.lib.fd23_get_chains(self.h) == 1
+
+    @chains.setter
+    def chains(self, on):
+        check(self.lib.fd23_set_chains(self.h, 1 if on else 0))
+
     def autotune(self, steps=4):
         best = ctypes.c_double()
         check(self.lib.fd23_autotune(self.h, steps, ctypes.byref(best)))

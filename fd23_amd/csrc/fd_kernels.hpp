@@ -66,6 +66,7 @@ struct Work {
     int ix0, ix1, iz0, iz1; // interior rectangle (tile indices)
     int ntiles;            // tiles of this launch's region
     int nwork;             // ntiles*ngroups (logical workgroups)
+    int g0;                // first shot group of the launch
 };
 struct TileId {
     int tile, group, x0, z0; // tile: index in the whole grid (ntx per row)
@@ -76,8 +77,9 @@ template <int TZ, int REGION = REGION_ALL> __device__ __forceinline__ bool map_t
     if (L >= w.nwork) return false;
     // tiles of one shot group are consecutive, so neighbouring tiles (which
     // read each other's halos) run together on one XCD and share its L2
-    t.group = L / w.ntiles;
-    int k = L - t.group * w.ntiles;
+    const int gl = L / w.ntiles;
+    int k = L - gl * w.ntiles;
+    t.group = w.g0 + gl;
     int tx, tz;
     if (REGION == REGION_INTERIOR) {
         const int wx = w.ix1 - w.ix0;

@@ -193,6 +193,34 @@ def test_all_tiles_bitwise_equal():
         assert st == ref
 
 
+def test_two_chains_bitwise_equal_one_chain():
+    """Chunks captured as two chains of passes over the two halves of the
+    shot groups give states and seismograms bitwise equal to one chain,
+    on unsplit, split and two-shots-per-workgroup shapes, over a full
+    128-step chunk and a remainder, with 5 shots (halves of unequal size)."""
+    lib = _lib.load()
+    srcs = [(30, 6), (60, 9), (90, 4), (120, 12), (45, 20)]
+    p = Problem(_lib.WAVE_PSV, 150, 90, 12, 10.0, 7e-4, 150, seed=17, kind=_lib.SRC_P, sources=srcs, rec_z=3)
+    # RZ 2 and 4 unsplit, 4 shots per workgroup (two groups: 4 and 1 shots),
+    # RZ 2 split, 2 shots per workgroup split
+    for t in (0, 1, 4, 14, 16):
+        assert lib.fd23_tile_kind(t) == _lib.TILE_TILED
+        out = []
+        for on in (False, True):
+            with p.context() as ctx:
+                ctx.tile = t
+                ctx.chains = on
+                ctx.run(0, p.nt, 1)
+                assert ctx.chains == on
+                ctx.sync()
+                out.append((b"".join(ctx.get_state(s) for s in range(len(srcs))),
+                            [ctx.seismograms(s, c, p.nt) for s in range(len(srcs)) for c in range(2)]))
+        _print("chains tile", t, out[0][0] == out[1][0])
+        assert out[0][0] == out[1][0]
+        for a_, b_ in zip(out[0][1], out[1][1]):
+            assert np.array_equal(a_, b_)
+
+
 def _fused_tiles():
     """Fused P-SV shapes: the strip-marching step (segments of 32/64/128 rows
     in blocks of 4, 64 rows in 2) and the tile-parallel step (segments of
