@@ -2124,6 +2124,7 @@ extern "C" int fd23_autotune(fd23_ctx *c, int steps, double *best_ms) {
     const int orig = c->tile, orig_set = c->cur;
     int best = orig;
     double bt = 1e30;
+    std::vector<std::pair<double, int>> timed; // pass time, shape
     for (int t = 0; t < kNumTiles; t++) {
         if (kTiles[t].spb > 1 && c->d.nshots < kTiles[t].spb) continue;
         // shapes that would run as another one: SH and a first-arrival
@@ -2142,24 +2143,39 @@ extern "C" int fd23_autotune(fd23_ctx *c, int steps, double *best_ms) {
         for (int rep = 0; rep < 2 && r == FD23_OK; rep++)
             if ((r = fd23_time_passes(c, steps, &a, &b)) == FD23_OK) tt = std::min(tt, a + b);
         if (r) break;
+        timed.emplace_back(tt, t);
         if (tt < bt) {
             bt = tt;
             best = t;
         }
     }
-    // two chains for the best shape, when they apply: kept if 1% faster
+    // Two chains, when the best shape can take them: the shapes within 5%
+    // of it (at most three) run 32 graph-replayed steps with one chain and
+    // with two, and the fastest pair wins, two chains by a 1% margin
     c->chains = 0;
-    if (r == FD23_OK) {
-        c->tile = best;
-        if (c->cur != orig_set) swap_sets(c);
-        if (chains_possible(c)) {
-            double off = 0, on = 0;
-            if ((r = time_chunk(c, 32, &off)) == FD23_OK) {
-                c->chains = 1;
-                r = time_chunk(c, 32, &on);
+    if (c->cur != orig_set) swap_sets(c);
+    c->tile = best;
+    if (r == FD23_OK && chains_possible(c)) {
+        std::sort(timed.begin(), timed.end());
+        double top = 1e30;
+        int top_t = best, top_ch = 0, n = 0;
+        for (const auto &ct : timed) {
+            if (n == 3 || ct.first > 1.05 * bt || r) break;
+            c->tile = ct.second;
+            if (!chains_possible(c)) continue;
+            n++;
+            for (int ch = 0; ch < 2 && r == FD23_OK; ch++) {
+                c->chains = ch;
+                double ms = 0;
+                if ((r = time_chunk(c, 32, &ms)) == FD23_OK && ms * (ch ? 1.01 : 1.0) < top) {
+                    top = ms * (ch ? 1.01 : 1.0);
+                    top_t = ct.second;
+                    top_ch = ch;
+                }
             }
-            c->chains = (r == FD23_OK && on < 0.99 * off) ? 1 : 0;
         }
+        best = top_t;
+        c->chains = r == FD23_OK ? top_ch : 0;
     }
     // restore (the fused candidates may have left the other set current)
     if (c->cur != orig_set) swap_sets(c);
