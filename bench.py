@@ -104,7 +104,7 @@ def run_gpu(wl, steps, warmup, autotune=True, tile=None, passes=True):
     elif autotune:
         _, tune_ms = ctx.autotune(16)
     tile = ctx.tile
-    chains = ctx.chains  # two chains of passes over the two halves of the shots
+    chains = ctx.chains  # independent chains of passes over parts of the shots
     # the graphs of both runs are captured before any step runs
     ctx.prepare(warmup)
     ctx.prepare(steps)
@@ -583,7 +583,7 @@ def measure_line(wl, steps, warmup, autotune=True, tile=None):
         "data": "synthetic (seeded layered + GRF model, Ricker source)",
         "config": {"workload": workload_id(wl), "nx": wl.nx, "nz": wl.nz, "nshots": wl.nshots,
                    "npml": wl.npml, "dx": wl.dx, "dt": wl.dt, "nt": wl.nt, "tile": r["tile"],
-                   "two_chains": r["chains"]},
+                   "chains": r["chains"]},
         "shots_per_s": round(wl.nshots / (ms * 1e-3 * wl.nt), 4),
         "hbm_fraction_step": round(step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "bytes_per_step": step_bytes,
@@ -618,7 +618,7 @@ def bench_batching(steps, warmup, single_steps=512):
     one = measure_line(make_workload("single1024").build(), single_steps, warmup)
     return {"workload": "psv_1024x512_x64_fp32", "shots_per_s": b["shots_per_s"], "value": b["value"],
             "ms_per_step": b["ms_per_step"], "steps": steps, "tile": b["config"]["tile"],
-            "two_chains": b["config"]["two_chains"], "roofline_frac": b["roofline"]["frac"],
+            "chains": b["config"]["chains"], "roofline_frac": b["roofline"]["frac"],
             "kernel_ms": b["roofline"]["kernel_ms"], "kernel_name": b["roofline"]["kernel_name"],
             "single_shot": {"shots_per_s": one["shots_per_s"], "value": one["value"],
                             "ms_per_step": one["ms_per_step"], "steps": single_steps},

@@ -265,13 +265,13 @@ class Context:
 
     @property
     def chains(self):
-        """True when run() captures two chains of passes over the two halves
-        of the shot groups."""
-        return self.lib.fd23_get_chains(self.h) == 1
+        """Number of independent chains of passes run() captures, each over
+        a part of the shot groups (1: one chain)."""
+        return self.lib.fd23_get_chains(self.h)
 
     @chains.setter
-    def chains(self, on):
-        check(self.lib.fd23_set_chains(self.h, 1 if on else 0))
+    def chains(self, n):
+        check(self.lib.fd23_set_chains(self.h, int(n)))
 
     def autotune(self, steps=4):
         best = ctypes.c_double()

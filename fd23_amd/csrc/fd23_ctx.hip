@@ -111,10 +111,13 @@ struct fd23_ctx {
     // interior kernel (fork and join through two events)
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    // two-chain capture (fd23_set_chains, chosen by the autotuner): the
-    // launches of a chain cover half the shot groups; -1 all of them
+    // chains (fd23_set_chains, chosen by the autotuner): the launches of
+    // chain i of n cover the i-th of n parts of the shot groups, on the
+    // context stream (i = 0) or chain_stream[i-1]; chain -1: all groups
+    hipStream_t chain_stream[3] = {};
+    hipEvent_t chain_join[3] = {};
     int chain = -1;
-    int chains = 0;
+    int chains = 1;
     bool timing = false; // graphs captured for timing: no injection, no recording
     int esize = 4;     // bytes per stored field/psi element
     int csize = 4;     // bytes per compute/material element
@@ -311,6 +314,10 @@ static void free_ctx(fd23_ctx *c) {
     for (auto p : others) if (p) hipFree(p);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
     if (c->side) hipStreamDestroy(c->side);
+    for (int i = 0; i < 3; i++) {
+        if (c->chain_stream[i]) hipStreamDestroy(c->chain_stream[i]);
+        if (c->chain_join[i]) hipEventDestroy(c->chain_join[i]);
+    }
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
     if (c->ev_join) hipEventDestroy(c->ev_join);
     delete c;
@@ -368,7 +375,13 @@ extern "C" int fd23_ctx_create(const fd23_desc *desc, fd23_ctx **out) {
     }
     if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->chain_stream[0], hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->chain_stream[1], hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->chain_stream[2], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->chain_join[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->chain_join[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->chain_join[2], hipEventDisableTiming) != hipSuccess) {
         free_ctx(c);
         return fail(FD23_EHIP, "stream creation failed");
     }
@@ -1043,10 +1056,10 @@ static Work make_work(const fd23_ctx *c, int RZ, int SPB, dim3 &grid, int region
     w.ntz = (c->d.nz + TZ - 1) / TZ;
     w.ngroups = (c->d.nshots + SPB - 1) / SPB;
     w.g0 = 0;
-    if (c->chain >= 0) { // half of the shot groups
-        const int h = (w.ngroups + 1) / 2;
-        w.g0 = c->chain ? h : 0;
-        w.ngroups = c->chain ? w.ngroups - h : h;
+    if (c->chain >= 0) { // part c->chain of c->chains of the shot groups
+        const int h = (w.ngroups + c->chains - 1) / c->chains;
+        w.g0 = std::min(c->chain * h, w.ngroups);
+        w.ngroups = std::min(h, w.ngroups - w.g0);
     }
     const int lo = std::max(c->d.npml, 4), ztop = c->d.topabsorb ? lo : 4;
     w.ix0 = (lo + TX - 1) / TX;
@@ -1688,11 +1701,12 @@ static int enqueue_launch(fd23_ctx *c, int k, int left, int itrecord, int *taken
 // 64-shot 1024x512 batch: +3.7%; the 8-shot 4096x2048 loop loses 3.5%, so
 // the autotuner decides).  The two-pass path with more than one shot group
 // and no state shared by the halves (scales, box, window).
-static bool chains_possible(const fd23_ctx *c) {
+static bool chains_possible(const fd23_ctx *c, int n = 2) {
     if (use_fused(c) || use_block(c) || scaled_storage(c) || c->box || c->punch) return false;
-    return c->d.nshots > kTiles[pass_tile(c)].spb;
+    const int spb = kTiles[pass_tile(c)].spb;
+    return (c->d.nshots + spb - 1) / spb >= n; // a shot group per chain at least
 }
-static bool use_chains(const fd23_ctx *c) { return c->chains && chains_possible(c); }
+static bool use_chains(const fd23_ctx *c) { return c->chains > 1 && chains_possible(c, c->chains); }
 
 static int get_graph(fd23_ctx *c, int itrecord, int steps, fd23_ctx::Chunk *out) {
     const auto key = std::make_tuple(itrecord, steps, c->cur);
@@ -1706,22 +1720,24 @@ static int get_graph(fd23_ctx *c, int itrecord, int steps, fd23_ctx::Chunk *out)
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int r = FD23_OK, taken = 1;
     if (use_chains(c)) {
-        // two independent chains of passes over the two halves of the shot
-        // groups, forked at the chunk's start and joined before its tick
+        // independent chains of passes over parts of the shot groups,
+        // forked at the chunk's start and joined before its tick
+        const int n = c->chains;
         hipEventRecord(c->ev_fork, c->stream);
-        hipStreamWaitEvent(c->side, c->ev_fork, 0);
+        for (int i = 0; i < n - 1; i++) hipStreamWaitEvent(c->chain_stream[i], c->ev_fork, 0);
         const hipStream_t main = c->stream;
-        for (int s = 0; s < steps && r == FD23_OK; s++) {
-            c->chain = 0;
-            r = enqueue_step(c, s, itrecord);
-            c->stream = c->side;
-            c->chain = 1;
-            if (r == FD23_OK) r = enqueue_step(c, s, itrecord);
-            c->stream = main;
-        }
+        for (int s = 0; s < steps && r == FD23_OK; s++)
+            for (int i = 0; i < n && r == FD23_OK; i++) {
+                c->chain = i;
+                c->stream = i ? c->chain_stream[i - 1] : main;
+                r = enqueue_step(c, s, itrecord);
+                c->stream = main;
+            }
         c->chain = -1;
-        hipEventRecord(c->ev_join, c->side);
-        hipStreamWaitEvent(c->stream, c->ev_join, 0);
+        for (int i = 0; i < n - 1; i++) {
+            hipEventRecord(c->chain_join[i], c->chain_stream[i]);
+            hipStreamWaitEvent(c->stream, c->chain_join[i], 0);
+        }
     } else {
         for (int s = 0; s < steps && r == FD23_OK; s += taken) r = enqueue_launch(c, s, steps - s, itrecord, &taken);
     }
@@ -2080,15 +2096,17 @@ static int time_chunk(fd23_ctx *c, int n, double *ms) {
     return r;
 }
 
-extern "C" int fd23_set_chains(fd23_ctx *c, int on) {
+extern "C" int fd23_set_chains(fd23_ctx *c, int n) {
     if (!c) return fail(FD23_EINVAL, "null context");
-    if ((on != 0) != (c->chains != 0)) {
-        c->chains = on != 0;
+    if (n < 0 || n > 4 || n == 3) return fail(FD23_EINVAL, "chains: 0, 1, 2 or 4");
+    n = std::max(n, 1);
+    if (n != c->chains) {
+        c->chains = n;
         fd23_ctx_invalidate_graphs(c);
     }
     return FD23_OK;
 }
-extern "C" int fd23_get_chains(const fd23_ctx *c) { return c ? (use_chains(c) ? 1 : 0) : -1; }
+extern "C" int fd23_get_chains(const fd23_ctx *c) { return c ? (use_chains(c) ? c->chains : 1) : -1; }
 
 extern "C" int fd23_autotune(fd23_ctx *c, int steps, double *best_ms) {
     int r = ready(c);
@@ -2152,30 +2170,32 @@ extern "C" int fd23_autotune(fd23_ctx *c, int steps, double *best_ms) {
     // Two chains, when the best shape can take them: the shapes within 5%
     // of it (at most three) run 32 graph-replayed steps with one chain and
     // with two, and the fastest pair wins, two chains by a 1% margin
-    c->chains = 0;
+    c->chains = 1;
     if (c->cur != orig_set) swap_sets(c);
     c->tile = best;
     if (r == FD23_OK && chains_possible(c)) {
         std::sort(timed.begin(), timed.end());
         double top = 1e30;
-        int top_t = best, top_ch = 0, n = 0;
+        int top_t = best, top_ch = 1, n = 0;
         for (const auto &ct : timed) {
             if (n == 3 || ct.first > 1.05 * bt || r) break;
             c->tile = ct.second;
             if (!chains_possible(c)) continue;
             n++;
-            for (int ch = 0; ch < 2 && r == FD23_OK; ch++) {
+            for (int ch : {1, 2, 4}) {
+                if (r != FD23_OK || !chains_possible(c, ch)) continue;
                 c->chains = ch;
                 double ms = 0;
-                if ((r = time_chunk(c, 32, &ms)) == FD23_OK && ms * (ch ? 1.01 : 1.0) < top) {
-                    top = ms * (ch ? 1.01 : 1.0);
+                const double margin = ch > 1 ? 1.01 : 1.0;
+                if ((r = time_chunk(c, 32, &ms)) == FD23_OK && ms * margin < top) {
+                    top = ms * margin;
                     top_t = ct.second;
                     top_ch = ch;
                 }
             }
         }
         best = top_t;
-        c->chains = r == FD23_OK ? top_ch : 0;
+        c->chains = r == FD23_OK ? top_ch : 1;
     }
     // restore (the fused candidates may have left the other set current)
     if (c->cur != orig_set) swap_sets(c);

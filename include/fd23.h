@@ -296,15 +296,15 @@ int fd23_tile_kind(int tile);
  * registers per lane, one more launch per pass); 0 otherwise           */
 int fd23_tile_split(int tile);
 int fd23_autotune(fd23_ctx *ctx, int steps, double *best_ms);
-/* Two chains (no reference counterpart): fd23_run captures each chunk's
- * steps as two independent chains of passes over the two halves of the
- * shot groups, on the context stream and a second stream, so that one
- * chain's kernels fill the other's gaps between dependent passes.  Applies
- * to the two-pass path with more than one shot group and FP32/FP64
- * storage, no box and no first-arrival window; results are bitwise the
- * same.  fd23_autotune turns it on when it times faster for the chosen
- * tile shape.  fd23_get_chains: 1 when fd23_run uses it.                */
-int fd23_set_chains(fd23_ctx *ctx, int on);
+/* Chains (no reference counterpart): fd23_run captures each chunk's steps
+ * as n = 2 or 4 independent chains of passes, chain i over the i-th of n
+ * parts of the shot groups, on the context stream and n-1 more streams,
+ * so that one chain's kernels fill the others' gaps between dependent
+ * passes.  Applies to the two-pass path with at least n shot groups and
+ * FP32/FP64 storage, no box and no first-arrival window; results are
+ * bitwise the same.  n = 0 or 1: one chain.  fd23_autotune chooses n with
+ * the tile shape.  fd23_get_chains: the number fd23_run uses.           */
+int fd23_set_chains(fd23_ctx *ctx, int n);
 int fd23_get_chains(const fd23_ctx *ctx);
 /* FP16/BF16 storage: per-array power-of-two scales (stored = value/scale);
  * P-SV order U W Txx Tzz Txz then psi Txx_x Txz_x U_x W_x Tzz_z Txz_z U_z

@@ -193,11 +193,12 @@ def test_all_tiles_bitwise_equal():
         assert st == ref
 
 
-def test_two_chains_bitwise_equal_one_chain():
-    """Chunks captured as two chains of passes over the two halves of the
-    shot groups give states and seismograms bitwise equal to one chain,
-    on unsplit, split and two-shots-per-workgroup shapes, over a full
-    128-step chunk and a remainder, with 5 shots (halves of unequal size)."""
+def test_chains_bitwise_equal_one_chain():
+    """Chunks captured as 2 or 4 chains of passes over parts of the shot
+    groups give states and seismograms bitwise equal to one chain, on
+    unsplit, split and several-shots-per-workgroup shapes, over a full
+    128-step chunk and a remainder, with 5 shots (parts of unequal size,
+    an empty one where 4 chains share 3 groups)."""
     lib = _lib.load()
     srcs = [(30, 6), (60, 9), (90, 4), (120, 12), (45, 20)]
     p = Problem(_lib.WAVE_PSV, 150, 90, 12, 10.0, 7e-4, 150, seed=17, kind=_lib.SRC_P, sources=srcs, rec_z=3)
@@ -206,19 +207,21 @@ def test_two_chains_bitwise_equal_one_chain():
     for t in (0, 1, 4, 14, 16):
         assert lib.fd23_tile_kind(t) == _lib.TILE_TILED
         out = []
-        for on in (False, True):
+        for n in (1, 2, 4):
             with p.context() as ctx:
                 ctx.tile = t
-                ctx.chains = on
+                ctx.chains = n
                 ctx.run(0, p.nt, 1)
-                assert ctx.chains == on
+                groups = -(-len(srcs) // ctx.tile_shape()[1])
+                assert ctx.chains == (n if groups >= n else 1)
                 ctx.sync()
                 out.append((b"".join(ctx.get_state(s) for s in range(len(srcs))),
                             [ctx.seismograms(s, c, p.nt) for s in range(len(srcs)) for c in range(2)]))
-        _print("chains tile", t, out[0][0] == out[1][0])
-        assert out[0][0] == out[1][0]
-        for a_, b_ in zip(out[0][1], out[1][1]):
-            assert np.array_equal(a_, b_)
+        for k in (1, 2):
+            _print("chains tile", t, (1, 2, 4)[k], out[0][0] == out[k][0])
+            assert out[0][0] == out[k][0]
+            for a_, b_ in zip(out[0][1], out[k][1]):
+                assert np.array_equal(a_, b_)
 
 
 def _fused_tiles():
