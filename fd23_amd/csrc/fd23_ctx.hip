@@ -2162,34 +2162,40 @@ extern "C" int fd23_autotune(fd23_ctx *c, int steps, double *best_ms) {
             if ((r = fd23_time_passes(c, steps, &a, &b)) == FD23_OK) tt = std::min(tt, a + b);
         if (r) break;
         timed.emplace_back(tt, t);
-        if (tt < bt) {
+        // a later shape wins by 1% at least, so that shapes within noise of
+        // each other do not alternate from one tuning to the next
+        if (tt < (bt < 1e30 ? 0.99 * bt : bt)) {
             bt = tt;
             best = t;
         }
     }
-    // Two chains, when the best shape can take them: the shapes within 5%
-    // of it (at most three) run 32 graph-replayed steps with one chain and
-    // with two, and the fastest pair wins, two chains by a 1% margin
+    // When the best shape is a two-pass one (and there is no box or window),
+    // the two-pass shapes within 5% of it run 32 graph-replayed steps, the
+    // way fd23_run steps, with each number of chains that applies; in table
+    // order, an option replaces the pick only when 1% faster, so shapes
+    // within noise of each other do not alternate from one tuning to the
+    // next.  (Fused and block shapes read event lists that tuning does not
+    // build: they are not timed this way.)
     c->chains = 1;
     if (c->cur != orig_set) swap_sets(c);
     c->tile = best;
-    if (r == FD23_OK && chains_possible(c)) {
-        std::sort(timed.begin(), timed.end());
+    auto two_pass = [](int t) { return !tile_fused(t) && !tile_block(t); };
+    if (r == FD23_OK && two_pass(best) && !c->box && !c->punch) {
+        std::vector<int> cands;
+        for (const auto &ct : timed)
+            if (ct.first <= 1.05 * bt && two_pass(ct.second)) cands.push_back(ct.second);
+        std::sort(cands.begin(), cands.end());
         double top = 1e30;
-        int top_t = best, top_ch = 1, n = 0;
-        for (const auto &ct : timed) {
-            if (n == 3 || ct.first > 1.05 * bt || r) break;
-            c->tile = ct.second;
-            if (!chains_possible(c)) continue;
-            n++;
+        int top_t = best, top_ch = 1;
+        for (int t : cands) {
+            c->tile = t;
             for (int ch : {1, 2, 4}) {
-                if (r != FD23_OK || !chains_possible(c, ch)) continue;
+                if (r != FD23_OK || (ch > 1 && !chains_possible(c, ch))) continue;
                 c->chains = ch;
                 double ms = 0;
-                const double margin = ch > 1 ? 1.01 : 1.0;
-                if ((r = time_chunk(c, 32, &ms)) == FD23_OK && ms * margin < top) {
-                    top = ms * margin;
-                    top_t = ct.second;
+                if ((r = time_chunk(c, 32, &ms)) == FD23_OK && (top >= 1e30 || ms < 0.99 * top)) {
+                    top = ms;
+                    top_t = t;
                     top_ch = ch;
                 }
             }
