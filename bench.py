@@ -94,13 +94,15 @@ def sync():
         pass
 
 
-def run_gpu(wl, steps, warmup, autotune=True, tile=None, passes=True):
+def run_gpu(wl, steps, warmup, autotune=True, tile=None, passes=True, chains=None):
     from fd23_amd import _lib
 
     ctx = wl.context(max_frames=warmup + steps)
     tune_ms = None
     if tile is not None:
         ctx.tile = tile
+        if chains is not None:
+            ctx.chains = chains
     elif autotune:
         _, tune_ms = ctx.autotune(16)
     tile = ctx.tile
@@ -495,6 +497,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tile", type=int, default=None, help="use this tile shape instead of tuning")
+    ap.add_argument("--chains", type=int, default=None,
+                    help="with --tile: independent chains of passes over parts of the shots (1, 2 or 4)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (shots and requests shard "
@@ -514,7 +518,8 @@ def main():
     from fd23_amd import _lib
 
     wl = make_workload(args.workload, args.storage).build()
-    out = measure_line(wl, args.steps, args.warmup, autotune=not args.no_autotune, tile=args.tile)
+    out = measure_line(wl, args.steps, args.warmup, autotune=not args.no_autotune, tile=args.tile,
+                       chains=args.chains)
     headline = args.workload == "c4" and args.storage in (None, "f32") and not args.quick
     if headline:
         # FP16 / BF16 storage of the same workload, timed the same way
@@ -541,12 +546,12 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def measure_line(wl, steps, warmup, autotune=True, tile=None):
+def measure_line(wl, steps, warmup, autotune=True, tile=None, chains=None):
     """One workload timed over `steps` graph-replayed steps after `warmup`
     untimed ones, with the dominant pass's roofline."""
     from fd23_amd import _lib
 
-    r = run_gpu(wl, steps, warmup, autotune=autotune, tile=tile)
+    r = run_gpu(wl, steps, warmup, autotune=autotune, tile=tile, chains=chains)
     cells = float(wl.nshots) * wl.nx * wl.nz * steps
     gcell = cells / r["elapsed"] / 1e9
     ms = r["elapsed"] / steps * 1e3
