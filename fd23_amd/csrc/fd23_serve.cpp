@@ -444,7 +444,8 @@ int fd23_server::quiet(Group &g, int shot) {
     int r;
     if ((r = fd23_reset_shot(g.ctx, shot))) return r;
     if ((r = fd23_set_source(g.ctx, shot, FD23_SRC_LINE, 0, 0, nullptr, 0, nullptr))) return r;
-    return fd23_set_shot_slot(g.ctx, shot, 0, 0);
+    // slots from 64 on cannot be idle: they step zeros, which stay zero
+    return fd23_set_shot_slot(g.ctx, shot, shot < 64 ? 0 : 1, 0);
 }
 
 // copy a finished member's seismograms out
