@@ -61,7 +61,7 @@ struct TileCfg {
 };
 // tile 0, the shape untuned contexts run (the serving engine, the driver),
 // is the one the autotuner picks on most workloads measured
-static const TileCfg kTiles[] = {
+static constexpr TileCfg kTiles[] = {
     {2, 1, 0, 0, 0},   {4, 1, 0, 0, 0},   {8, 1, 0, 0, 0},    {4, 2, 0, 0, 0},   {4, 4, 0, 0, 0},
     {4, 8, 0, 0, 0},   {8, 2, 0, 0, 0},   {8, 4, 0, 0, 0},    {2, 4, 0, 0, 0},   {2, 8, 0, 0, 0},
     {0, 1, 32, 4, 0},  {0, 1, 64, 4, 0},  {0, 1, 128, 4, 0},  {0, 1, 64, 2, 0},  {2, 1, 0, 0, 1},
@@ -107,10 +107,8 @@ struct fd23_ctx {
     fd23_desc d{};
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    // split passes: the edge tiles' kernel on a second stream beside the
-    // interior kernel (fork and join through two events)
-    hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // chain forks and joins (used under stream capture only)
+    hipEvent_t ev_fork = nullptr;
     // chains (fd23_set_chains, chosen by the autotuner): the launches of
     // chain i of n cover the i-th of n parts of the shot groups, on the
     // context stream (i = 0) or chain_stream[i-1]; chain -1: all groups
@@ -313,13 +311,11 @@ static void free_ctx(fd23_ctx *c) {
                       c->d_btaps, c->d_brecs};
     for (auto p : others) if (p) hipFree(p);
     if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
-    if (c->side) hipStreamDestroy(c->side);
     for (int i = 0; i < 3; i++) {
         if (c->chain_stream[i]) hipStreamDestroy(c->chain_stream[i]);
         if (c->chain_join[i]) hipEventDestroy(c->chain_join[i]);
     }
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
-    if (c->ev_join) hipEventDestroy(c->ev_join);
     delete c;
 }
 
@@ -373,9 +369,7 @@ extern "C" int fd23_ctx_create(const fd23_desc *desc, fd23_ctx **out) {
         }
         c->own_stream = true;
     }
-    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+    if (hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipStreamCreateWithFlags(&c->chain_stream[0], hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->chain_stream[1], hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->chain_stream[2], hipStreamNonBlocking) != hipSuccess ||
@@ -1095,23 +1089,6 @@ static Events make_events(const fd23_ctx *c, int k, int itrecord) {
     return e;
 }
 
-// fork: the stream of the edge kernel, after an event on the context
-// stream; join: the context stream waits for it.  Under stream capture the
-// two kernels become parallel nodes of the graph, and the edge tiles fill
-// the interior kernel's tail: +1.6% on the 64-shot 1024x512 batch, +1% on
-// the 8-shot 4096x2048 FP16 loop, against the two kernels in sequence.
-static hipStream_t fork_begin(fd23_ctx *c, bool fk) {
-    if (!fk) return c->stream;
-    hipEventRecord(c->ev_fork, c->stream);
-    hipStreamWaitEvent(c->side, c->ev_fork, 0);
-    return c->side;
-}
-static void fork_end(fd23_ctx *c, bool fk) {
-    if (!fk) return;
-    hipEventRecord(c->ev_join, c->side);
-    hipStreamWaitEvent(c->stream, c->ev_join, 0);
-}
-
 template <class S, int RZ, int SPB, bool SPLIT>
 static void launch_stress_t(fd23_ctx *c, const Events &ev, unsigned *amax) {
     using C = typename Store<S>::C;
@@ -1134,15 +1111,12 @@ static void launch_stress_t(fd23_ctx *c, const Events &ev, unsigned *amax) {
                 dim3 gi, ge;
                 const Work wi = make_work(c, RZ, SPB, gi, REGION_INTERIOR);
                 const Work we = make_work(c, RZ, SPB, ge, REGION_EDGE);
-                const bool fk = wi.nwork > 0 && we.nwork > 0 && c->chain < 0;
-                hipStream_t se = fork_begin(c, fk);
-                if (we.nwork > 0)
-                    hipLaunchKernelGGL((k_psv_stress<S, RZ, SPB, REGION_EDGE>), ge, block, 0, se, f, m, ps, pr, g,
-                                       we, ev, c->scl, amax);
                 if (wi.nwork > 0)
                     hipLaunchKernelGGL((k_psv_stress<S, RZ, SPB, REGION_INTERIOR>), gi, block, 0, c->stream, f, m, ps,
                                        pr, g, wi, ev, c->scl, amax);
-                fork_end(c, fk);
+                if (we.nwork > 0)
+                    hipLaunchKernelGGL((k_psv_stress<S, RZ, SPB, REGION_EDGE>), ge, block, 0, c->stream, f, m, ps,
+                                       pr, g, we, ev, c->scl, amax);
                 split_done = true;
             }
         }
@@ -1181,15 +1155,12 @@ static void launch_velocity_t(fd23_ctx *c, const Events &ev, unsigned *amax) {
                 dim3 gi, ge;
                 const Work wi = make_work(c, RZ, SPB, gi, REGION_INTERIOR);
                 const Work we = make_work(c, RZ, SPB, ge, REGION_EDGE);
-                const bool fk = wi.nwork > 0 && we.nwork > 0 && c->chain < 0;
-                hipStream_t se = fork_begin(c, fk);
-                if (we.nwork > 0)
-                    hipLaunchKernelGGL((k_psv_velocity<S, RZ, SPB, REGION_EDGE>), ge, block, 0, se, f, m, ps, pr, g,
-                                       we, ev, c->scl, amax);
                 if (wi.nwork > 0)
                     hipLaunchKernelGGL((k_psv_velocity<S, RZ, SPB, REGION_INTERIOR>), gi, block, 0, c->stream, f, m,
                                        ps, pr, g, wi, ev, c->scl, amax);
-                fork_end(c, fk);
+                if (we.nwork > 0)
+                    hipLaunchKernelGGL((k_psv_velocity<S, RZ, SPB, REGION_EDGE>), ge, block, 0, c->stream, f, m, ps,
+                                       pr, g, we, ev, c->scl, amax);
                 split_done = true;
             }
         }
@@ -2069,8 +2040,9 @@ extern "C" int fd23_time_passes(fd23_ctx *c, int nsteps, double *stress_ms, doub
 // on the context's own arrays and the faster run counts; the state is saved
 // first and restored afterwards so tuning leaves the simulation untouched.
 // milliseconds per step of n graph-replayed steps (no injection, no
-// recording) with the context's current shape and chain setting: the faster
-// of two replays after the capture; the graphs are dropped afterwards
+// recording) with the context's current shape and chain setting: the
+// fastest of three replays after the capture; the graphs are dropped
+// afterwards
 static int time_chunk(fd23_ctx *c, int n, double *ms) {
     fd23_ctx_invalidate_graphs(c);
     c->timing = true;
@@ -2079,7 +2051,7 @@ static int time_chunk(fd23_ctx *c, int n, double *ms) {
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) r = fail(FD23_EHIP, "event");
     if (r == FD23_OK) r = launch_chunk(c, 1, n); // captures and runs once
     double best = 1e30;
-    for (int rep = 0; rep < 2 && r == FD23_OK; rep++) {
+    for (int rep = 0; rep < 3 && r == FD23_OK; rep++) {
         hipEventRecord(a, c->stream);
         r = launch_chunk(c, 1, n);
         hipEventRecord(b, c->stream);
@@ -2162,28 +2134,31 @@ extern "C" int fd23_autotune(fd23_ctx *c, int steps, double *best_ms) {
             if ((r = fd23_time_passes(c, steps, &a, &b)) == FD23_OK) tt = std::min(tt, a + b);
         if (r) break;
         timed.emplace_back(tt, t);
-        // a later shape wins by 1% at least, so that shapes within noise of
-        // each other do not alternate from one tuning to the next
-        if (tt < (bt < 1e30 ? 0.99 * bt : bt)) {
+        if (tt < bt) {
             bt = tt;
             best = t;
         }
     }
-    // When the best shape is a two-pass one (and there is no box or window),
-    // the two-pass shapes within 5% of it run 32 graph-replayed steps, the
-    // way fd23_run steps, with each number of chains that applies; in table
-    // order, an option replaces the pick only when 1% faster, so shapes
-    // within noise of each other do not alternate from one tuning to the
-    // next.  (Fused and block shapes read event lists that tuning does not
-    // build: they are not timed this way.)
+    // When the best shape can run as chains (a two-pass shape with FP32 or
+    // FP64 storage, several shot groups, no box or window), the two-pass
+    // shapes within 10% of it that can run as chains run 64 graph-replayed
+    // steps, the way fd23_run steps, with one, two and four chains; in
+    // table order, an option replaces the pick only when 1% faster, so
+    // shapes within noise of each other do not alternate from one tuning to
+    // the next.  (Fused and block shapes read event lists that tuning does
+    // not build: they are not timed this way.)
     c->chains = 1;
     if (c->cur != orig_set) swap_sets(c);
     c->tile = best;
     auto two_pass = [](int t) { return !tile_fused(t) && !tile_block(t); };
-    if (r == FD23_OK && two_pass(best) && !c->box && !c->punch) {
+    if (r == FD23_OK && two_pass(best) && chains_possible(c)) {
         std::vector<int> cands;
-        for (const auto &ct : timed)
-            if (ct.first <= 1.05 * bt && two_pass(ct.second)) cands.push_back(ct.second);
+        for (const auto &ct : timed) {
+            if (ct.first > 1.10 * bt || !two_pass(ct.second)) continue;
+            c->tile = ct.second;
+            if (chains_possible(c)) cands.push_back(ct.second);
+        }
+        c->tile = best;
         std::sort(cands.begin(), cands.end());
         double top = 1e30;
         int top_t = best, top_ch = 1;
@@ -2193,7 +2168,7 @@ extern "C" int fd23_autotune(fd23_ctx *c, int steps, double *best_ms) {
                 if (r != FD23_OK || (ch > 1 && !chains_possible(c, ch))) continue;
                 c->chains = ch;
                 double ms = 0;
-                if ((r = time_chunk(c, 32, &ms)) == FD23_OK && (top >= 1e30 || ms < 0.99 * top)) {
+                if ((r = time_chunk(c, 64, &ms)) == FD23_OK && (top >= 1e30 || ms < 0.99 * top)) {
                     top = ms;
                     top_t = t;
                     top_ch = ch;
