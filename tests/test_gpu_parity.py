@@ -224,6 +224,35 @@ def test_chains_bitwise_equal_one_chain():
                 assert np.array_equal(a_, b_)
 
 
+def test_chains_apply_only_where_they_can():
+    """fd23_set_chains takes 0, 1, 2 or 4; a context runs one chain where
+    chains cannot apply: a fused shape, 16-bit storage (scales shared by
+    all shots), a single shot group."""
+    lib = _lib.load()
+    srcs = [(30, 6), (60, 9), (90, 4), (45, 20)]
+    p = Problem(_lib.WAVE_PSV, 128, 64, 8, 10.0, 7e-4, 20, seed=3, sources=srcs)
+    with p.context() as ctx:
+        for bad in (-1, 3, 5):
+            assert lib.fd23_set_chains(ctx.h, bad) == _lib.EINVAL
+        ctx.tile = 0
+        ctx.chains = 4
+        assert ctx.chains == 4
+        ctx.chains = 2
+        assert ctx.chains == 2
+        ctx.tile = _fused_tiles()[0]
+        assert ctx.chains == 1
+        ctx.tile = 5  # 8 shots per workgroup: one group of 4 shots
+        assert ctx.chains == 1
+    with p.context(storage=_lib.STORE_F16) as ctx:
+        ctx.tile = 0
+        ctx.chains = 2
+        assert ctx.chains == 1
+    q = Problem(_lib.WAVE_PSV, 128, 64, 8, 10.0, 7e-4, 20, seed=3, sources=srcs[:1])
+    with q.context() as ctx:
+        ctx.chains = 2
+        assert ctx.chains == 1
+
+
 def _fused_tiles():
     """Fused P-SV shapes: the strip-marching step (segments of 32/64/128 rows
     in blocks of 4, 64 rows in 2) and the tile-parallel step (segments of
